@@ -1,0 +1,81 @@
+// common.hpp -- shared host-side definitions of the HSDL engine.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hsdl.h"
+
+struct hsdl_graph {
+  uint32_t n = 0;
+  std::vector<uint64_t> off;  // n+1
+  std::vector<uint32_t> col;  // neighbour ids, sorted per row
+  std::vector<int32_t> w;     // weights >= 1
+  int32_t max_w = 0;
+};
+
+namespace hsdl {
+
+// Thread-local error detail for hsdl_last_error().
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+// Wall clock in ns (CLOCK_MONOTONIC).
+uint64_t now_ns();
+
+// ---------------------------------------------------------------- PRNG
+// splitmix64 (Steele, Lea, Flood 2014) seeds xoshiro256** (Blackman, Vigna
+// 2018); bounded integers use Lemire's nearly-divisionless method (2019).
+struct SplitMix64 {
+  uint64_t s;
+  explicit SplitMix64(uint64_t seed) : s(seed) {}
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+};
+
+struct Xoshiro256ss {
+  uint64_t s[4];
+  explicit Xoshiro256ss(uint64_t seed) {
+    SplitMix64 sm(seed);
+    for (auto &x : s) x = sm.next();
+  }
+  static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+  uint64_t next() {
+    const uint64_t r = rotl(s[1] * 5, 7) * 9;
+    const uint64_t t = s[1] << 17;
+    s[2] ^= s[0];
+    s[3] ^= s[1];
+    s[1] ^= s[2];
+    s[0] ^= s[3];
+    s[2] ^= t;
+    s[3] = rotl(s[3], 45);
+    return r;
+  }
+  // uniform in [0, range), range >= 1
+  uint32_t bounded(uint32_t range) {
+    uint64_t m = (uint64_t)(uint32_t)(next() >> 32) * range;
+    uint32_t l = (uint32_t)m;
+    if (l < range) {
+      uint32_t t = (uint32_t)(-range) % range;
+      while (l < t) {
+        m = (uint64_t)(uint32_t)(next() >> 32) * range;
+        l = (uint32_t)m;
+      }
+    }
+    return (uint32_t)(m >> 32);
+  }
+};
+
+// Graph construction helpers (graph.cpp).
+// Builds symmetric CSR from canonical unique edges (a<=b, sorted by (a,b)).
+void csr_from_unique_edges(uint32_t n, const std::vector<uint64_t> &keys,
+                           const std::vector<int32_t> &w, hsdl_graph &g);
+// (degree desc, id asc) order; fills new_to_old and rebuilds g in place.
+void reorder_by_degree(hsdl_graph &g, std::vector<uint32_t> &new_to_old);
+
+}  // namespace hsdl

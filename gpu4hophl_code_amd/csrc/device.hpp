@@ -1,0 +1,141 @@
+// device.hpp -- device-side types and helpers shared by the HSDL kernels.
+//
+// Label encoding in HBM (8 B, one u64 per label):
+//     bits 63..32  hub (u32 vertex id)
+//     bits 31..27  hop (0..31)
+//     bits 26..0   dist (0..2^27-1)
+// Sorting the u64 keys orders a vertex's labels by (hub, hop, dist), the
+// reference's per-vertex order (CPU_HopHL/include/HBPLL/
+// hop_constrained_two_hop_labels_generation.h:373-387).  The logical label of
+// the C ABI is (hub u32, hop u8, dist i32) = 9 B.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "common.hpp"
+
+namespace hsdl {
+
+constexpr uint32_t kDistBits = 27;
+constexpr uint32_t kMaxDist = (1u << kDistBits) - 1;
+constexpr uint32_t kMaxK = 15;
+constexpr uint32_t kInf32 = 0xFFFFFFFFu;
+constexpr int kWave = 64;
+
+__host__ __device__ inline uint64_t pack_label(uint32_t hub, uint32_t hop, uint32_t dist) {
+  return ((uint64_t)hub << 32) | ((uint64_t)hop << kDistBits) | (uint64_t)dist;
+}
+__host__ __device__ inline uint32_t label_hub(uint64_t l) { return (uint32_t)(l >> 32); }
+__host__ __device__ inline uint32_t label_hop(uint64_t l) { return ((uint32_t)l) >> kDistBits; }
+__host__ __device__ inline uint32_t label_dist(uint64_t l) { return ((uint32_t)l) & kMaxDist; }
+
+// HIP error to status code with detail.
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define HSDL_HIP(call)                                                  \
+  do {                                                                  \
+    hipError_t _e = (call);                                             \
+    if (_e != hipSuccess) return ::hsdl::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+// Selects `device` (or keeps the current one for -1); fails with HSDL_ENODEV
+// when no GPU is visible.
+int use_device(int32_t device, int *out_device);
+
+// Device buffer owned by RAII.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  int alloc(size_t n) {
+    release();
+    if (n == 0) n = 16;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return fail(HSDL_ENOMEM, "hipMalloc of " + std::to_string(n) + " bytes failed");
+    }
+    bytes = n;
+    return HSDL_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Wave-level helpers (wave64).
+__device__ inline int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t t = __shfl_xor(v, o, kWave);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Inclusive prefix sum across the wave.
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, kWave);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+}  // namespace hsdl
+
+// ---------------------------------------------------------------- index
+// A label index resident in HBM: per-vertex CSR of packed labels, sorted by
+// (hub, hop, dist) within each vertex.
+struct hsdl_index {
+  int device = 0;
+  uint32_t n = 0;
+  uint8_t K = 0;
+  uint64_t total = 0;
+  hsdl::DevBuf off;  // u64[n+1]
+  hsdl::DevBuf lab;  // u64[total]
+  bool cleaned = false;
+  hsdl_stats stats{};
+};
+
+namespace hsdl {
+
+// Internal entry points shared between translation units.
+// Sorts each vertex segment of `lab` (u64 keys) given u64 offsets; in place
+// through a temporary buffer.
+int segmented_sort_labels(uint64_t *d_lab, uint64_t total, const uint64_t *d_off, uint32_t n,
+                          hipStream_t st);
+// Exclusive scan of u32 counts into u64 offsets (n+1 entries, last = total).
+int scan_counts_u32(const uint32_t *d_cnt, uint64_t *d_off, uint32_t n, hipStream_t st);
+int scan_counts_u64(const uint64_t *d_cnt, uint64_t *d_off, uint32_t n, hipStream_t st);
+
+// Query launch used by hsdl_query_device and the server.
+int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,
+                 const int8_t *d_k, int32_t *d_out, hipStream_t st);
+
+}  // namespace hsdl

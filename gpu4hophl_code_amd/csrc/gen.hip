@@ -1,0 +1,707 @@
+// gen.hip -- K-round hop-bounded label generation on gfx950.
+//
+// Algorithm (the set the reference's GPU path and its CPU twin produce):
+// Hybrid_Generation_Clean/src/label_gen.cu:477-593 (frontier expansion,
+// min-dedupe, rank pruning y > s), :87-110 (accept iff candidate < query),
+// :113-150 (append), round driver :603-837; CPU twin
+// Hybrid_Generation_Clean/include/HBPLL/hop_constrained_two_hop_labels_generation.h:136-243.
+//
+//   L(v) = {(v,0,0)}, F_0(s) = {(s,0)}
+//   round h = 1..K, every hub s:
+//     C(y) = min over (x,dx) in F_{h-1}(s), (y,w) in N(x), y > s of dx + w
+//     accept (s,h,C(y)) into L(y) and F_h(s) iff
+//       C(y) < min{ da+db : (a,ha,da) in L(y), (a,hb,db) in L(s), ha+hb <= h }
+//
+// Schedule.  Hubs are processed in batches [s0, s0+B) in id order; a batch
+// runs all K rounds before the next starts.  This gives the reference's set:
+// in hub s's round-h query a common hub a < s pairs only labels with
+// ha, hb <= h-1 (hb = 0 would need a = s, ha = 0 would need a = y > s), so it
+// suffices that every hub a < s has finished round h-1, which the batch order
+// guarantees; labels of hubs > s never pair (L(s) holds hubs <= s only).
+// B = 1 is the plain one-hub-per-launch form.
+//
+// Per round, all on one stream, no host synchronisation:
+//   round_prep  -> zero per-hub candidate counters
+//   expand      -> wave-cooperative CSR scan of the frontier: each wave takes
+//                  64 frontier items, prefix-sums their degrees in registers
+//                  and walks the concatenated adjacency 64 edges at a time
+//                  (coalesced 8 B (col,w) loads); candidates are deduplicated
+//                  by atomicMin on a dense per-(hub,vertex) slot and the first
+//                  toucher appends the vertex to the hub's candidate list
+//   chunk_scan  -> per-hub candidate lists cut into work items of `tile`
+//   prune       -> persistent: a workgroup takes a work item, stages the hub's
+//                  labels L(s) in an LDS hash tile (hub -> prefix-min dist per
+//                  hop budget), then one wavefront per candidate streams L(y)
+//                  64 labels at a time and stops at the first witness (ballot)
+//   reserve     -> grows label segments that the round's accepted labels
+//                  overflow (bump allocation in a device pool; no locks)
+//   append      -> writes accepted labels; they also form the next frontier
+//   finalize    -> publishes segment lengths
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "device.hpp"
+
+namespace hsdl {
+
+struct Item {  // frontier entry / accepted label: hub slot, vertex, distance
+  uint32_t b;
+  uint32_t y;
+  uint32_t d;
+};
+
+struct Ctrl {
+  unsigned long long pool_top;
+  unsigned long long gtile_top;
+  unsigned long long cand_total;
+  unsigned long long item_total;
+  unsigned long long edge_total;
+  uint32_t fcnt[2];
+  uint32_t touched_cnt;
+  uint32_t work_ctr;
+  uint32_t total_chunks;
+  uint32_t overflow;  // bit 0: label pool, bit 1: tile arena
+  uint32_t maxlen;
+  uint32_t pad;
+};
+
+struct GenArgs {
+  uint32_t n;
+  uint32_t K;
+  uint32_t s0;
+  uint32_t B;
+  uint32_t h;
+  uint32_t cur;  // frontier buffer read this round
+  uint32_t tile; // candidates per prune work item
+  uint32_t lds_slots;  // LDS tile capacity (slots)
+  const unsigned long long *off;
+  const uint2 *adj;
+  unsigned long long *pool;
+  unsigned long long pool_cap;
+  unsigned long long *seg_start;
+  uint32_t *seg_len;
+  uint32_t *seg_cap;
+  uint32_t *add_cnt;
+  uint32_t *add_cur;
+  uint32_t *touched;
+  uint32_t *cand_d;
+  uint32_t *cand_list;
+  uint32_t *hub_ccnt;
+  uint32_t *chunk_off;
+  Item *F0;
+  Item *F1;
+  uint32_t *gtile;
+  unsigned long long gtile_cap;  // u32 words
+  Ctrl *ctrl;
+};
+
+__device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
+
+// ------------------------------------------------------------------ init
+__global__ void k_init_labels(GenArgs a, uint32_t c0) {
+  for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < a.n;
+       v += (uint64_t)gridDim.x * blockDim.x) {
+    a.seg_start[v] = v * c0;
+    a.seg_len[v] = 1;
+    a.seg_cap[v] = c0;
+    a.add_cnt[v] = 0;
+    a.add_cur[v] = 0;
+    a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t val) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = val;
+}
+
+__global__ void k_batch_begin(GenArgs a) {
+  Item *F = frontier(a, a.cur);
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x)
+    F[b] = Item{b, a.s0 + b, 0u};
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->fcnt[a.cur] = a.B;
+}
+
+__global__ void k_round_prep(GenArgs a) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x)
+    a.hub_ccnt[b] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctrl->fcnt[a.cur ^ 1] = 0;
+    a.ctrl->touched_cnt = 0;
+    a.ctrl->work_ctr = 0;
+    a.ctrl->gtile_top = 0;
+  }
+}
+
+// ---------------------------------------------------------------- expand
+__global__ void __launch_bounds__(256) k_expand(GenArgs a) {
+  const Item *F = frontier(a, a.cur);
+  const uint32_t fcnt = a.ctrl->fcnt[a.cur];
+  const int lane = lane_id();
+  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
+  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / kWave;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint64_t edges_seen = 0;
+  for (uint64_t base = wave * kWave; base < fcnt; base += nwaves * kWave) {
+    const uint64_t i = base + lane;
+    uint32_t ib = 0, ix = 0, id = 0;
+    unsigned long long beg = 0;
+    uint32_t deg = 0;
+    if (i < fcnt) {
+      Item it = F[i];
+      ib = it.b;
+      ix = it.y;
+      id = it.d;
+      beg = a.off[ix];
+      deg = (uint32_t)(a.off[ix + 1] - beg);
+    }
+    const uint32_t incl = wave_incl_scan_u32(deg);
+    const uint32_t excl = incl - deg;
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    edges_seen += total;
+    for (uint32_t jb = 0; jb < total; jb += kWave) {
+      const uint32_t j = jb + lane;
+      // largest k with excl_k <= j (items with deg 0 are skipped over)
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        uint32_t e = __shfl(excl, k + step, kWave);
+        if (e <= j) k += step;
+      }
+      const uint32_t kb = __shfl(ib, k, kWave);
+      const uint32_t kd = __shfl(id, k, kWave);
+      const uint32_t kex = __shfl(excl, k, kWave);
+      const unsigned long long kbeg = __shfl(beg, k, kWave);
+      bool first = false;
+      uint32_t y = 0;
+      if (j < total) {
+        const uint2 ed = a.adj[kbeg + (j - kex)];
+        y = ed.x;
+        if (y > a.s0 + kb) {
+          const uint32_t nd = kd + ed.y;
+          const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], nd);
+          first = (old == kInf32);
+        }
+      }
+      // append first touches to per-hub candidate lists, one atomic per hub
+      unsigned long long pend = __ballot(first);
+      while (pend) {
+        const int leader = __ffsll(pend) - 1;
+        const uint32_t bl = __shfl(kb, leader, kWave);
+        const unsigned long long grp = __ballot(first && kb == bl);
+        uint32_t basepos = 0;
+        if (lane == leader) basepos = atomicAdd(&a.hub_ccnt[bl], (uint32_t)__popcll(grp));
+        basepos = __shfl(basepos, leader, kWave);
+        if (first && kb == bl)
+          a.cand_list[(uint64_t)bl * a.n + basepos + __popcll(grp & lt_mask)] = y;
+        pend &= ~grp;
+      }
+    }
+  }
+  if (lane == 0 && edges_seen) atomicAdd(&a.ctrl->edge_total, (unsigned long long)edges_seen);
+}
+
+// ------------------------------------------------------------ chunk scan
+// One workgroup: chunk_off[b] = sum_{b'<b} ceil(hub_ccnt[b'] / tile).
+__global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
+  __shared__ uint32_t part[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (a.B + blockDim.x - 1) / blockDim.x;
+  const uint32_t lo = min(a.B, t * per), hi = min(a.B, lo + per);
+  uint32_t sum = 0;
+  unsigned long long cs = 0;
+  for (uint32_t b = lo; b < hi; ++b) {
+    sum += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
+    cs += a.hub_ccnt[b];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
+    uint32_t v = (t >= o) ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (uint32_t b = lo; b < hi; ++b) {
+    a.chunk_off[b] = run;
+    run += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
+  }
+  if (cs) atomicAdd(&a.ctrl->cand_total, cs);
+  if (t == blockDim.x - 1) {
+    a.chunk_off[a.B] = part[t];
+    a.ctrl->total_chunks = part[t];
+    a.ctrl->work_ctr = 0;
+  }
+  if (t == 0) atomicAdd(&a.ctrl->item_total, (unsigned long long)a.ctrl->fcnt[a.cur]);
+}
+
+// ----------------------------------------------------------------- prune
+__device__ inline uint32_t hash_slot(uint32_t key, uint32_t mask) {
+  return (key * 0x9E3779B1u) >> 7 & mask;
+}
+
+__device__ inline uint32_t tile_cap_for(uint32_t len) {
+  uint32_t want = len + (len >> 1) + 1;  // load factor <= 2/3
+  uint32_t c = 64;
+  while (c < want) c <<= 1;
+  return c;
+}
+
+// Build the hash tile of L(s): keys[c] = hub, vals[c*(K+1)+x] = min dist over
+// labels (hub, hb, db) of L(s) with hb <= x.
+__device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, uint32_t *vals,
+                           uint32_t cap) {
+  const uint32_t K1 = a.K + 1;
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) keys[i] = kInf32;
+  for (uint32_t i = threadIdx.x; i < cap * K1; i += blockDim.x) vals[i] = kInf32;
+  __syncthreads();
+  const unsigned long long st = a.seg_start[s];
+  const uint32_t len = a.seg_len[s];
+  const uint32_t mask = cap - 1;
+  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+    const unsigned long long l = a.pool[st + i];
+    const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
+    uint32_t slot = hash_slot(hub, mask);
+    for (;;) {
+      uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
+      if (prev == kInf32 || prev == hub) break;
+      slot = (slot + 1) & mask;
+    }
+    for (uint32_t x = hb; x < K1; ++x) atomicMin(&vals[slot * K1 + x], db);
+  }
+  __syncthreads();
+}
+
+// kGlobal: the tile lives in the global arena; its slots were filled by
+// atomics, so reads are agent-scope atomic loads (coherent in L2).
+template <bool kGlobal>
+__device__ inline uint32_t tile_ld(const uint32_t *p) {
+  if (kGlobal) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return *p;
+}
+
+template <bool kGlobal>
+__device__ inline uint32_t tile_lookup(const uint32_t *keys, const uint32_t *vals, uint32_t cap,
+                                       uint32_t K1, uint32_t hub, uint32_t x) {
+  const uint32_t mask = cap - 1;
+  uint32_t slot = hash_slot(hub, mask);
+  for (;;) {
+    const uint32_t k = tile_ld<kGlobal>(&keys[slot]);
+    if (k == hub) return tile_ld<kGlobal>(&vals[slot * K1 + x]);
+    if (k == kInf32) return kInf32;
+    slot = (slot + 1) & mask;
+  }
+}
+
+// Returns true when some common hub gives da + db <= dy (candidate pruned).
+template <bool kGlobal>
+__device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
+                               const uint32_t *vals, uint32_t cap) {
+  const int lane = lane_id();
+  const uint32_t K1 = a.K + 1;
+  const unsigned long long st = a.seg_start[y];
+  const uint32_t len = a.seg_len[y];
+  for (uint32_t j = 0; j < len; j += kWave) {
+    bool f = false;
+    if (j + lane < len) {
+      const unsigned long long l = a.pool[st + j + lane];
+      const uint32_t ha = label_hop(l);
+      if (ha <= a.h) {
+        const uint32_t v = tile_lookup<kGlobal>(keys, vals, cap, K1, label_hub(l), a.h - ha);
+        if (v != kInf32 && label_dist(l) + v <= dy) f = true;
+      }
+    }
+    if (__ballot(f)) return true;
+  }
+  return false;
+}
+
+__global__ void __launch_bounds__(256) k_prune(GenArgs a) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base;
+  __shared__ Item sh_acc[256];
+  const int lane = lane_id();
+  const uint32_t wv = threadIdx.x / kWave;
+  const uint32_t nwv = blockDim.x / kWave;
+  const uint32_t K1 = a.K + 1;
+  Item *Fn = frontier(a, a.cur ^ 1);
+  uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
+  uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * (a.gtile_cap / gridDim.x);
+  const uint64_t gwords = a.gtile_cap / gridDim.x;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      sh_w = atomicAdd(&a.ctrl->work_ctr, 1u);
+      sh_acc_n = 0;
+    }
+    __syncthreads();
+    const uint32_t w = sh_w;
+    if (w >= a.ctrl->total_chunks) break;
+    // hub slot b with chunk_off[b] <= w < chunk_off[b+1]
+    uint32_t lo = 0, hi = a.B;  // invariant chunk_off[lo] <= w < chunk_off[hi]
+    while (hi - lo > 1) {
+      uint32_t mid = (lo + hi) >> 1;
+      if (a.chunk_off[mid] <= w) lo = mid; else hi = mid;
+    }
+    const uint32_t b = lo;
+    const uint32_t s = a.s0 + b;
+    const uint32_t c_beg = (w - a.chunk_off[b]) * a.tile;
+    const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
+    const uint32_t cap = tile_cap_for(a.seg_len[s]);
+    uint32_t *keys, *vals;
+    const bool in_lds = cap <= a.lds_slots;
+    if (in_lds) {
+      keys = lds;
+      vals = lds + cap;
+      if (tile_b != b) {
+        build_tile(a, s, keys, vals, cap);
+        tile_b = b;
+      }
+    } else {
+      // per-workgroup tile in the global arena
+      if ((uint64_t)cap * (K1 + 1) > gwords) {
+        if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, 2u);
+        __syncthreads();
+        continue;  // the host reports HSDL_ENOMEM for this build
+      }
+      keys = gkeys;
+      vals = gkeys + cap;
+      build_tile(a, s, keys, vals, cap);
+      tile_b = kInf32;
+    }
+    for (uint32_t c = c_beg + wv; c < c_end; c += nwv) {
+      const uint32_t y = a.cand_list[(uint64_t)b * a.n + c];
+      const uint64_t slot = (uint64_t)b * a.n + y;
+      const uint32_t dy = a.cand_d[slot];
+      const bool pruned = in_lds ? witness_exists<false>(a, y, dy, keys, vals, cap)
+                                 : witness_exists<true>(a, y, dy, keys, vals, cap);
+      if (lane == 0) {
+        a.cand_d[slot] = kInf32;  // slot ready for the next round
+        if (!pruned) {
+          uint32_t p = atomicAdd(&sh_acc_n, 1u);
+          sh_acc[p] = Item{b, y, dy};
+          if (atomicAdd(&a.add_cnt[y], 1u) == 0) {
+            uint32_t t = atomicAdd(&a.ctrl->touched_cnt, 1u);
+            a.touched[t] = y;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && sh_acc_n) sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], sh_acc_n);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < sh_acc_n; i += blockDim.x) Fn[sh_acc_base + i] = sh_acc[i];
+    __syncthreads();
+  }
+}
+
+// -------------------------------------------------------- reserve / append
+__global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
+  const int lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  const uint32_t cnt = a.ctrl->touched_cnt;
+  for (uint32_t t = wave; t < cnt; t += nwaves) {
+    const uint32_t y = a.touched[t];
+    const uint32_t len = a.seg_len[y], cap = a.seg_cap[y];
+    const uint32_t need = len + a.add_cnt[y];
+    if (need <= cap) continue;
+    uint32_t ncap = max(need, cap * 2u);
+    ncap = (ncap + 7u) & ~7u;
+    unsigned long long ns = 0;
+    if (lane == 0) ns = atomicAdd(&a.ctrl->pool_top, (unsigned long long)ncap);
+    ns = __shfl(ns, 0, kWave);
+    if (ns + ncap > a.pool_cap) {
+      if (lane == 0) atomicOr(&a.ctrl->overflow, 1u);
+      continue;
+    }
+    const unsigned long long os = a.seg_start[y];
+    for (uint32_t i = lane; i < len; i += kWave) a.pool[ns + i] = a.pool[os + i];
+    if (lane == 0) {
+      a.seg_start[y] = ns;
+      a.seg_cap[y] = ncap;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_append(GenArgs a) {
+  const Item *Fn = frontier(a, a.cur ^ 1);
+  const uint32_t cnt = a.ctrl->fcnt[a.cur ^ 1];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const Item it = Fn[i];
+    const uint32_t p = a.seg_len[it.y] + atomicAdd(&a.add_cur[it.y], 1u);
+    if (p < a.seg_cap[it.y])
+      a.pool[a.seg_start[it.y] + p] = pack_label(a.s0 + it.b, a.h, it.d);
+    else
+      atomicOr(&a.ctrl->overflow, 1u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
+  const uint32_t cnt = a.ctrl->touched_cnt;
+  uint32_t mx = 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
+    const uint32_t y = a.touched[t];
+    const uint32_t nl = a.seg_len[y] + a.add_cnt[y];
+    a.seg_len[y] = nl;
+    a.add_cnt[y] = 0;
+    a.add_cur[y] = 0;
+    mx = max(mx, nl);
+  }
+  if (mx) atomicMax(&a.ctrl->maxlen, mx);
+}
+
+// --------------------------------------------------------------- export
+__global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long long *out_off,
+                                                unsigned long long *out) {
+  const int lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t v = wave; v < a.n; v += nwaves) {
+    const unsigned long long src = a.seg_start[v], dst = out_off[v];
+    const uint32_t len = a.seg_len[v];
+    for (uint32_t i = lane; i < len; i += kWave) out[dst + i] = a.pool[src + i];
+  }
+}
+
+// ----------------------------------------------------------------- host
+struct Gen {
+  GenArgs a{};
+  DevBuf off, adj, pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
+  DevBuf cand_d, cand_list, hub_ccnt, chunk_off, F0, F1, gtile, ctrl;
+  int grid = 0;
+  size_t lds_bytes = 0;
+  hipStream_t st = nullptr;
+};
+
+static int grid_for_device(int dev) {
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus * 4;
+}
+
+static int read_ctrl(Gen &g, Ctrl &c) {
+  HSDL_HIP(hipMemcpyAsync(&c, g.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, g.st));
+  HSDL_HIP(hipStreamSynchronize(g.st));
+  return HSDL_OK;
+}
+
+// Enqueues one batch (K rounds) of hubs [s0, s0+B).
+static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
+  GenArgs a = g.a;
+  a.s0 = s0;
+  a.B = B;
+  a.cur = 0;
+  const int small_grid = (int)std::min<uint32_t>((B + 255) / 256, 1024);
+  hipLaunchKernelGGL(k_batch_begin, dim3(small_grid), dim3(256), 0, g.st, a);
+  for (uint32_t h = 1; h <= K; ++h) {
+    a.h = h;
+    hipLaunchKernelGGL(k_round_prep, dim3(small_grid), dim3(256), 0, g.st, a);
+    hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
+    hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
+    hipLaunchKernelGGL(k_prune, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
+    hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
+    hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
+    hipLaunchKernelGGL(k_finalize, dim3(g.grid), dim3(256), 0, g.st, a);
+    a.cur ^= 1;
+  }
+  HSDL_HIP(hipGetLastError());
+  return HSDL_OK;
+}
+
+// Chooses the next hub batch from the last one's duration (the autotuner of
+// the batched form; the plain form pins B = 1).
+struct BatchTuner {
+  uint32_t B, Bmax;
+  double target_s;
+  uint32_t next(double last_s) {
+    double f = last_s > 0 ? target_s / last_s : 2.0;
+    f = std::min(2.0, std::max(0.5, f));
+    uint64_t nb = (uint64_t)std::llround(B * f);
+    nb = std::max<uint64_t>(1, std::min<uint64_t>(nb, Bmax));
+    B = (uint32_t)nb;
+    return B;
+  }
+};
+
+int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index **out) {
+  const uint32_t n = hg->n;
+  if (K < 1 || K > kMaxK) return fail(HSDL_EINVAL, "K must be in 1..15");
+  if (n == 0) return fail(HSDL_EINVAL, "empty graph");
+  if ((uint64_t)K * (uint64_t)std::max(hg->max_w, 1) > kMaxDist)
+    return fail(HSDL_ERANGE, "K * max edge weight exceeds the 27-bit label distance");
+  int dev = 0;
+  int rc = use_device(opts->device, &dev);
+  if (rc) return rc;
+  const uint64_t t_start = now_ns();
+  Gen g;
+  HSDL_HIP(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { if (s) (void)hipStreamDestroy(s); }
+  } sg{g.st};
+  g.grid = grid_for_device(dev);
+  const uint64_t nnz = hg->off[n];
+
+  size_t free_b = 0, total_b = 0;
+  HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
+  uint64_t budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
+  const uint64_t graph_bytes = (n + 1ull) * 8 + nnz * 8;
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 6);
+  if (budget < graph_bytes + vert_bytes + (64ull << 20))
+    return fail(HSDL_ENOMEM, "HBM budget below the graph and per-vertex state");
+  uint64_t rest = budget - graph_bytes - vert_bytes;
+  // label pool: 40% of the rest; batch buffers 45%; tile arena 15%
+  const uint32_t c0 = 16;
+  uint64_t pool_cap = std::max<uint64_t>((uint64_t)(rest * 0.40) / 8, (uint64_t)n * c0 * 2);
+  uint64_t batch_bytes = (uint64_t)(rest * 0.45);
+  const uint64_t per_hub = (uint64_t)n * (4 + 4 + 12 + 12);
+  uint32_t Bmax = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(batch_bytes / per_hub, n));
+  Bmax = std::min<uint32_t>(Bmax, 1u << 16);
+  const bool plain = opts->mode == HSDL_MODE_PLAIN;
+  if (plain) Bmax = 1;
+  if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
+  uint64_t gtile_words = std::max<uint64_t>((uint64_t)(rest * 0.15) / 4, (uint64_t)g.grid * 4096);
+
+  // device graph
+  if ((rc = g.off.alloc((n + 1ull) * 8)) || (rc = g.adj.alloc(std::max<uint64_t>(nnz, 1) * 8)))
+    return rc;
+  {
+    std::vector<uint2> adj(nnz);
+    for (uint64_t e = 0; e < nnz; ++e) adj[e] = make_uint2(hg->col[e], (uint32_t)hg->w[e]);
+    HSDL_HIP(hipMemcpy(g.off.p, hg->off.data(), (n + 1ull) * 8, hipMemcpyHostToDevice));
+    if (nnz) HSDL_HIP(hipMemcpy(g.adj.p, adj.data(), nnz * 8, hipMemcpyHostToDevice));
+  }
+  const uint64_t t_csr = now_ns();  // build time starts with the CSR resident
+  if ((rc = g.pool.alloc(pool_cap * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
+      (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
+      (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
+      (rc = g.touched.alloc(n * 4ull)) ||
+      (rc = g.cand_d.alloc((uint64_t)Bmax * n * 4)) ||
+      (rc = g.cand_list.alloc((uint64_t)Bmax * n * 4)) ||
+      (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
+      (rc = g.F0.alloc((uint64_t)Bmax * n * sizeof(Item))) ||
+      (rc = g.F1.alloc((uint64_t)Bmax * n * sizeof(Item))) ||
+      (rc = g.gtile.alloc(gtile_words * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))))
+    return rc;
+
+  // LDS tile: 56 KB per workgroup
+  const uint32_t K1 = K + 1;
+  uint32_t lds_slots = 64;
+  while ((lds_slots * 2) * (K1 + 1) * 4 <= 56 * 1024) lds_slots *= 2;
+  g.lds_bytes = (size_t)lds_slots * (K1 + 1) * 4;
+
+  GenArgs &a = g.a;
+  a.n = n;
+  a.K = K;
+  // accepted candidates of a work item are staged in 256 LDS slots
+  a.tile = opts->tile ? std::min<uint32_t>(opts->tile, 256u) : 256u;
+  a.lds_slots = lds_slots;
+  a.off = g.off.as<unsigned long long>();
+  a.adj = g.adj.as<uint2>();
+  a.pool = g.pool.as<unsigned long long>();
+  a.pool_cap = pool_cap;
+  a.seg_start = g.seg_start.as<unsigned long long>();
+  a.seg_len = g.seg_len.as<uint32_t>();
+  a.seg_cap = g.seg_cap.as<uint32_t>();
+  a.add_cnt = g.add_cnt.as<uint32_t>();
+  a.add_cur = g.add_cur.as<uint32_t>();
+  a.touched = g.touched.as<uint32_t>();
+  a.cand_d = g.cand_d.as<uint32_t>();
+  a.cand_list = g.cand_list.as<uint32_t>();
+  a.hub_ccnt = g.hub_ccnt.as<uint32_t>();
+  a.chunk_off = g.chunk_off.as<uint32_t>();
+  a.F0 = g.F0.as<Item>();
+  a.F1 = g.F1.as<Item>();
+  a.gtile = g.gtile.as<uint32_t>();
+  a.gtile_cap = gtile_words;
+  a.ctrl = g.ctrl.as<Ctrl>();
+
+  HSDL_HIP(hipMemsetAsync(g.ctrl.p, 0, sizeof(Ctrl), g.st));
+  {
+    Ctrl c0v{};
+    c0v.pool_top = (unsigned long long)n * c0;
+    HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &c0v, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
+  }
+  hipLaunchKernelGGL(k_init_labels, dim3(1024), dim3(256), 0, g.st, a, c0);
+  hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, a.cand_d,
+                     (uint64_t)Bmax * n, kInf32);
+  HSDL_HIP(hipGetLastError());
+
+  BatchTuner tuner{plain ? 1u : std::min<uint32_t>(Bmax, 64u), Bmax, 0.025};
+  if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
+  uint64_t batches = 0;
+  uint32_t s0 = 0;
+  Ctrl c{};
+  while (s0 < n) {
+    const uint32_t B = std::min<uint32_t>(tuner.B, n - s0);
+    const uint64_t tb = now_ns();
+    if ((rc = enqueue_batch(g, s0, B, K))) return rc;
+    if ((rc = read_ctrl(g, c))) return rc;
+    if (c.overflow & 1u) return fail(HSDL_ENOMEM, "label pool exhausted; raise hbm_budget");
+    if (c.overflow & 2u) return fail(HSDL_ENOMEM, "tile arena exhausted; raise hbm_budget");
+    s0 += B;
+    ++batches;
+    const double dt = (now_ns() - tb) * 1e-9;
+    if (!plain && !opts->hub_batch) tuner.next(dt);
+    if (opts->deadline_s > 0 && (now_ns() - t_start) * 1e-9 > opts->deadline_s)
+      return fail(HSDL_ETIMEDOUT, "generation passed its deadline");
+  }
+
+  // final CSR sorted per vertex
+  hsdl_index *idx = new hsdl_index();
+  idx->device = dev;
+  idx->n = n;
+  idx->K = K;
+  auto cleanup = [&](int code) {
+    delete idx;
+    return code;
+  };
+  if ((rc = idx->off.alloc((n + 1ull) * 8))) return cleanup(rc);
+  if ((rc = scan_counts_u32(a.seg_len, idx->off.as<uint64_t>(), n, g.st))) return cleanup(rc);
+  uint64_t total = 0;
+  if (hipMemcpy(&total, idx->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost) != hipSuccess)
+    return cleanup(fail(HSDL_EDEVICE, "reading label total failed"));
+  idx->total = total;
+  if ((rc = idx->lab.alloc(std::max<uint64_t>(total, 1) * 8))) return cleanup(rc);
+  hipLaunchKernelGGL(k_gather, dim3(g.grid), dim3(256), 0, g.st, a,
+                     idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>());
+  if (hipStreamSynchronize(g.st) != hipSuccess) return cleanup(fail(HSDL_EDEVICE, "gather failed"));
+  const uint64_t t_sort0 = now_ns();
+  if ((rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st)))
+    return cleanup(rc);
+  const uint64_t t_end = now_ns();
+  idx->stats.labels_uncleaned = total;
+  idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
+  idx->stats.t_sort_s = (t_end - t_sort0) * 1e-9;
+  idx->stats.hub_batch = tuner.B;
+  idx->stats.tile = a.tile;
+  idx->stats.rounds = K;
+  idx->stats.batches = batches;
+  idx->stats.candidates = c.cand_total;
+  idx->stats.frontier_items = c.item_total;
+  idx->stats.scanned_edges = c.edge_total;
+  *out = idx;
+  return HSDL_OK;
+}
+
+}  // namespace hsdl
+
+extern "C" int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
+                             hsdl_index **out) {
+  if (!g || !out) return hsdl::fail(HSDL_EINVAL, "null argument");
+  hsdl_opts o;
+  if (opts)
+    o = *opts;
+  else
+    hsdl_opts_init(&o);
+  return hsdl::generate(g, K, &o, out);
+}

@@ -1,0 +1,344 @@
+// server.hip -- continuous-batching (s,t,k) query server over a resident index.
+//
+// The reference answers one query per launch triple with a host sync in
+// between (Query_mdnd/src/gpu_query.cu:139-155).  Here requests go into a host
+// queue; a server thread closes a batch when it holds max_batch requests or
+// its oldest request has waited max_wait_us, and launches it into one of
+// `inflight` slots.  A slot owns pinned host staging, device buffers and its
+// own stream (default priority), so batches that arrive while others run are
+// copied and answered concurrently; a batch waits only for a free slot, and
+// the slot count is capped by the device-memory budget.  Answers are the same
+// kernel's as hsdl_query: a request's result does not depend on its batch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "device.hpp"
+
+namespace hsdl {
+
+struct Slot {
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  uint32_t *h_s = nullptr, *h_t = nullptr;
+  int8_t *h_k = nullptr;
+  int32_t *h_out = nullptr;
+  DevBuf d_s, d_t, d_k, d_out;
+  std::vector<uint64_t> ids, arrival;
+  size_t count = 0;
+  bool busy = false;
+};
+
+struct Result {
+  uint64_t id;
+  int32_t ans;
+  uint64_t lat;
+};
+
+}  // namespace hsdl
+
+struct hsdl_server {
+  const hsdl_index *idx = nullptr;
+  hsdl_server_opts o{};
+  std::vector<hsdl::Slot> slots;
+  // pending requests (ring of plain arrays guarded by mu)
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<uint32_t> q_s, q_t;
+  std::vector<int8_t> q_k;
+  std::vector<uint64_t> q_arr;
+  uint64_t q_head = 0, q_tail = 0;  // absolute positions; id == position
+  // finished results
+  std::mutex rmu;
+  std::deque<hsdl::Result> results;
+  std::atomic<uint64_t> completed{0}, batches{0}, batched_items{0};
+  std::atomic<bool> stop{false};
+  std::thread worker;
+  int device = 0;
+  int error = 0;
+};
+
+namespace hsdl {
+
+static void launch_slot(hsdl_server *srv, Slot &sl) {
+  const size_t n = sl.count;
+  (void)hipMemcpyAsync(sl.d_s.p, sl.h_s, n * 4, hipMemcpyHostToDevice, sl.st);
+  (void)hipMemcpyAsync(sl.d_t.p, sl.h_t, n * 4, hipMemcpyHostToDevice, sl.st);
+  (void)hipMemcpyAsync(sl.d_k.p, sl.h_k, n, hipMemcpyHostToDevice, sl.st);
+  if (launch_query(srv->idx, n, sl.d_s.as<uint32_t>(), sl.d_t.as<uint32_t>(), sl.d_k.as<int8_t>(),
+                   sl.d_out.as<int32_t>(), sl.st) != HSDL_OK)
+    srv->error = HSDL_EDEVICE;
+  (void)hipMemcpyAsync(sl.h_out, sl.d_out.p, n * 4, hipMemcpyDeviceToHost, sl.st);
+  (void)hipEventRecord(sl.done, sl.st);
+  sl.busy = true;
+}
+
+static void worker_loop(hsdl_server *srv) {
+  (void)hipSetDevice(srv->device);
+  const uint64_t cap = srv->o.queue_capacity;
+  const uint64_t wait_ns = (uint64_t)srv->o.max_wait_us * 1000ull;
+  std::vector<Result> done;
+  while (true) {
+    bool progressed = false;
+    // 1. retire finished slots
+    for (Slot &sl : srv->slots) {
+      if (!sl.busy) continue;
+      const hipError_t qe = hipEventQuery(sl.done);
+      if (qe == hipErrorNotReady) continue;
+      if (qe != hipSuccess) {
+        srv->error = HSDL_EDEVICE;
+        for (size_t i = 0; i < sl.count; ++i) sl.h_out[i] = INT32_MAX;
+      }
+      const uint64_t t = now_ns();
+      done.clear();
+      for (size_t i = 0; i < sl.count; ++i)
+        done.push_back(Result{sl.ids[i], sl.h_out[i], t > sl.arrival[i] ? t - sl.arrival[i] : 0});
+      {
+        std::lock_guard<std::mutex> g(srv->rmu);
+        srv->results.insert(srv->results.end(), done.begin(), done.end());
+      }
+      srv->completed += sl.count;
+      sl.busy = false;
+      sl.count = 0;
+      progressed = true;
+    }
+    // 2. admit a batch into a free slot
+    Slot *free_slot = nullptr;
+    for (Slot &sl : srv->slots)
+      if (!sl.busy) { free_slot = &sl; break; }
+    if (free_slot) {
+      std::unique_lock<std::mutex> g(srv->mu);
+      const uint64_t pending = srv->q_tail - srv->q_head;
+      bool close = false;
+      if (pending >= srv->o.max_batch) close = true;
+      else if (pending > 0) {
+        const uint64_t oldest = srv->q_arr[srv->q_head % cap];
+        const uint64_t now = now_ns();
+        if (now >= oldest + wait_ns || srv->stop.load()) close = true;
+      }
+      if (close) {
+        const size_t n = (size_t)std::min<uint64_t>(pending, srv->o.max_batch);
+        Slot &sl = *free_slot;
+        for (size_t i = 0; i < n; ++i) {
+          const uint64_t p = (srv->q_head + i) % cap;
+          sl.h_s[i] = srv->q_s[p];
+          sl.h_t[i] = srv->q_t[p];
+          sl.h_k[i] = srv->q_k[p];
+          sl.ids[i] = srv->q_head + i;
+          sl.arrival[i] = srv->q_arr[p];
+        }
+        srv->q_head += n;
+        g.unlock();
+        srv->cv.notify_all();
+        sl.count = n;
+        launch_slot(srv, sl);
+        srv->batches += 1;
+        srv->batched_items += n;
+        progressed = true;
+      }
+    }
+    if (srv->stop.load()) {
+      bool idle = true;
+      for (Slot &sl : srv->slots) idle = idle && !sl.busy;
+      std::lock_guard<std::mutex> g(srv->mu);
+      if (idle && srv->q_tail == srv->q_head) break;
+    }
+    if (!progressed) std::this_thread::yield();
+  }
+}
+
+}  // namespace hsdl
+
+using namespace hsdl;
+
+extern "C" {
+
+int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl_server **out) {
+  if (!idx || !out) return fail(HSDL_EINVAL, "null argument");
+  hsdl_server_opts o;
+  if (opts) o = *opts; else hsdl_server_opts_init(&o);
+  if (o.max_batch == 0 || o.inflight == 0 || o.queue_capacity == 0)
+    return fail(HSDL_EINVAL, "max_batch, inflight and queue_capacity must be positive");
+  int rc = use_device(idx->device, nullptr);
+  if (rc) return rc;
+  // admission by device memory: each slot holds 13 B per request
+  const uint64_t per_slot = (uint64_t)o.max_batch * 13 + 4096;
+  uint64_t budget = o.hbm_budget;
+  if (!budget) {
+    size_t fr = 0, tot = 0;
+    HSDL_HIP(hipMemGetInfo(&fr, &tot));
+    budget = fr / 4;
+  }
+  const uint64_t fit = budget / per_slot;
+  if (fit == 0) return fail(HSDL_ENOMEM, "device budget below one batch slot");
+  o.inflight = (uint32_t)std::min<uint64_t>(o.inflight, fit);
+  hsdl_server *srv = new hsdl_server();
+  srv->idx = idx;
+  srv->o = o;
+  srv->device = idx->device;
+  srv->q_s.resize(o.queue_capacity);
+  srv->q_t.resize(o.queue_capacity);
+  srv->q_k.resize(o.queue_capacity);
+  srv->q_arr.resize(o.queue_capacity);
+  srv->slots.resize(o.inflight);
+  for (Slot &sl : srv->slots) {
+    hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_s, o.max_batch * 4ull, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_t, o.max_batch * 4ull, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_k, o.max_batch * 1ull, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_out, o.max_batch * 4ull, hipHostMallocDefault);
+    if (e != hipSuccess) {
+      hsdl_server_destroy(srv);
+      return hip_fail(e, "server slot setup", __FILE__, __LINE__);
+    }
+    if ((rc = sl.d_s.alloc(o.max_batch * 4ull)) || (rc = sl.d_t.alloc(o.max_batch * 4ull)) ||
+        (rc = sl.d_k.alloc(o.max_batch * 1ull)) || (rc = sl.d_out.alloc(o.max_batch * 4ull))) {
+      hsdl_server_destroy(srv);
+      return rc;
+    }
+    sl.ids.resize(o.max_batch);
+    sl.arrival.resize(o.max_batch);
+  }
+  srv->worker = std::thread(worker_loop, srv);
+  *out = srv;
+  return HSDL_OK;
+}
+
+int hsdl_server_submit(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, const uint64_t *arrival_ns, uint64_t *first_id) {
+  if (!srv) return fail(HSDL_EINVAL, "null server");
+  if (n && (!s || !t || !k)) return fail(HSDL_EINVAL, "null request arrays");
+  for (size_t i = 0; i < n; ++i)
+    if (s[i] >= srv->idx->n || t[i] >= srv->idx->n) return fail(HSDL_EINVAL, "vertex out of range");
+  const uint64_t cap = srv->o.queue_capacity;
+  const uint64_t now = now_ns();
+  std::lock_guard<std::mutex> g(srv->mu);
+  if (srv->q_tail - srv->q_head + n > cap) return fail(HSDL_EBUSY, "request queue full");
+  if (first_id) *first_id = srv->q_tail;
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t p = (srv->q_tail + i) % cap;
+    srv->q_s[p] = s[i];
+    srv->q_t[p] = t[i];
+    srv->q_k[p] = k[i];
+    srv->q_arr[p] = arrival_ns ? arrival_ns[i] : now;
+  }
+  srv->q_tail += n;
+  return HSDL_OK;
+}
+
+int hsdl_server_poll(hsdl_server *srv, size_t cap, uint64_t *ids, int32_t *answers,
+                     uint64_t *latency_ns, size_t *n_out) {
+  if (!srv || !n_out) return fail(HSDL_EINVAL, "null argument");
+  std::lock_guard<std::mutex> g(srv->rmu);
+  size_t n = std::min(cap, srv->results.size());
+  for (size_t i = 0; i < n; ++i) {
+    const Result &r = srv->results[i];
+    if (ids) ids[i] = r.id;
+    if (answers) answers[i] = r.ans;
+    if (latency_ns) latency_ns[i] = r.lat;
+  }
+  srv->results.erase(srv->results.begin(), srv->results.begin() + n);
+  *n_out = n;
+  return srv->error ? fail(srv->error, "a batch launch failed") : HSDL_OK;
+}
+
+int hsdl_server_drain(hsdl_server *srv) {
+  if (!srv) return fail(HSDL_EINVAL, "null server");
+  uint64_t target;
+  {
+    std::lock_guard<std::mutex> g(srv->mu);
+    target = srv->q_tail;
+  }
+  while (srv->completed.load() < target) {
+    if (srv->error) return fail(srv->error, "a batch launch failed");
+    std::this_thread::yield();
+  }
+  return HSDL_OK;
+}
+
+int hsdl_server_stats_get(const hsdl_server *srv, hsdl_server_stats *st) {
+  if (!srv || !st) return fail(HSDL_EINVAL, "null argument");
+  st->submitted = srv->q_tail;
+  st->completed = srv->completed.load();
+  st->batches = srv->batches.load();
+  st->mean_batch = st->batches ? (double)srv->batched_items.load() / st->batches : 0.0;
+  return HSDL_OK;
+}
+
+void hsdl_server_destroy(hsdl_server *srv) {
+  if (!srv) return;
+  srv->stop = true;
+  if (srv->worker.joinable()) srv->worker.join();
+  for (Slot &sl : srv->slots) {
+    if (sl.st) (void)hipStreamSynchronize(sl.st);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.st) (void)hipStreamDestroy(sl.st);
+    if (sl.h_s) (void)hipHostFree(sl.h_s);
+    if (sl.h_t) (void)hipHostFree(sl.h_t);
+    if (sl.h_k) (void)hipHostFree(sl.h_k);
+    if (sl.h_out) (void)hipHostFree(sl.h_out);
+  }
+  delete srv;
+}
+
+int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, const uint64_t *arrival_offset_ns, int32_t *answers,
+                       uint64_t *latency_ns) {
+  if (!srv || (n && (!s || !t || !k || !arrival_offset_ns))) return fail(HSDL_EINVAL, "null argument");
+  for (size_t i = 1; i < n; ++i)
+    if (arrival_offset_ns[i] < arrival_offset_ns[i - 1])
+      return fail(HSDL_EINVAL, "arrival offsets must be non-decreasing");
+  uint64_t base_id = 0;
+  {
+    std::lock_guard<std::mutex> g(srv->mu);
+    if (srv->q_tail != srv->q_head) return fail(HSDL_ESTATE, "server has pending requests");
+    base_id = srv->q_tail;
+  }
+  {
+    std::lock_guard<std::mutex> g(srv->rmu);
+    srv->results.clear();
+  }
+  const uint64_t start = now_ns();
+  std::vector<uint64_t> arr;
+  size_t sent = 0, got = 0;
+  std::vector<Result> batch;
+  while (got < n) {
+    const uint64_t now = now_ns();
+    size_t upto = sent;
+    while (upto < n && start + arrival_offset_ns[upto] <= now) ++upto;
+    if (upto > sent) {
+      arr.resize(upto - sent);
+      for (size_t i = sent; i < upto; ++i) arr[i - sent] = start + arrival_offset_ns[i];
+      int rc = hsdl_server_submit(srv, upto - sent, s + sent, t + sent, k + sent, arr.data(), nullptr);
+      if (rc == HSDL_OK) sent = upto;
+      else if (rc != HSDL_EBUSY) return rc;
+    }
+    {
+      std::lock_guard<std::mutex> g(srv->rmu);
+      batch.assign(srv->results.begin(), srv->results.end());
+      srv->results.clear();
+    }
+    for (const Result &r : batch) {
+      const uint64_t i = r.id - base_id;
+      if (i < n) {
+        if (answers) answers[i] = r.ans;
+        if (latency_ns) latency_ns[i] = r.lat;
+        ++got;
+      }
+    }
+    if (srv->error) return fail(srv->error, "a batch launch failed");
+    if (batch.empty() && upto == sent) std::this_thread::yield();
+  }
+  return HSDL_OK;
+}
+
+}  // extern "C"

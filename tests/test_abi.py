@@ -1,0 +1,160 @@
+"""C ABI and host-side graph logic (CPU only: no compute call needs a GPU)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import gpu4hophl_code_amd as H
+import oracle_lib as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "hsdl.h")).read()
+    return sorted(set(re.findall(r"\b(hsdl_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = H.lib()
+    decl = declared_symbols()
+    assert len(decl) >= 30
+    for name in decl:
+        assert hasattr(L, name), name
+    assert sorted(H.EXPORTED) == decl
+
+
+def test_strerror_codes():
+    L = H.lib()
+    assert L.hsdl_strerror(0) == b"ok"
+    for code in range(-1, -10, -1):
+        assert L.hsdl_strerror(code) != b"unknown error"
+
+
+def csr_from(g):
+    off, col, w = g.csr()
+    return off.astype(np.int64), col.astype(np.int32), w
+
+
+@pytest.mark.parametrize("name", ["simple_iterative_tests.txt", "simple_iterative_tests_2023www.txt",
+                                  "euroroad2.txt"])
+def test_txt_loader_matches_reference_rules(golden_dir, name):
+    p = os.path.join(golden_dir, name)
+    g = H.Graph.load_txt(p)
+    og = O.load_txt(p)
+    off, col, w = csr_from(g)
+    assert np.array_equal(off, og.off) and np.array_equal(col, og.col) and np.array_equal(w, og.w)
+    # degree renaming matches the oracle's restatement
+    n2o = g.reorder_by_degree()
+    og2, on2o = O.reorder(og)
+    off, col, w = csr_from(g)
+    assert np.array_equal(n2o.astype(np.int32), on2o)
+    assert np.array_equal(off, og2.off) and np.array_equal(col, og2.col) and np.array_equal(w, og2.w)
+
+
+def test_from_edges_last_wins_and_self_loops(tmp_path):
+    rng = np.random.default_rng(3)
+    n, m = 50, 400
+    u = rng.integers(0, n, m)
+    v = rng.integers(0, n, m)
+    w = rng.integers(1, 20, m)
+    p = tmp_path / "multi.txt"
+    with open(p, "w") as f:
+        f.write(f"|V|= {n}\n|E|= {m}\n\n")
+        for a, b, c in zip(u, v, w):
+            f.write(f"Edge {a} {b} {c}\n")
+    og = O.load_txt(str(p))
+    g = H.Graph.from_edges(n, u, v, w)
+    off, col, ww = csr_from(g)
+    assert np.array_equal(off, og.off) and np.array_equal(col, og.col) and np.array_equal(ww, og.w)
+    g2 = H.Graph.load_txt(str(p))
+    assert all(np.array_equal(a, b) for a, b in zip(csr_from(g2), (og.off, og.col, og.w)))
+    # flags: drop self loops, keep minimum duplicate weight
+    g3 = H.Graph.from_edges(n, u, v, w, H.DROP_SELF_LOOPS | H.DEDUP_MIN)
+    off3, col3, w3 = csr_from(g3)
+    best = {}
+    for a, b, c in zip(u, v, w):
+        if a == b:
+            continue
+        key = (min(a, b), max(a, b))
+        best[key] = min(best.get(key, 1 << 30), c)
+    got = {}
+    for x in range(n):
+        for e in range(off3[x], off3[x + 1]):
+            assert col3[e] != x
+            got[(min(x, col3[e]), max(x, col3[e]))] = w3[e]
+    assert got == best
+
+
+def test_save_and_reload_txt(tmp_path, golden_dir):
+    g = H.Graph.load_txt(os.path.join(golden_dir, "euroroad2.txt"))
+    p = tmp_path / "round.txt"
+    g.save_txt(str(p))
+    g2 = H.Graph.load_txt(str(p))
+    for a, b in zip(csr_from(g), csr_from(g2)):
+        assert np.array_equal(a, b)
+
+
+def test_edgelist_loader(tmp_path):
+    p = tmp_path / "x.e"
+    p.write_text("Endpoint1 Endpoint2 Random-Weight (vertex ID from 0 to |V|-1; |V|= 6; |E|= 4)\n"
+                 "0 1 35\n1 2 48\n4 5 7\n1 0 9\n")
+    g = H.Graph.load_edgelist(str(p))
+    off, col, w = csr_from(g)
+    assert len(off) == 7
+    assert list(zip(col[off[0]:off[1]], w[off[0]:off[1]])) == [(1, 9)]
+    assert list(zip(col[off[1]:off[2]], w[off[1]:off[2]])) == [(0, 9), (2, 48)]
+    assert off[4] - off[3] == 0  # vertex 3 isolated
+
+
+def check_degree_order(off, col, w):
+    n = len(off) - 1
+    deg = np.diff(off)
+    assert np.all(deg[:-1] >= deg[1:])
+    for v in range(0, n, max(1, n // 200)):
+        nb = col[off[v]:off[v + 1]]
+        assert np.all(np.diff(nb.astype(np.int64)) > 0)  # sorted, no duplicates
+        assert not np.any(nb == v)
+
+
+def test_rmat_generator_matches_committed_fixture(golden_dir):
+    z = np.load(os.path.join(golden_dir, "c1_rmat12_k4.npz"))
+    g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    off, col, w = csr_from(g)
+    assert np.array_equal(off, z["off"]) and np.array_equal(col, z["col"]) and np.array_equal(w, z["w"])
+    check_degree_order(off, col, w)
+    assert w.min() >= 1 and w.max() <= 100
+    assert len(col) // 2 <= 8 * 4096
+    # symmetric
+    pairs = set()
+    for v in range(len(off) - 1):
+        for e in range(off[v], off[v + 1]):
+            pairs.add((v, int(col[e]), int(w[e])))
+    assert all((b, a, c) in pairs for a, b, c in pairs)
+
+
+def test_grid_generator():
+    g = H.Graph.grid(16, 20, 7, 1, 1000)
+    off, col, w = csr_from(g)
+    assert len(off) - 1 == 320
+    assert len(col) == 2 * (2 * 16 * 20 - 16 - 20)
+    check_degree_order(off, col, w)
+    deg = np.diff(off)
+    assert (deg == 4).sum() == 14 * 18 and (deg == 2).sum() == 4
+
+
+def test_generation_needs_gpu_no_cpu_fallback():
+    import ctypes
+    L = H.lib()
+    n = ctypes.c_int()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present")
+    except ImportError:
+        pass
+    g = H.Graph.rmat(8, 4, 1)
+    with pytest.raises(H.HSDLError) as e:
+        H.hop_constrained_two_hop_labels_generation(g, 3)
+    assert e.value.code == -8
