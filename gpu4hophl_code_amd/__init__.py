@@ -51,7 +51,8 @@ class hsdl_opts(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_uint32), ("hub_batch", ctypes.c_uint32),
                 ("tile", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
                 ("deadline_s", ctypes.c_double), ("device", ctypes.c_int32),
-                ("reserved", ctypes.c_uint32 * 8)]
+                ("pool_init_labels", ctypes.c_uint64), ("arena_init_words", ctypes.c_uint64),
+                ("kernel_timing", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class hsdl_stats(ctypes.Structure):
@@ -61,7 +62,9 @@ class hsdl_stats(ctypes.Structure):
                 ("tile", ctypes.c_uint32), ("rounds", ctypes.c_uint32),
                 ("clean_passes", ctypes.c_uint32), ("batches", ctypes.c_uint64),
                 ("candidates", ctypes.c_uint64), ("frontier_items", ctypes.c_uint64),
-                ("scanned_edges", ctypes.c_uint64)]
+                ("scanned_edges", ctypes.c_uint64), ("t_expand_s", ctypes.c_double),
+                ("t_prune_s", ctypes.c_double), ("prune_launches", ctypes.c_uint64),
+                ("prune_label_reads", ctypes.c_uint64), ("prune_tile_reads", ctypes.c_uint64)]
 
 
 class hsdl_server_opts(ctypes.Structure):
@@ -84,6 +87,7 @@ _SIGNATURES = [
     ("hsdl_graph_reorder_by_degree", ctypes.c_int, [_vp, _vp]),
     ("hsdl_graph_rmat", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_vp)]),
     ("hsdl_graph_grid", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_vp)]),
+    ("hsdl_graph_upload", ctypes.c_int, [_vp, ctypes.c_int32]),
     ("hsdl_graph_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
     ("hsdl_graph_csr", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("hsdl_free_graph", None, [_vp]),
@@ -194,6 +198,10 @@ class Graph:
         h = _vp()
         _check(lib().hsdl_graph_grid(rows, cols, seed, wmin, wmax, ctypes.byref(h)))
         return Graph(h)
+
+    def upload(self, device=-1):
+        """Keeps a device copy of the CSR for later builds."""
+        _check(lib().hsdl_graph_upload(self.h, device))
 
     def save_txt(self, path):
         _check(lib().hsdl_graph_save_txt(self.h, os.fsencode(path)))
@@ -306,9 +314,13 @@ class Index:
                                        _vp(stream)))
 
 
-def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, device=-1):
+def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, device=-1,
+          pool_init_labels=0, arena_init_words=0, kernel_timing=False):
     o = hsdl_opts()
     lib().hsdl_opts_init(ctypes.byref(o))
+    o.kernel_timing = 1 if kernel_timing else 0
+    o.pool_init_labels = pool_init_labels
+    o.arena_init_words = arena_init_words
     o.mode = mode
     o.hub_batch = hub_batch
     o.tile = tile
@@ -320,10 +332,12 @@ def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, 
 
 def hop_constrained_two_hop_labels_generation(graph, K, mode=MODE_BATCHED, hub_batch=0, tile=0,
                                               hbm_budget=0, deadline_s=0.0, device=-1,
-                                              clean=False):
+                                              clean=False, pool_init_labels=0,
+                                              arena_init_words=0, kernel_timing=False):
     """Generates the K-hop label index of `graph` on the GPU (uncleaned unless
     clean=True, which also applies hop_constrained_clean_L)."""
-    o = _opts(mode, hub_batch, tile, hbm_budget, deadline_s, device)
+    o = _opts(mode, hub_batch, tile, hbm_budget, deadline_s, device, pool_init_labels,
+              arena_init_words, kernel_timing)
     h = _vp()
     _check(lib().hsdl_generate(graph.h, K, ctypes.byref(o), ctypes.byref(h)))
     idx = Index(h)

@@ -13,6 +13,18 @@ struct hsdl_graph {
   std::vector<uint32_t> col;  // neighbour ids, sorted per row
   std::vector<int32_t> w;     // weights >= 1
   int32_t max_w = 0;
+  // device copy made by hsdl_graph_upload (off u64[n+1], adj (col,w) u32x2)
+  void *d_off = nullptr;
+  void *d_adj = nullptr;
+  int d_device = -1;
+  void (*d_free)(hsdl_graph *) = nullptr;
+  ~hsdl_graph() {
+    if (d_free) d_free(this);
+  }
+  void drop_device_copy() {
+    if (d_free) d_free(this);
+    d_free = nullptr;
+  }
 };
 
 namespace hsdl {

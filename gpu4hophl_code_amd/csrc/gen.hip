@@ -68,6 +68,8 @@ struct Ctrl {
   uint32_t overflow;  // bit 0: label pool, bit 1: tile arena
   uint32_t maxlen;
   uint32_t pad;
+  unsigned long long label_reads;  // labels of L(y) streamed by the prune
+  unsigned long long tile_reads;   // labels of L(s) staged into tiles
 };
 
 struct GenArgs {
@@ -301,9 +303,10 @@ __device__ inline uint32_t tile_lookup(const uint32_t *keys, const uint32_t *val
 }
 
 // Returns true when some common hub gives da + db <= dy (candidate pruned).
+// `reads` receives the number of labels of L(y) loaded.
 template <bool kGlobal>
 __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
-                               const uint32_t *vals, uint32_t cap) {
+                               const uint32_t *vals, uint32_t cap, uint32_t &reads) {
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
   const unsigned long long st = a.seg_start[y];
@@ -318,8 +321,12 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
         if (v != kInf32 && label_dist(l) + v <= dy) f = true;
       }
     }
-    if (__ballot(f)) return true;
+    if (__ballot(f)) {
+      reads = min(len, j + kWave);
+      return true;
+    }
   }
+  reads = len;
   return false;
 }
 
@@ -333,6 +340,7 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
   const uint32_t K1 = a.K + 1;
   Item *Fn = frontier(a, a.cur ^ 1);
   uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
+  unsigned long long wave_reads = 0, tile_reads = 0;
   uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * (a.gtile_cap / gridDim.x);
   const uint64_t gwords = a.gtile_cap / gridDim.x;
   for (;;) {
@@ -362,25 +370,35 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
       if (tile_b != b) {
         build_tile(a, s, keys, vals, cap);
         tile_b = b;
+        tile_reads += a.seg_len[s];
       }
     } else {
       // per-workgroup tile in the global arena
       if ((uint64_t)cap * (K1 + 1) > gwords) {
+        // arena too small: the host rolls the batch back, grows the arena
+        // and runs the batch again; reset the slots so they stay clean
         if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, 2u);
+        for (uint32_t c = c_beg + threadIdx.x; c < c_end; c += blockDim.x) {
+          const uint32_t y = a.cand_list[(uint64_t)b * a.n + c];
+          a.cand_d[(uint64_t)b * a.n + y] = kInf32;
+        }
         __syncthreads();
-        continue;  // the host reports HSDL_ENOMEM for this build
+        continue;
       }
       keys = gkeys;
       vals = gkeys + cap;
       build_tile(a, s, keys, vals, cap);
       tile_b = kInf32;
+      tile_reads += a.seg_len[s];
     }
     for (uint32_t c = c_beg + wv; c < c_end; c += nwv) {
       const uint32_t y = a.cand_list[(uint64_t)b * a.n + c];
       const uint64_t slot = (uint64_t)b * a.n + y;
       const uint32_t dy = a.cand_d[slot];
-      const bool pruned = in_lds ? witness_exists<false>(a, y, dy, keys, vals, cap)
-                                 : witness_exists<true>(a, y, dy, keys, vals, cap);
+      uint32_t reads = 0;
+      const bool pruned = in_lds ? witness_exists<false>(a, y, dy, keys, vals, cap, reads)
+                                 : witness_exists<true>(a, y, dy, keys, vals, cap, reads);
+      wave_reads += reads;
       if (lane == 0) {
         a.cand_d[slot] = kInf32;  // slot ready for the next round
         if (!pruned) {
@@ -399,6 +417,8 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
     for (uint32_t i = threadIdx.x; i < sh_acc_n; i += blockDim.x) Fn[sh_acc_base + i] = sh_acc[i];
     __syncthreads();
   }
+  if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
+  if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
 }
 
 // -------------------------------------------------------- reserve / append
@@ -448,13 +468,33 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
   uint32_t mx = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
     const uint32_t y = a.touched[t];
-    const uint32_t nl = a.seg_len[y] + a.add_cnt[y];
+    // capped: when the pool ran out some appends were dropped (the host then
+    // rolls the batch back)
+    const uint32_t nl = min(a.seg_len[y] + a.add_cnt[y], a.seg_cap[y]);
     a.seg_len[y] = nl;
     a.add_cnt[y] = 0;
     a.add_cur[y] = 0;
     mx = max(mx, nl);
   }
   if (mx) atomicMax(&a.ctrl->maxlen, mx);
+}
+
+// Removes the labels of hubs [s0, s0+B) (the batch being rolled back): they
+// sit at the tail of every segment, after all labels of earlier batches; the
+// self label at position 0 stays.
+__global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+    uint32_t len = a.seg_len[v];
+    const unsigned long long st = a.seg_start[v];
+    while (len > 1) {
+      const uint32_t hub = label_hub(a.pool[st + len - 1]);
+      if (hub < a.s0 || hub - a.s0 >= a.B) break;
+      --len;
+    }
+    a.seg_len[v] = len;
+    a.add_cnt[v] = 0;
+    a.add_cur[v] = 0;
+  }
 }
 
 // --------------------------------------------------------------- export
@@ -471,13 +511,64 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
 }
 
 // ----------------------------------------------------------------- host
+static void free_device_graph(hsdl_graph *g) {
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  if (g->d_device >= 0 && cur != g->d_device) (void)hipSetDevice(g->d_device);
+  if (g->d_off) (void)hipFree(g->d_off);
+  if (g->d_adj) (void)hipFree(g->d_adj);
+  if (cur >= 0 && cur != g->d_device) (void)hipSetDevice(cur);
+  g->d_off = g->d_adj = nullptr;
+  g->d_device = -1;
+}
+
+static int upload_graph(hsdl_graph *g, int dev) {
+  if (g->d_off && g->d_device == dev) return HSDL_OK;
+  g->drop_device_copy();
+  const uint32_t n = g->n;
+  const uint64_t nnz = g->off[n];
+  std::vector<uint2> adj(nnz);
+  for (uint64_t e = 0; e < nnz; ++e) adj[e] = make_uint2(g->col[e], (uint32_t)g->w[e]);
+  void *doff = nullptr, *dadj = nullptr;
+  hipError_t e = hipMalloc(&doff, (n + 1ull) * 8);
+  if (e == hipSuccess) e = hipMalloc(&dadj, std::max<uint64_t>(nnz, 1) * 8);
+  if (e == hipSuccess) e = hipMemcpy(doff, g->off.data(), (n + 1ull) * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(dadj, adj.data(), nnz * 8, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (doff) (void)hipFree(doff);
+    if (dadj) (void)hipFree(dadj);
+    return hip_fail(e, "graph upload", __FILE__, __LINE__);
+  }
+  g->d_off = doff;
+  g->d_adj = dadj;
+  g->d_device = dev;
+  g->d_free = free_device_graph;
+  return HSDL_OK;
+}
+
 struct Gen {
   GenArgs a{};
-  DevBuf off, adj, pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
+  DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
   DevBuf cand_d, cand_list, hub_ccnt, chunk_off, F0, F1, gtile, ctrl;
+  uint32_t Bcap = 0;       // hub slots the batch buffers hold
+  uint64_t budget = 0;     // device bytes the build may hold
+  uint64_t fixed = 0;      // bytes outside pool / batch buffers / arena
   int grid = 0;
   size_t lds_bytes = 0;
   hipStream_t st = nullptr;
+  bool verbose = false;
+  // kernel timing (events around every expand and prune launch of a batch)
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  double t_expand = 0, t_prune = 0;
+  uint64_t n_prune = 0;
+  ~Gen() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  uint64_t held() const {
+    return fixed + pool.bytes + cand_d.bytes + cand_list.bytes + F0.bytes + F1.bytes +
+           gtile.bytes;
+  }
 };
 
 static int grid_for_device(int dev) {
@@ -494,6 +585,60 @@ static int read_ctrl(Gen &g, Ctrl &c) {
   return HSDL_OK;
 }
 
+// Batch buffers for B hub slots: dense candidate slots (kept at "no
+// candidate" between batches), candidate lists and two frontier buffers.
+static uint64_t batch_bytes(uint64_t B, uint32_t n) { return B * n * (4ull + 4 + 2 * sizeof(Item)); }
+
+static int ensure_batch_capacity(Gen &g, uint32_t B) {
+  if (B <= g.Bcap) return HSDL_OK;
+  const uint32_t n = g.a.n;
+  g.cand_d.release();
+  g.cand_list.release();
+  g.F0.release();
+  g.F1.release();
+  if (g.held() + batch_bytes(B, n) > g.budget)
+    return fail(HSDL_ENOMEM, "batch buffers exceed the HBM budget");
+  int rc;
+  if ((rc = g.cand_d.alloc((uint64_t)B * n * 4)) || (rc = g.cand_list.alloc((uint64_t)B * n * 4)) ||
+      (rc = g.F0.alloc((uint64_t)B * n * sizeof(Item))) ||
+      (rc = g.F1.alloc((uint64_t)B * n * sizeof(Item))))
+    return rc;
+  g.Bcap = B;
+  g.a.cand_d = g.cand_d.as<uint32_t>();
+  g.a.cand_list = g.cand_list.as<uint32_t>();
+  g.a.F0 = g.F0.as<Item>();
+  g.a.F1 = g.F1.as<Item>();
+  hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, g.a.cand_d, (uint64_t)B * n, kInf32);
+  HSDL_HIP(hipGetLastError());
+  return HSDL_OK;
+}
+
+// Doubles the label pool; segment offsets stay valid (prefix copy).
+static int grow_pool(Gen &g, uint64_t used) {
+  const uint64_t ncap = g.a.pool_cap * 2;
+  if (g.held() + ncap * 8 > g.budget) return fail(HSDL_ENOMEM, "label pool exceeds the HBM budget");
+  DevBuf np;
+  int rc = np.alloc(ncap * 8);
+  if (rc) return rc;
+  HSDL_HIP(hipMemcpyAsync(np.p, g.pool.p, used * 8, hipMemcpyDeviceToDevice, g.st));
+  HSDL_HIP(hipStreamSynchronize(g.st));
+  g.pool = std::move(np);
+  g.a.pool = g.pool.as<unsigned long long>();
+  g.a.pool_cap = ncap;
+  return HSDL_OK;
+}
+
+static int grow_gtile(Gen &g) {
+  const uint64_t nw = g.a.gtile_cap * 2;
+  if (g.held() + nw * 4 > g.budget) return fail(HSDL_ENOMEM, "tile arena exceeds the HBM budget");
+  g.gtile.release();
+  int rc = g.gtile.alloc(nw * 4);
+  if (rc) return rc;
+  g.a.gtile = g.gtile.as<uint32_t>();
+  g.a.gtile_cap = nw;
+  return HSDL_OK;
+}
+
 // Enqueues one batch (K rounds) of hubs [s0, s0+B).
 static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
   GenArgs a = g.a;
@@ -505,9 +650,13 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
   for (uint32_t h = 1; h <= K; ++h) {
     a.h = h;
     hipLaunchKernelGGL(k_round_prep, dim3(small_grid), dim3(256), 0, g.st, a);
+    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 0], g.st));
     hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
+    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 1], g.st));
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
+    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
     hipLaunchKernelGGL(k_prune, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
+    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_finalize, dim3(g.grid), dim3(256), 0, g.st, a);
@@ -543,6 +692,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   if (rc) return rc;
   const uint64_t t_start = now_ns();
   Gen g;
+  g.verbose = getenv("HSDL_VERBOSE") != nullptr;
   HSDL_HIP(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
   struct StreamGuard {
     hipStream_t s;
@@ -553,47 +703,45 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
 
   size_t free_b = 0, total_b = 0;
   HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
-  uint64_t budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
+  g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
   const uint64_t graph_bytes = (n + 1ull) * 8 + nnz * 8;
-  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 6);
-  if (budget < graph_bytes + vert_bytes + (64ull << 20))
-    return fail(HSDL_ENOMEM, "HBM budget below the graph and per-vertex state");
-  uint64_t rest = budget - graph_bytes - vert_bytes;
-  // label pool: 40% of the rest; batch buffers 45%; tile arena 15%
-  const uint32_t c0 = 16;
-  uint64_t pool_cap = std::max<uint64_t>((uint64_t)(rest * 0.40) / 8, (uint64_t)n * c0 * 2);
-  uint64_t batch_bytes = (uint64_t)(rest * 0.45);
-  const uint64_t per_hub = (uint64_t)n * (4 + 4 + 12 + 12);
-  uint32_t Bmax = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(batch_bytes / per_hub, n));
-  Bmax = std::min<uint32_t>(Bmax, 1u << 16);
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5);
+  g.fixed = graph_bytes + vert_bytes + (2ull << 20);
+  const uint32_t c0 = 16;  // initial label slots per vertex
+  const uint64_t pool0 = opts->pool_init_labels
+                             ? std::max<uint64_t>(opts->pool_init_labels, (uint64_t)n * c0)
+                             : std::max<uint64_t>((uint64_t)n * c0 * 2, 32ull << 20);
+  const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
+                                                 : (uint64_t)g.grid * 16384;
+  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + batch_bytes(1, n))
+    return fail(HSDL_ENOMEM, "HBM budget below the graph, label pool and one hub batch");
+  // batch buffers may take half of what is left after room for the pool to
+  // grow 8x and the tile arena 4x
+  const uint64_t reserve = g.fixed + pool0 * 8 * 8 + gtile0 * 4 * 4;
+  const uint64_t spare = g.budget > reserve ? (g.budget - reserve) / 2 : batch_bytes(1, n);
+  uint32_t Bmax = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>({spare / batch_bytes(1, n), (uint64_t)n, 1ull << 16}));
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
   if (plain) Bmax = 1;
   if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
-  uint64_t gtile_words = std::max<uint64_t>((uint64_t)(rest * 0.15) / 4, (uint64_t)g.grid * 4096);
 
-  // device graph
-  if ((rc = g.off.alloc((n + 1ull) * 8)) || (rc = g.adj.alloc(std::max<uint64_t>(nnz, 1) * 8)))
-    return rc;
-  {
-    std::vector<uint2> adj(nnz);
-    for (uint64_t e = 0; e < nnz; ++e) adj[e] = make_uint2(hg->col[e], (uint32_t)hg->w[e]);
-    HSDL_HIP(hipMemcpy(g.off.p, hg->off.data(), (n + 1ull) * 8, hipMemcpyHostToDevice));
-    if (nnz) HSDL_HIP(hipMemcpy(g.adj.p, adj.data(), nnz * 8, hipMemcpyHostToDevice));
+  // device graph (kept with the graph handle)
+  if ((rc = upload_graph(const_cast<hsdl_graph *>(hg), dev))) return rc;
+  g.timing = opts->kernel_timing != 0;
+  if (g.timing) {
+    g.ev.resize(4 * K);
+    for (auto &e : g.ev) HSDL_HIP(hipEventCreate(&e));
   }
   const uint64_t t_csr = now_ns();  // build time starts with the CSR resident
-  if ((rc = g.pool.alloc(pool_cap * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
+  if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
       (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
       (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
-      (rc = g.touched.alloc(n * 4ull)) ||
-      (rc = g.cand_d.alloc((uint64_t)Bmax * n * 4)) ||
-      (rc = g.cand_list.alloc((uint64_t)Bmax * n * 4)) ||
-      (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
-      (rc = g.F0.alloc((uint64_t)Bmax * n * sizeof(Item))) ||
-      (rc = g.F1.alloc((uint64_t)Bmax * n * sizeof(Item))) ||
-      (rc = g.gtile.alloc(gtile_words * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))))
+      (rc = g.touched.alloc(n * 4ull)) || (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) ||
+      (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) || (rc = g.gtile.alloc(gtile0 * 4)) ||
+      (rc = g.ctrl.alloc(sizeof(Ctrl))))
     return rc;
 
-  // LDS tile: 56 KB per workgroup
+  // LDS tile per workgroup
   const uint32_t K1 = K + 1;
   uint32_t lds_slots = 64;
   while ((lds_slots * 2) * (K1 + 1) * 4 <= 56 * 1024) lds_slots *= 2;
@@ -605,56 +753,100 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   // accepted candidates of a work item are staged in 256 LDS slots
   a.tile = opts->tile ? std::min<uint32_t>(opts->tile, 256u) : 256u;
   a.lds_slots = lds_slots;
-  a.off = g.off.as<unsigned long long>();
-  a.adj = g.adj.as<uint2>();
+  a.off = static_cast<const unsigned long long *>(hg->d_off);
+  a.adj = static_cast<const uint2 *>(hg->d_adj);
   a.pool = g.pool.as<unsigned long long>();
-  a.pool_cap = pool_cap;
+  a.pool_cap = pool0;
   a.seg_start = g.seg_start.as<unsigned long long>();
   a.seg_len = g.seg_len.as<uint32_t>();
   a.seg_cap = g.seg_cap.as<uint32_t>();
   a.add_cnt = g.add_cnt.as<uint32_t>();
   a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
-  a.cand_d = g.cand_d.as<uint32_t>();
-  a.cand_list = g.cand_list.as<uint32_t>();
   a.hub_ccnt = g.hub_ccnt.as<uint32_t>();
   a.chunk_off = g.chunk_off.as<uint32_t>();
-  a.F0 = g.F0.as<Item>();
-  a.F1 = g.F1.as<Item>();
   a.gtile = g.gtile.as<uint32_t>();
-  a.gtile_cap = gtile_words;
+  a.gtile_cap = gtile0;
   a.ctrl = g.ctrl.as<Ctrl>();
 
-  HSDL_HIP(hipMemsetAsync(g.ctrl.p, 0, sizeof(Ctrl), g.st));
   {
     Ctrl c0v{};
     c0v.pool_top = (unsigned long long)n * c0;
     HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &c0v, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
   }
   hipLaunchKernelGGL(k_init_labels, dim3(1024), dim3(256), 0, g.st, a, c0);
-  hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, a.cand_d,
-                     (uint64_t)Bmax * n, kInf32);
   HSDL_HIP(hipGetLastError());
 
   BatchTuner tuner{plain ? 1u : std::min<uint32_t>(Bmax, 64u), Bmax, 0.025};
   if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
-  uint64_t batches = 0;
+  uint64_t batches = 0, redos = 0;
   uint32_t s0 = 0;
   Ctrl c{};
   while (s0 < n) {
     const uint32_t B = std::min<uint32_t>(tuner.B, n - s0);
+    // room for the tuner to double B without reallocating
+    if ((rc = ensure_batch_capacity(g, std::min<uint32_t>(Bmax, 2 * B)))) return rc;
     const uint64_t tb = now_ns();
     if ((rc = enqueue_batch(g, s0, B, K))) return rc;
     if ((rc = read_ctrl(g, c))) return rc;
-    if (c.overflow & 1u) return fail(HSDL_ENOMEM, "label pool exhausted; raise hbm_budget");
-    if (c.overflow & 2u) return fail(HSDL_ENOMEM, "tile arena exhausted; raise hbm_budget");
+    if (g.timing) {
+      for (uint32_t h = 0; h < K; ++h) {
+        float ms1 = 0, ms2 = 0;
+        HSDL_HIP(hipEventElapsedTime(&ms1, g.ev[4 * h + 0], g.ev[4 * h + 1]));
+        HSDL_HIP(hipEventElapsedTime(&ms2, g.ev[4 * h + 2], g.ev[4 * h + 3]));
+        g.t_expand += ms1 * 1e-3;
+        g.t_prune += ms2 * 1e-3;
+      }
+      g.n_prune += K;
+    }
+    if (c.overflow) {
+      // roll the batch back, grow what ran out and run it again
+      GenArgs ra = g.a;
+      ra.s0 = s0;
+      ra.B = B;
+      hipLaunchKernelGGL(k_rollback, dim3(1024), dim3(256), 0, g.st, ra);
+      HSDL_HIP(hipGetLastError());
+      HSDL_HIP(hipStreamSynchronize(g.st));
+      const uint64_t used = std::min<uint64_t>(c.pool_top, g.a.pool_cap);
+      if (c.overflow & 1u) {
+        if ((rc = grow_pool(g, used))) return rc;
+        while (g.a.pool_cap < used * 3 / 2)
+          if ((rc = grow_pool(g, used))) return rc;
+      }
+      if ((c.overflow & 2u) && (rc = grow_gtile(g))) return rc;
+      Ctrl fix = c;
+      fix.overflow = 0;
+      fix.pool_top = used;
+      HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &fix, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
+      HSDL_HIP(hipStreamSynchronize(g.st));
+      if (g.verbose)
+        fprintf(stderr, "[hsdl] batch s0=%u B=%u overflow=%u: pool %llu words, arena %llu words; redo\n",
+                s0, B, c.overflow, (unsigned long long)g.a.pool_cap,
+                (unsigned long long)g.a.gtile_cap);
+      if (++redos > 64) return fail(HSDL_ENOMEM, "label pool or tile arena kept overflowing");
+      continue;
+    }
+    // keep headroom: grow the pool between batches when it is 3/4 full
+    if (c.pool_top * 4 > g.a.pool_cap * 3 && (rc = grow_pool(g, c.pool_top))) return rc;
     s0 += B;
     ++batches;
     const double dt = (now_ns() - tb) * 1e-9;
+    if (g.verbose)
+      fprintf(stderr, "[hsdl] batch %llu s0=%u B=%u %.3f ms cand=%llu pool=%llu/%llu\n",
+              (unsigned long long)batches, s0 - B, B, dt * 1e3,
+              (unsigned long long)c.cand_total, (unsigned long long)c.pool_top,
+              (unsigned long long)g.a.pool_cap);
     if (!plain && !opts->hub_batch) tuner.next(dt);
     if (opts->deadline_s > 0 && (now_ns() - t_start) * 1e-9 > opts->deadline_s)
       return fail(HSDL_ETIMEDOUT, "generation passed its deadline");
   }
+  const uint64_t t_gen = now_ns();
+  // release batch buffers before the export needs its own
+  g.cand_d.release();
+  g.cand_list.release();
+  g.F0.release();
+  g.F1.release();
+  g.gtile.release();
 
   // final CSR sorted per vertex
   hsdl_index *idx = new hsdl_index();
@@ -675,10 +867,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   hipLaunchKernelGGL(k_gather, dim3(g.grid), dim3(256), 0, g.st, a,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>());
   if (hipStreamSynchronize(g.st) != hipSuccess) return cleanup(fail(HSDL_EDEVICE, "gather failed"));
+  g.pool.release();
   const uint64_t t_sort0 = now_ns();
   if ((rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st)))
     return cleanup(rc);
   const uint64_t t_end = now_ns();
+  if (g.verbose)
+    fprintf(stderr, "[hsdl] generate: %.3f s batches (%llu, %llu redo), %.3f s export+sort, %llu labels\n",
+            (t_gen - t_csr) * 1e-9, (unsigned long long)batches, (unsigned long long)redos,
+            (t_end - t_gen) * 1e-9, (unsigned long long)total);
   idx->stats.labels_uncleaned = total;
   idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
   idx->stats.t_sort_s = (t_end - t_sort0) * 1e-9;
@@ -689,11 +886,24 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   idx->stats.candidates = c.cand_total;
   idx->stats.frontier_items = c.item_total;
   idx->stats.scanned_edges = c.edge_total;
+  idx->stats.t_expand_s = g.t_expand;
+  idx->stats.t_prune_s = g.t_prune;
+  idx->stats.prune_launches = g.n_prune;
+  idx->stats.prune_label_reads = c.label_reads;
+  idx->stats.prune_tile_reads = c.tile_reads;
   *out = idx;
   return HSDL_OK;
 }
 
 }  // namespace hsdl
+
+extern "C" int hsdl_graph_upload(hsdl_graph *g, int32_t device) {
+  if (!g) return hsdl::fail(HSDL_EINVAL, "null graph");
+  int dev = 0;
+  int rc = hsdl::use_device(device, &dev);
+  if (rc) return rc;
+  return hsdl::upload_graph(g, dev);
+}
 
 extern "C" int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
                              hsdl_index **out) {

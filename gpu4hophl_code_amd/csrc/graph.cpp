@@ -318,6 +318,7 @@ int hsdl_graph_load_edgelist(const char *path, uint32_t flags, hsdl_graph **out)
 
 int hsdl_graph_reorder_by_degree(hsdl_graph *g, uint32_t *new_to_old) {
   if (!g) return fail(HSDL_EINVAL, "null graph");
+  g->drop_device_copy();
   std::vector<uint32_t> n2o;
   reorder_by_degree(*g, n2o);
   if (new_to_old) std::copy(n2o.begin(), n2o.end(), new_to_old);

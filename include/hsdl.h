@@ -91,6 +91,11 @@ int hsdl_graph_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, do
 int hsdl_graph_grid(uint32_t rows, uint32_t cols, uint64_t seed, int32_t wmin,
                     int32_t wmax, hsdl_graph **out);
 
+/* Copies the CSR to HBM on `device` (-1 = current) and keeps it with the
+ * graph, so hsdl_generate starts from a device-resident graph.  Optional:
+ * hsdl_generate uploads it itself otherwise.  Freed with the graph. */
+int hsdl_graph_upload(hsdl_graph *g, int32_t device);
+
 /* n and stored adjacency entries (2*undirected edges, self loops once). */
 int hsdl_graph_info(const hsdl_graph *g, uint32_t *n, uint64_t *nnz);
 /* Symmetric CSR copy: off[n+1], col[nnz], w[nnz]. */
@@ -109,7 +114,14 @@ typedef struct hsdl_opts {
                              of free HBM at the call                          */
   double deadline_s;      /* wall-clock limit, 0 = none (HSDL_ETIMEDOUT)     */
   int32_t device;         /* HIP device ordinal, -1 = current                */
-  uint32_t reserved[8];
+  uint64_t pool_init_labels;  /* first label-pool size, 0 = automatic; the
+                                 pool grows (rolling back and re-running the
+                                 batch that ran out) as labels accumulate   */
+  uint64_t arena_init_words;  /* first size of the global tile arena for hubs
+                                 whose label tile exceeds LDS, 0 = automatic */
+  uint32_t kernel_timing;     /* 1: time the expand and prune launches with
+                                 HIP events on the build stream (stats)     */
+  uint32_t reserved[3];
 } hsdl_opts;
 
 /* Fills opts with the defaults above. */
@@ -140,6 +152,11 @@ typedef struct hsdl_stats {
   uint64_t candidates;     /* deduplicated (hub,vertex) candidates checked  */
   uint64_t frontier_items;
   uint64_t scanned_edges;
+  /* filled when hsdl_opts.kernel_timing is set */
+  double t_expand_s, t_prune_s;     /* summed HIP-event durations          */
+  uint64_t prune_launches;
+  uint64_t prune_label_reads;       /* labels of L(y) streamed by the prune */
+  uint64_t prune_tile_reads;        /* labels of L(s) staged into tiles     */
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 

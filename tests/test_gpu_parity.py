@@ -49,6 +49,22 @@ def test_c1_any_hub_batch_gives_same_set(c1, hub_batch):
     same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
 
 
+def test_c1_pool_and_arena_growth_roll_back_and_redo(c1):
+    """A pool and tile arena far too small force the overflow path: the batch
+    is rolled back, the buffer grows and the batch runs again."""
+    g, z = c1
+    idx = H.hop_constrained_two_hop_labels_generation(
+        g, 4, hub_batch=512, pool_init_labels=4096 * 16 + 1, arena_init_words=1)
+    same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
+
+
+def test_deadline_reports_timeout(c1):
+    g, _ = c1
+    with pytest.raises(H.HSDLError) as e:
+        H.hop_constrained_two_hop_labels_generation(g, 4, mode=H.MODE_PLAIN, deadline_s=1e-6)
+    assert e.value.code == -5
+
+
 def test_c1_plain_one_hub_per_launch(c1):
     g, z = c1
     idx = H.hop_constrained_two_hop_labels_generation(g, 4, mode=H.MODE_PLAIN)
