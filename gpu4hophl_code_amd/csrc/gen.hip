@@ -21,14 +21,18 @@
 // B = 1 is the plain one-hub-per-launch form.
 //
 // Per round, all on one stream, no host synchronisation:
-//   round_prep  -> zero per-hub candidate counters
+//   round_prep  -> zero per-hub counters
 //   expand      -> wave-cooperative CSR scan of the frontier: each wave takes
-//                  64 frontier items, prefix-sums their degrees in registers
-//                  and walks the concatenated adjacency 64 edges at a time
-//                  (coalesced 8 B (col,w) loads); candidates are deduplicated
-//                  by atomicMin on a dense per-(hub,vertex) slot and the first
-//                  toucher appends the vertex to the hub's candidate list
-//   chunk_scan  -> per-hub candidate lists cut into work items of `tile`
+//                  64 frontier items, binary-searches each item's id-sorted
+//                  adjacency for the first neighbour above the hub (rank
+//                  pruning without reading the pruned edges), prefix-sums the
+//                  remaining degrees in registers and walks the concatenated
+//                  edges 64 at a time (coalesced 8 B (col,w) loads);
+//                  candidates are deduplicated by atomicMin on a dense
+//                  per-(hub,vertex) slot and the first toucher appends
+//                  (hub,vertex) to a flat list
+//   chunk_scan  -> per-hub offsets and work items of `tile` candidates
+//   group       -> flat candidates scattered into per-hub runs
 //   prune       -> persistent: a workgroup takes a work item, stages the hub's
 //                  labels L(s) in an LDS hash tile (hub -> prefix-min dist per
 //                  hop budget), then one wavefront per candidate streams L(y)
@@ -37,11 +41,16 @@
 //                  overflow (bump allocation in a device pool; no locks)
 //   append      -> writes accepted labels; they also form the next frontier
 //   finalize    -> publishes segment lengths
+// Capacities (label pool, tile arena, candidate and frontier lists) start
+// small; a batch that overflows one raises a flag, the later kernels of the
+// batch stand down, and the host rolls the batch back, grows the buffer and
+// runs the batch again.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -55,21 +64,27 @@ struct Item {  // frontier entry / accepted label: hub slot, vertex, distance
   uint32_t d;
 };
 
+enum : uint32_t {
+  kOvfPool = 1u,   // label pool
+  kOvfArena = 2u,  // global tile arena
+  kOvfCand = 4u,   // flat candidate list
+  kOvfFront = 8u,  // next frontier list
+};
+
 struct Ctrl {
   unsigned long long pool_top;
-  unsigned long long gtile_top;
   unsigned long long cand_total;
   unsigned long long item_total;
   unsigned long long edge_total;
+  unsigned long long label_reads;  // labels of L(y) streamed by the prune
+  unsigned long long tile_reads;   // labels of L(s) staged into tiles
   uint32_t fcnt[2];
+  uint32_t ccnt;  // flat candidates this round
   uint32_t touched_cnt;
   uint32_t work_ctr;
   uint32_t total_chunks;
-  uint32_t overflow;  // bit 0: label pool, bit 1: tile arena
+  uint32_t overflow;
   uint32_t maxlen;
-  uint32_t pad;
-  unsigned long long label_reads;  // labels of L(y) streamed by the prune
-  unsigned long long tile_reads;   // labels of L(s) staged into tiles
 };
 
 struct GenArgs {
@@ -78,8 +93,8 @@ struct GenArgs {
   uint32_t s0;
   uint32_t B;
   uint32_t h;
-  uint32_t cur;  // frontier buffer read this round
-  uint32_t tile; // candidates per prune work item
+  uint32_t cur;        // frontier buffer read this round
+  uint32_t tile;       // candidates per prune work item
   uint32_t lds_slots;  // LDS tile capacity (slots)
   const unsigned long long *off;
   const uint2 *adj;
@@ -91,9 +106,14 @@ struct GenArgs {
   uint32_t *add_cnt;
   uint32_t *add_cur;
   uint32_t *touched;
-  uint32_t *cand_d;
-  uint32_t *cand_list;
+  uint32_t *cand_d;           // dense slots [Bmax][n]
+  unsigned long long *cflat;  // (b << 32 | y), ccap entries
+  uint32_t *cgroup;           // y grouped by hub, ccap entries
+  uint32_t ccap;
+  uint32_t fcap;
   uint32_t *hub_ccnt;
+  uint32_t *hub_off;
+  uint32_t *hub_cur;
   uint32_t *chunk_off;
   Item *F0;
   Item *F1;
@@ -131,20 +151,32 @@ __global__ void k_batch_begin(GenArgs a) {
 }
 
 __global__ void k_round_prep(GenArgs a) {
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x)
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x) {
     a.hub_ccnt[b] = 0;
+    a.hub_cur[b] = 0;
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     a.ctrl->fcnt[a.cur ^ 1] = 0;
+    a.ctrl->ccnt = 0;
     a.ctrl->touched_cnt = 0;
     a.ctrl->work_ctr = 0;
-    a.ctrl->gtile_top = 0;
   }
 }
 
 // ---------------------------------------------------------------- expand
+// first index in adj[beg, end) whose neighbour id exceeds s
+__device__ inline unsigned long long first_above(const uint2 *adj, unsigned long long beg,
+                                                 unsigned long long end, uint32_t s) {
+  while (beg < end) {
+    const unsigned long long mid = (beg + end) >> 1;
+    if (adj[mid].x <= s) beg = mid + 1; else end = mid;
+  }
+  return beg;
+}
+
 __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   const Item *F = frontier(a, a.cur);
-  const uint32_t fcnt = a.ctrl->fcnt[a.cur];
+  const uint32_t fcnt = min(a.ctrl->fcnt[a.cur], a.fcap);
   const int lane = lane_id();
   const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
   const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / kWave;
@@ -152,16 +184,16 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   uint64_t edges_seen = 0;
   for (uint64_t base = wave * kWave; base < fcnt; base += nwaves * kWave) {
     const uint64_t i = base + lane;
-    uint32_t ib = 0, ix = 0, id = 0;
+    uint32_t ib = 0, id = 0;
     unsigned long long beg = 0;
     uint32_t deg = 0;
     if (i < fcnt) {
-      Item it = F[i];
+      const Item it = F[i];
       ib = it.b;
-      ix = it.y;
       id = it.d;
-      beg = a.off[ix];
-      deg = (uint32_t)(a.off[ix + 1] - beg);
+      const unsigned long long e0 = a.off[it.y], e1 = a.off[it.y + 1];
+      beg = first_above(a.adj, e0, e1, a.s0 + it.b);  // rank pruning y > s
+      deg = (uint32_t)(e1 - beg);
     }
     const uint32_t incl = wave_incl_scan_u32(deg);
     const uint32_t excl = incl - deg;
@@ -169,11 +201,11 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
     edges_seen += total;
     for (uint32_t jb = 0; jb < total; jb += kWave) {
       const uint32_t j = jb + lane;
-      // largest k with excl_k <= j (items with deg 0 are skipped over)
+      // largest k with excl_k <= j (items with no edges are skipped over)
       int k = 0;
 #pragma unroll
       for (int step = 32; step > 0; step >>= 1) {
-        uint32_t e = __shfl(excl, k + step, kWave);
+        const uint32_t e = __shfl(excl, k + step, kWave);
         if (e <= j) k += step;
       }
       const uint32_t kb = __shfl(ib, k, kWave);
@@ -185,24 +217,31 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       if (j < total) {
         const uint2 ed = a.adj[kbeg + (j - kex)];
         y = ed.x;
-        if (y > a.s0 + kb) {
-          const uint32_t nd = kd + ed.y;
-          const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], nd);
-          first = (old == kInf32);
-        }
+        const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], kd + ed.y);
+        first = (old == kInf32);
       }
-      // append first touches to per-hub candidate lists, one atomic per hub
-      unsigned long long pend = __ballot(first);
-      while (pend) {
-        const int leader = __ffsll(pend) - 1;
-        const uint32_t bl = __shfl(kb, leader, kWave);
-        const unsigned long long grp = __ballot(first && kb == bl);
-        uint32_t basepos = 0;
-        if (lane == leader) basepos = atomicAdd(&a.hub_ccnt[bl], (uint32_t)__popcll(grp));
-        basepos = __shfl(basepos, leader, kWave);
-        if (first && kb == bl)
-          a.cand_list[(uint64_t)bl * a.n + basepos + __popcll(grp & lt_mask)] = y;
-        pend &= ~grp;
+      // first touches: one flat-list atomic per wave, one count atomic per hub
+      const unsigned long long firsts = __ballot(first);
+      if (firsts) {
+        uint32_t fbase = 0;
+        const int lead = __ffsll((long long)firsts) - 1;
+        if (lane == lead) fbase = atomicAdd(&a.ctrl->ccnt, (uint32_t)__popcll(firsts));
+        fbase = __shfl(fbase, lead, kWave);
+        if (first) {
+          const uint32_t pos = fbase + (uint32_t)__popcll(firsts & lt_mask);
+          if (pos < a.ccap)
+            a.cflat[pos] = ((unsigned long long)kb << 32) | y;
+          else
+            atomicOr(&a.ctrl->overflow, kOvfCand);
+        }
+        unsigned long long pend = firsts;
+        while (pend) {
+          const int leader = __ffsll((long long)pend) - 1;
+          const uint32_t bl = __shfl(kb, leader, kWave);
+          const unsigned long long grp = __ballot(first && kb == bl);
+          if (lane == leader) atomicAdd(&a.hub_ccnt[bl], (uint32_t)__popcll(grp));
+          pend &= ~grp;
+        }
       }
     }
   }
@@ -210,47 +249,86 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
 }
 
 // ------------------------------------------------------------ chunk scan
-// One workgroup: chunk_off[b] = sum_{b'<b} ceil(hub_ccnt[b'] / tile).
+// One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
+// sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
 __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
   __shared__ uint32_t part[1024];
+  __shared__ uint32_t partc[1024];
   const uint32_t t = threadIdx.x;
   const uint32_t per = (a.B + blockDim.x - 1) / blockDim.x;
   const uint32_t lo = min(a.B, t * per), hi = min(a.B, lo + per);
-  uint32_t sum = 0;
-  unsigned long long cs = 0;
+  uint32_t sum = 0, csum = 0;
   for (uint32_t b = lo; b < hi; ++b) {
     sum += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
-    cs += a.hub_ccnt[b];
+    csum += a.hub_ccnt[b];
   }
   part[t] = sum;
+  partc[t] = csum;
   __syncthreads();
   for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
-    uint32_t v = (t >= o) ? part[t - o] : 0;
+    const uint32_t v = (t >= o) ? part[t - o] : 0;
+    const uint32_t vc = (t >= o) ? partc[t - o] : 0;
     __syncthreads();
     part[t] += v;
+    partc[t] += vc;
     __syncthreads();
   }
-  uint32_t run = part[t] - sum;
+  uint32_t run = part[t] - sum, runc = partc[t] - csum;
   for (uint32_t b = lo; b < hi; ++b) {
     a.chunk_off[b] = run;
+    a.hub_off[b] = runc;
     run += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
+    runc += a.hub_ccnt[b];
   }
-  if (cs) atomicAdd(&a.ctrl->cand_total, cs);
   if (t == blockDim.x - 1) {
+    const bool dead = a.ctrl->overflow != 0;
     a.chunk_off[a.B] = part[t];
-    a.ctrl->total_chunks = part[t];
+    a.hub_off[a.B] = partc[t];
+    a.ctrl->total_chunks = dead ? 0u : part[t];
     a.ctrl->work_ctr = 0;
+    a.ctrl->cand_total += partc[t];
+    a.ctrl->item_total += min(a.ctrl->fcnt[a.cur], a.fcap);
   }
-  if (t == 0) atomicAdd(&a.ctrl->item_total, (unsigned long long)a.ctrl->fcnt[a.cur]);
+}
+
+__global__ void __launch_bounds__(256) k_group(GenArgs a) {
+  if (a.ctrl->overflow) return;
+  const uint32_t cnt = a.ctrl->ccnt;
+  const int lane = lane_id();
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
+  // whole waves iterate together so the ballots see every lane
+  for (uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) - lane; base < cnt;
+       base += nthreads) {
+    const uint64_t i = base + lane;
+    const bool valid = i < cnt;
+    uint32_t b = 0, y = 0;
+    if (valid) {
+      const unsigned long long e = a.cflat[i];
+      b = (uint32_t)(e >> 32);
+      y = (uint32_t)e;
+    }
+    unsigned long long pend = __ballot(valid);
+    while (pend) {
+      const int leader = __ffsll((long long)pend) - 1;
+      const uint32_t bl = __shfl(b, leader, kWave);
+      const unsigned long long grp = __ballot(valid && b == bl);
+      uint32_t basepos = 0;
+      if (lane == leader) basepos = atomicAdd(&a.hub_cur[bl], (uint32_t)__popcll(grp));
+      basepos = __shfl(basepos, leader, kWave);
+      if (valid && b == bl) a.cgroup[a.hub_off[bl] + basepos + __popcll(grp & lt_mask)] = y;
+      pend &= ~grp;
+    }
+  }
 }
 
 // ----------------------------------------------------------------- prune
 __device__ inline uint32_t hash_slot(uint32_t key, uint32_t mask) {
-  return (key * 0x9E3779B1u) >> 7 & mask;
+  return ((key * 0x9E3779B1u) >> 7) & mask;
 }
 
 __device__ inline uint32_t tile_cap_for(uint32_t len) {
-  uint32_t want = len + (len >> 1) + 1;  // load factor <= 2/3
+  const uint32_t want = len + (len >> 1) + 1;  // load factor <= 2/3
   uint32_t c = 64;
   while (c < want) c <<= 1;
   return c;
@@ -272,7 +350,7 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, uint32_
     const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
     uint32_t slot = hash_slot(hub, mask);
     for (;;) {
-      uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
+      const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
       if (prev == kInf32 || prev == hub) break;
       slot = (slot + 1) & mask;
     }
@@ -302,8 +380,8 @@ __device__ inline uint32_t tile_lookup(const uint32_t *keys, const uint32_t *val
   }
 }
 
-// Returns true when some common hub gives da + db <= dy (candidate pruned).
-// `reads` receives the number of labels of L(y) loaded.
+// True when some common hub gives da + db <= dy (candidate pruned); `reads`
+// receives the number of labels of L(y) loaded.
 template <bool kGlobal>
 __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
                                const uint32_t *vals, uint32_t cap, uint32_t &reads) {
@@ -341,8 +419,8 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
   Item *Fn = frontier(a, a.cur ^ 1);
   uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
   unsigned long long wave_reads = 0, tile_reads = 0;
-  uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * (a.gtile_cap / gridDim.x);
   const uint64_t gwords = a.gtile_cap / gridDim.x;
+  uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
   for (;;) {
     if (threadIdx.x == 0) {
       sh_w = atomicAdd(&a.ctrl->work_ctr, 1u);
@@ -354,13 +432,14 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
     // hub slot b with chunk_off[b] <= w < chunk_off[b+1]
     uint32_t lo = 0, hi = a.B;  // invariant chunk_off[lo] <= w < chunk_off[hi]
     while (hi - lo > 1) {
-      uint32_t mid = (lo + hi) >> 1;
+      const uint32_t mid = (lo + hi) >> 1;
       if (a.chunk_off[mid] <= w) lo = mid; else hi = mid;
     }
     const uint32_t b = lo;
     const uint32_t s = a.s0 + b;
     const uint32_t c_beg = (w - a.chunk_off[b]) * a.tile;
     const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
+    const uint32_t *cands = a.cgroup + a.hub_off[b];
     const uint32_t cap = tile_cap_for(a.seg_len[s]);
     uint32_t *keys, *vals;
     const bool in_lds = cap <= a.lds_slots;
@@ -373,15 +452,10 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
         tile_reads += a.seg_len[s];
       }
     } else {
-      // per-workgroup tile in the global arena
       if ((uint64_t)cap * (K1 + 1) > gwords) {
         // arena too small: the host rolls the batch back, grows the arena
-        // and runs the batch again; reset the slots so they stay clean
-        if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, 2u);
-        for (uint32_t c = c_beg + threadIdx.x; c < c_end; c += blockDim.x) {
-          const uint32_t y = a.cand_list[(uint64_t)b * a.n + c];
-          a.cand_d[(uint64_t)b * a.n + y] = kInf32;
-        }
+        // and runs the batch again
+        if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, kOvfArena);
         __syncthreads();
         continue;
       }
@@ -392,7 +466,7 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
       tile_reads += a.seg_len[s];
     }
     for (uint32_t c = c_beg + wv; c < c_end; c += nwv) {
-      const uint32_t y = a.cand_list[(uint64_t)b * a.n + c];
+      const uint32_t y = cands[c];
       const uint64_t slot = (uint64_t)b * a.n + y;
       const uint32_t dy = a.cand_d[slot];
       uint32_t reads = 0;
@@ -402,19 +476,23 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
       if (lane == 0) {
         a.cand_d[slot] = kInf32;  // slot ready for the next round
         if (!pruned) {
-          uint32_t p = atomicAdd(&sh_acc_n, 1u);
+          const uint32_t p = atomicAdd(&sh_acc_n, 1u);
           sh_acc[p] = Item{b, y, dy};
           if (atomicAdd(&a.add_cnt[y], 1u) == 0) {
-            uint32_t t = atomicAdd(&a.ctrl->touched_cnt, 1u);
+            const uint32_t t = atomicAdd(&a.ctrl->touched_cnt, 1u);
             a.touched[t] = y;
           }
         }
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && sh_acc_n) sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], sh_acc_n);
+    if (threadIdx.x == 0 && sh_acc_n) {
+      sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], sh_acc_n);
+      if (sh_acc_base + sh_acc_n > a.fcap) atomicOr(&a.ctrl->overflow, kOvfFront);
+    }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < sh_acc_n; i += blockDim.x) Fn[sh_acc_base + i] = sh_acc[i];
+    for (uint32_t i = threadIdx.x; i < sh_acc_n; i += blockDim.x)
+      if (sh_acc_base + i < a.fcap) Fn[sh_acc_base + i] = sh_acc[i];
     __syncthreads();
   }
   if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
@@ -438,7 +516,7 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
     if (lane == 0) ns = atomicAdd(&a.ctrl->pool_top, (unsigned long long)ncap);
     ns = __shfl(ns, 0, kWave);
     if (ns + ncap > a.pool_cap) {
-      if (lane == 0) atomicOr(&a.ctrl->overflow, 1u);
+      if (lane == 0) atomicOr(&a.ctrl->overflow, kOvfPool);
       continue;
     }
     const unsigned long long os = a.seg_start[y];
@@ -452,36 +530,40 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
 
 __global__ void __launch_bounds__(256) k_append(GenArgs a) {
   const Item *Fn = frontier(a, a.cur ^ 1);
-  const uint32_t cnt = a.ctrl->fcnt[a.cur ^ 1];
+  const uint32_t cnt = min(a.ctrl->fcnt[a.cur ^ 1], a.fcap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
     const Item it = Fn[i];
     const uint32_t p = a.seg_len[it.y] + atomicAdd(&a.add_cur[it.y], 1u);
     if (p < a.seg_cap[it.y])
       a.pool[a.seg_start[it.y] + p] = pack_label(a.s0 + it.b, a.h, it.d);
     else
-      atomicOr(&a.ctrl->overflow, 1u);
+      atomicOr(&a.ctrl->overflow, kOvfPool);
   }
 }
 
 __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
   const uint32_t cnt = a.ctrl->touched_cnt;
+  // a lost frontier item leaves a counted label unwritten: publish nothing
+  // (the host rolls the batch back)
+  const bool keep = !(a.ctrl->overflow & kOvfFront);
   uint32_t mx = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
     const uint32_t y = a.touched[t];
-    // capped: when the pool ran out some appends were dropped (the host then
-    // rolls the batch back)
-    const uint32_t nl = min(a.seg_len[y] + a.add_cnt[y], a.seg_cap[y]);
-    a.seg_len[y] = nl;
+    if (keep) {
+      // capped: when the pool ran out some appends were dropped
+      const uint32_t nl = min(a.seg_len[y] + a.add_cnt[y], a.seg_cap[y]);
+      a.seg_len[y] = nl;
+      mx = max(mx, nl);
+    }
     a.add_cnt[y] = 0;
     a.add_cur[y] = 0;
-    mx = max(mx, nl);
   }
   if (mx) atomicMax(&a.ctrl->maxlen, mx);
 }
 
 // Removes the labels of hubs [s0, s0+B) (the batch being rolled back): they
 // sit at the tail of every segment, after all labels of earlier batches; the
-// self label at position 0 stays.
+// self label at position 0 stays.  Also re-arms the batch's dense slots.
 __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
     uint32_t len = a.seg_len[v];
@@ -495,6 +577,10 @@ __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
     a.add_cnt[v] = 0;
     a.add_cur[v] = 0;
   }
+  const uint64_t slots = (uint64_t)a.B * a.n;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    a.cand_d[i] = kInf32;
 }
 
 // --------------------------------------------------------------- export
@@ -549,10 +635,9 @@ static int upload_graph(hsdl_graph *g, int dev) {
 struct Gen {
   GenArgs a{};
   DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
-  DevBuf cand_d, cand_list, hub_ccnt, chunk_off, F0, F1, gtile, ctrl;
-  uint32_t Bcap = 0;       // hub slots the batch buffers hold
-  uint64_t budget = 0;     // device bytes the build may hold
-  uint64_t fixed = 0;      // bytes outside pool / batch buffers / arena
+  DevBuf cand_d, cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl;
+  uint64_t budget = 0;  // device bytes the build may hold
+  uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
   size_t lds_bytes = 0;
   hipStream_t st = nullptr;
@@ -562,12 +647,12 @@ struct Gen {
   std::vector<hipEvent_t> ev;
   double t_expand = 0, t_prune = 0;
   uint64_t n_prune = 0;
+  double h_grow = 0, h_enqueue = 0, h_wait = 0;  // host-side phase timers (verbose)
+  uint64_t held() const {
+    return fixed + pool.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes + gtile.bytes;
+  }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-  }
-  uint64_t held() const {
-    return fixed + pool.bytes + cand_d.bytes + cand_list.bytes + F0.bytes + F1.bytes +
-           gtile.bytes;
   }
 };
 
@@ -585,38 +670,13 @@ static int read_ctrl(Gen &g, Ctrl &c) {
   return HSDL_OK;
 }
 
-// Batch buffers for B hub slots: dense candidate slots (kept at "no
-// candidate" between batches), candidate lists and two frontier buffers.
-static uint64_t batch_bytes(uint64_t B, uint32_t n) { return B * n * (4ull + 4 + 2 * sizeof(Item)); }
-
-static int ensure_batch_capacity(Gen &g, uint32_t B) {
-  if (B <= g.Bcap) return HSDL_OK;
-  const uint32_t n = g.a.n;
-  g.cand_d.release();
-  g.cand_list.release();
-  g.F0.release();
-  g.F1.release();
-  if (g.held() + batch_bytes(B, n) > g.budget)
-    return fail(HSDL_ENOMEM, "batch buffers exceed the HBM budget");
-  int rc;
-  if ((rc = g.cand_d.alloc((uint64_t)B * n * 4)) || (rc = g.cand_list.alloc((uint64_t)B * n * 4)) ||
-      (rc = g.F0.alloc((uint64_t)B * n * sizeof(Item))) ||
-      (rc = g.F1.alloc((uint64_t)B * n * sizeof(Item))))
-    return rc;
-  g.Bcap = B;
-  g.a.cand_d = g.cand_d.as<uint32_t>();
-  g.a.cand_list = g.cand_list.as<uint32_t>();
-  g.a.F0 = g.F0.as<Item>();
-  g.a.F1 = g.F1.as<Item>();
-  hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, g.a.cand_d, (uint64_t)B * n, kInf32);
-  HSDL_HIP(hipGetLastError());
-  return HSDL_OK;
-}
-
-// Doubles the label pool; segment offsets stay valid (prefix copy).
-static int grow_pool(Gen &g, uint64_t used) {
-  const uint64_t ncap = g.a.pool_cap * 2;
-  if (g.held() + ncap * 8 > g.budget) return fail(HSDL_ENOMEM, "label pool exceeds the HBM budget");
+// Grows the label pool (at least doubling); segment offsets stay valid
+// because the used prefix is copied as is.
+static int grow_pool(Gen &g, uint64_t used, uint64_t min_cap) {
+  uint64_t ncap = g.a.pool_cap * 2;
+  while (ncap < min_cap) ncap *= 2;
+  if (g.held() - g.pool.bytes + ncap * 8 > g.budget)
+    return fail(HSDL_ENOMEM, "label pool exceeds the HBM budget");
   DevBuf np;
   int rc = np.alloc(ncap * 8);
   if (rc) return rc;
@@ -630,12 +690,41 @@ static int grow_pool(Gen &g, uint64_t used) {
 
 static int grow_gtile(Gen &g) {
   const uint64_t nw = g.a.gtile_cap * 2;
-  if (g.held() + nw * 4 > g.budget) return fail(HSDL_ENOMEM, "tile arena exceeds the HBM budget");
+  if (g.held() - g.gtile.bytes + nw * 4 > g.budget)
+    return fail(HSDL_ENOMEM, "tile arena exceeds the HBM budget");
   g.gtile.release();
   int rc = g.gtile.alloc(nw * 4);
   if (rc) return rc;
   g.a.gtile = g.gtile.as<uint32_t>();
   g.a.gtile_cap = nw;
+  return HSDL_OK;
+}
+
+static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
+  int rc;
+  if (ccap != g.a.ccap) {
+    g.cflat.release();
+    g.cgroup.release();
+    if (g.held() + (uint64_t)ccap * 12 > g.budget)
+      return fail(HSDL_ENOMEM, "candidate lists exceed the HBM budget");
+    if ((rc = g.cflat.alloc((uint64_t)ccap * 8)) || (rc = g.cgroup.alloc((uint64_t)ccap * 4)))
+      return rc;
+    g.a.cflat = g.cflat.as<unsigned long long>();
+    g.a.cgroup = g.cgroup.as<uint32_t>();
+    g.a.ccap = ccap;
+  }
+  if (fcap != g.a.fcap) {
+    g.F0.release();
+    g.F1.release();
+    if (g.held() + (uint64_t)fcap * 2 * sizeof(Item) > g.budget)
+      return fail(HSDL_ENOMEM, "frontier lists exceed the HBM budget");
+    if ((rc = g.F0.alloc((uint64_t)fcap * sizeof(Item))) ||
+        (rc = g.F1.alloc((uint64_t)fcap * sizeof(Item))))
+      return rc;
+    g.a.F0 = g.F0.as<Item>();
+    g.a.F1 = g.F1.as<Item>();
+    g.a.fcap = fcap;
+  }
   return HSDL_OK;
 }
 
@@ -654,6 +743,7 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 1], g.st));
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
+    hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
     hipLaunchKernelGGL(k_prune, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
@@ -699,31 +789,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     ~StreamGuard() { if (s) (void)hipStreamDestroy(s); }
   } sg{g.st};
   g.grid = grid_for_device(dev);
-  const uint64_t nnz = hg->off[n];
-
-  size_t free_b = 0, total_b = 0;
-  HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
-  g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
-  const uint64_t graph_bytes = (n + 1ull) * 8 + nnz * 8;
-  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5);
-  g.fixed = graph_bytes + vert_bytes + (2ull << 20);
-  const uint32_t c0 = 16;  // initial label slots per vertex
-  const uint64_t pool0 = opts->pool_init_labels
-                             ? std::max<uint64_t>(opts->pool_init_labels, (uint64_t)n * c0)
-                             : std::max<uint64_t>((uint64_t)n * c0 * 2, 32ull << 20);
-  const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
-                                                 : (uint64_t)g.grid * 16384;
-  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + batch_bytes(1, n))
-    return fail(HSDL_ENOMEM, "HBM budget below the graph, label pool and one hub batch");
-  // batch buffers may take half of what is left after room for the pool to
-  // grow 8x and the tile arena 4x
-  const uint64_t reserve = g.fixed + pool0 * 8 * 8 + gtile0 * 4 * 4;
-  const uint64_t spare = g.budget > reserve ? (g.budget - reserve) / 2 : batch_bytes(1, n);
-  uint32_t Bmax = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>({spare / batch_bytes(1, n), (uint64_t)n, 1ull << 16}));
-  const bool plain = opts->mode == HSDL_MODE_PLAIN;
-  if (plain) Bmax = 1;
-  if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
 
   // device graph (kept with the graph handle)
   if ((rc = upload_graph(const_cast<hsdl_graph *>(hg), dev))) return rc;
@@ -733,12 +798,37 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     for (auto &e : g.ev) HSDL_HIP(hipEventCreate(&e));
   }
   const uint64_t t_csr = now_ns();  // build time starts with the CSR resident
+
+  size_t free_b = 0, total_b = 0;
+  HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
+  g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
+  const uint32_t c0 = 16;  // initial label slots per vertex
+  const uint64_t pool0 = opts->pool_init_labels
+                             ? std::max<uint64_t>(opts->pool_init_labels, (uint64_t)n * c0)
+                             : std::max<uint64_t>((uint64_t)n * c0 * 4, 64ull << 20);
+  const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
+                                                 : (uint64_t)g.grid * 16384;
+  const uint32_t list0 =
+      (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30);
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5);
+  // dense dedupe slots: at most a quarter of the budget, <= 65536 hubs
+  const bool plain = opts->mode == HSDL_MODE_PLAIN;
+  uint32_t Bmax =
+      (uint32_t)std::min<uint64_t>({g.budget / 4 / (4ull * n), (uint64_t)n, 1ull << 16});
+  if (plain) Bmax = 1;
+  if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
+  if (Bmax == 0) return fail(HSDL_ENOMEM, "HBM budget below one hub's dedupe slots");
+  g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20);
+  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 36)
+    return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
+
   if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
       (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
       (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
-      (rc = g.touched.alloc(n * 4ull)) || (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) ||
-      (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) || (rc = g.gtile.alloc(gtile0 * 4)) ||
-      (rc = g.ctrl.alloc(sizeof(Ctrl))))
+      (rc = g.touched.alloc(n * 4ull)) || (rc = g.cand_d.alloc((uint64_t)Bmax * n * 4)) ||
+      (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
+      (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
+      (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))))
     return rc;
 
   // LDS tile per workgroup
@@ -763,11 +853,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.add_cnt = g.add_cnt.as<uint32_t>();
   a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
+  a.cand_d = g.cand_d.as<uint32_t>();
   a.hub_ccnt = g.hub_ccnt.as<uint32_t>();
+  a.hub_off = g.hub_off.as<uint32_t>();
+  a.hub_cur = g.hub_cur.as<uint32_t>();
   a.chunk_off = g.chunk_off.as<uint32_t>();
   a.gtile = g.gtile.as<uint32_t>();
   a.gtile_cap = gtile0;
   a.ctrl = g.ctrl.as<Ctrl>();
+  if ((rc = alloc_lists(g, list0, list0))) return rc;
 
   {
     Ctrl c0v{};
@@ -775,6 +869,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &c0v, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
   }
   hipLaunchKernelGGL(k_init_labels, dim3(1024), dim3(256), 0, g.st, a, c0);
+  hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, a.cand_d, (uint64_t)Bmax * n,
+                     kInf32);
   HSDL_HIP(hipGetLastError());
 
   BatchTuner tuner{plain ? 1u : std::min<uint32_t>(Bmax, 64u), Bmax, 0.025};
@@ -784,11 +880,12 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   Ctrl c{};
   while (s0 < n) {
     const uint32_t B = std::min<uint32_t>(tuner.B, n - s0);
-    // room for the tuner to double B without reallocating
-    if ((rc = ensure_batch_capacity(g, std::min<uint32_t>(Bmax, 2 * B)))) return rc;
     const uint64_t tb = now_ns();
     if ((rc = enqueue_batch(g, s0, B, K))) return rc;
+    const uint64_t te = now_ns();
     if ((rc = read_ctrl(g, c))) return rc;
+    g.h_enqueue += (te - tb) * 1e-9;
+    g.h_wait += (now_ns() - te) * 1e-9;
     if (g.timing) {
       for (uint32_t h = 0; h < K; ++h) {
         float ms1 = 0, ms2 = 0;
@@ -801,33 +898,42 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     }
     if (c.overflow) {
       // roll the batch back, grow what ran out and run it again
+      const uint64_t tg = now_ns();
       GenArgs ra = g.a;
       ra.s0 = s0;
       ra.B = B;
-      hipLaunchKernelGGL(k_rollback, dim3(1024), dim3(256), 0, g.st, ra);
+      hipLaunchKernelGGL(k_rollback, dim3(2048), dim3(256), 0, g.st, ra);
       HSDL_HIP(hipGetLastError());
       HSDL_HIP(hipStreamSynchronize(g.st));
       const uint64_t used = std::min<uint64_t>(c.pool_top, g.a.pool_cap);
-      if (c.overflow & 1u) {
-        if ((rc = grow_pool(g, used))) return rc;
-        while (g.a.pool_cap < used * 3 / 2)
-          if ((rc = grow_pool(g, used))) return rc;
+      if ((c.overflow & kOvfPool) && (rc = grow_pool(g, used, used * 2))) return rc;
+      if ((c.overflow & kOvfArena) && (rc = grow_gtile(g))) return rc;
+      if (c.overflow & (kOvfCand | kOvfFront)) {
+        const uint64_t nc = std::max<uint64_t>((uint64_t)g.a.ccap * 2, (uint64_t)c.ccnt * 5 / 4);
+        const uint64_t nf = std::max<uint64_t>((uint64_t)g.a.fcap * 2, nc);
+        if (nc > 0xFFFFFFF0ull || nf > 0xFFFFFFF0ull)
+          return fail(HSDL_ENOMEM, "a round holds more than 2^32 candidates; lower hub_batch");
+        if ((rc = alloc_lists(g, (uint32_t)nc, (uint32_t)nf))) return rc;
       }
-      if ((c.overflow & 2u) && (rc = grow_gtile(g))) return rc;
       Ctrl fix = c;
       fix.overflow = 0;
       fix.pool_top = used;
       HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &fix, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
       HSDL_HIP(hipStreamSynchronize(g.st));
+      g.h_grow += (now_ns() - tg) * 1e-9;
       if (g.verbose)
-        fprintf(stderr, "[hsdl] batch s0=%u B=%u overflow=%u: pool %llu words, arena %llu words; redo\n",
+        fprintf(stderr, "[hsdl] batch s0=%u B=%u overflow=%u: pool %llu, arena %llu, lists %u/%u; redo\n",
                 s0, B, c.overflow, (unsigned long long)g.a.pool_cap,
-                (unsigned long long)g.a.gtile_cap);
-      if (++redos > 64) return fail(HSDL_ENOMEM, "label pool or tile arena kept overflowing");
+                (unsigned long long)g.a.gtile_cap, g.a.ccap, g.a.fcap);
+      if (++redos > 64) return fail(HSDL_ENOMEM, "buffers kept overflowing");
       continue;
     }
-    // keep headroom: grow the pool between batches when it is 3/4 full
-    if (c.pool_top * 4 > g.a.pool_cap * 3 && (rc = grow_pool(g, c.pool_top))) return rc;
+    // keep headroom: grow the pool between batches when it is half full
+    if (c.pool_top * 2 > g.a.pool_cap) {
+      const uint64_t tg = now_ns();
+      if ((rc = grow_pool(g, c.pool_top, 0))) return rc;
+      g.h_grow += (now_ns() - tg) * 1e-9;
+    }
     s0 += B;
     ++batches;
     const double dt = (now_ns() - tb) * 1e-9;
@@ -843,7 +949,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint64_t t_gen = now_ns();
   // release batch buffers before the export needs its own
   g.cand_d.release();
-  g.cand_list.release();
+  g.cflat.release();
+  g.cgroup.release();
   g.F0.release();
   g.F1.release();
   g.gtile.release();
@@ -873,9 +980,11 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     return cleanup(rc);
   const uint64_t t_end = now_ns();
   if (g.verbose)
-    fprintf(stderr, "[hsdl] generate: %.3f s batches (%llu, %llu redo), %.3f s export+sort, %llu labels\n",
+    fprintf(stderr, "[hsdl] generate: %.3f s batches (%llu, %llu redo; host enqueue %.3f s, wait %.3f s, "
+            "grow %.3f s), %.3f s export (%.3f s sort), %llu labels\n",
             (t_gen - t_csr) * 1e-9, (unsigned long long)batches, (unsigned long long)redos,
-            (t_end - t_gen) * 1e-9, (unsigned long long)total);
+            g.h_enqueue, g.h_wait, g.h_grow, (t_end - t_gen) * 1e-9, (t_end - t_sort0) * 1e-9,
+            (unsigned long long)total);
   idx->stats.labels_uncleaned = total;
   idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
   idx->stats.t_sort_s = (t_end - t_sort0) * 1e-9;
