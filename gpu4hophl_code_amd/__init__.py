@@ -64,7 +64,8 @@ class hsdl_stats(ctypes.Structure):
                 ("candidates", ctypes.c_uint64), ("frontier_items", ctypes.c_uint64),
                 ("scanned_edges", ctypes.c_uint64), ("t_expand_s", ctypes.c_double),
                 ("t_prune_s", ctypes.c_double), ("prune_launches", ctypes.c_uint64),
-                ("prune_label_reads", ctypes.c_uint64), ("prune_tile_reads", ctypes.c_uint64)]
+                ("prune_label_reads", ctypes.c_uint64), ("prune_tile_reads", ctypes.c_uint64),
+                ("clean_seq_vertices", ctypes.c_uint64)]
 
 
 class hsdl_server_opts(ctypes.Structure):

@@ -8,12 +8,14 @@
 // The reference's GPU query (Query_mdnd/src/gpu_query.cu:5-177) launches three
 // kernels per query; here one launch answers a whole batch.
 //
-// A group of G lanes answers one query (G = 16: four queries per wavefront,
-// which suits the short cleaned lists of degree-ranked graphs).  The group
-// streams the shorter list in chunks of G labels; every lane binary-searches
-// its label's hub in the longer list, walks that hub's (<= K+1) labels in hop
-// order while h1 + h2 <= k, and keeps a running minimum.  A shuffle
-// min-reduction inside the group gives the answer.
+// One wavefront per query.  The hubs of the longer list are staged in LDS
+// with coalesced loads (64 labels per instruction); the lanes then take the
+// shorter list 64 labels at a time and binary-search each hub in LDS, walk
+// that hub's run (<= K+1 labels, hops ascending) while h1 + h2 <= k, and keep
+// a running minimum; a wave min-reduction gives the answer.  A query's
+// critical path is a few coalesced loads plus LDS probes, which keeps the
+// latency of small serving batches low; a longer list than the LDS slot holds
+// is searched in global memory instead.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -23,13 +25,15 @@
 
 namespace hsdl {
 
-constexpr int kQueryGroup = 16;
+constexpr uint32_t kQueryWaves = 4;        // queries (wavefronts) per workgroup
+constexpr uint32_t kQueryLdsHubs = 2048;   // hubs of the longer list per wave
 
-__device__ inline uint32_t lower_bound_hub(const unsigned long long *L, uint32_t len, uint32_t w) {
+template <class Getter>
+__device__ inline uint32_t lower_bound_by(Getter get, uint32_t len, uint32_t w) {
   uint32_t lo = 0, hi = len;
   while (lo < hi) {
     const uint32_t mid = (lo + hi) >> 1;
-    if (label_hub(L[mid]) < w) lo = mid + 1; else hi = mid;
+    if (get(mid) < w) lo = mid + 1; else hi = mid;
   }
   return lo;
 }
@@ -38,62 +42,58 @@ __global__ void __launch_bounds__(256) k_query(uint32_t n, const unsigned long l
                                                const unsigned long long *lab, uint64_t nq,
                                                const uint32_t *qs, const uint32_t *qt,
                                                const int8_t *qk, int32_t *out) {
-  const uint32_t lane = threadIdx.x & (kQueryGroup - 1);
-  const uint64_t q = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kQueryGroup;
-  const bool valid = q < nq;
-  uint32_t s = 0, t = 0;
-  int k = -1;
-  if (valid) {
-    s = qs[q];
-    t = qt[q];
-    k = qk[q];
+  __shared__ uint32_t hubs_all[kQueryWaves * kQueryLdsHubs];
+  const int lane = lane_id();
+  const uint32_t wv = threadIdx.x / kWave;
+  uint32_t *hubs = hubs_all + wv * kQueryLdsHubs;
+  const uint64_t q = (uint64_t)blockIdx.x * kQueryWaves + wv;
+  if (q >= nq) return;  // whole wave leaves together
+  const uint32_t s = qs[q], t = qt[q];
+  const int k = qk[q];
+  if (k < 0 || s >= n || t >= n || s == t || k == 0) {
+    if (lane == 0) out[q] = (k >= 0 && s == t && s < n) ? 0 : INT32_MAX;
+    return;
   }
-  // Cases decided without labels; they are uniform within a group.
-  bool trivial = !valid || k < 0 || s == t || k == 0 || s >= n || t >= n;
+  const unsigned long long sb = off[s], tb = off[t];
+  uint32_t ls = (uint32_t)(off[s + 1] - sb), lt = (uint32_t)(off[t + 1] - tb);
+  const unsigned long long *A = lab + sb, *B = lab + tb;
+  if (ls > lt) {  // A = shorter list, B = longer list
+    const unsigned long long *T = A; A = B; B = T;
+    const uint32_t tl = ls; ls = lt; lt = tl;
+  }
+  const uint32_t kk = (uint32_t)k;
+  const bool staged = lt <= kQueryLdsHubs;
+  if (staged) {
+    for (uint32_t i = lane; i < lt; i += kWave) hubs[i] = label_hub(B[i]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   uint32_t best = kInf32;
-  if (!trivial) {
-    const unsigned long long sb = off[s], tb = off[t];
-    uint32_t ls = (uint32_t)(off[s + 1] - sb), lt = (uint32_t)(off[t + 1] - tb);
-    const unsigned long long *A = lab + sb, *B = lab + tb;
-    if (ls > lt) {  // stream the shorter list
-      const unsigned long long *T = A; A = B; B = T;
-      uint32_t tl = ls; ls = lt; lt = tl;
-    }
-    const uint32_t kk = (uint32_t)k;
-    for (uint32_t j = lane; j < ls; j += kQueryGroup) {
-      const unsigned long long a = A[j];
-      const uint32_t w = label_hub(a), h1 = label_hop(a), d1 = label_dist(a);
-      if (h1 > kk) continue;
-      for (uint32_t p = lower_bound_hub(B, lt, w); p < lt; ++p) {
-        const unsigned long long b = B[p];
-        if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend within a hub
-        const uint32_t dsum = d1 + label_dist(b);
-        best = dsum < best ? dsum : best;
-      }
+  for (uint32_t j = lane; j < ls; j += kWave) {
+    const unsigned long long a = A[j];
+    const uint32_t w = label_hub(a), h1 = label_hop(a), d1 = label_dist(a);
+    if (h1 > kk) continue;
+    uint32_t p = staged ? lower_bound_by([&](uint32_t i) { return hubs[i]; }, lt, w)
+                        : lower_bound_by([&](uint32_t i) { return label_hub(B[i]); }, lt, w);
+    for (; p < lt; ++p) {
+      if ((staged ? hubs[p] : label_hub(B[p])) != w) break;
+      const unsigned long long b = B[p];
+      if (h1 + label_hop(b) > kk) break;  // hops ascend within a hub
+      const uint32_t dsum = d1 + label_dist(b);
+      best = dsum < best ? dsum : best;
     }
   }
-#pragma unroll
-  for (int o = kQueryGroup / 2; o > 0; o >>= 1) {
-    const uint32_t other = __shfl_xor(best, o, kQueryGroup);
-    best = other < best ? other : best;
-  }
-  if (valid && lane == 0) {
-    int32_t r;
-    if (k < 0 || s >= n || t >= n) r = INT32_MAX;
-    else if (s == t) r = 0;
-    else if (k == 0) r = INT32_MAX;
-    else r = (best == kInf32) ? INT32_MAX : (int32_t)best;
-    out[q] = r;
-  }
+  best = wave_min_u32(best);
+  if (lane == 0) out[q] = (best == kInf32) ? INT32_MAX : (int32_t)best;
 }
 
 int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,
                  const int8_t *d_k, int32_t *d_out, hipStream_t st) {
   if (nq == 0) return HSDL_OK;
-  const uint64_t threads = (uint64_t)nq * kQueryGroup;
-  const uint64_t blocks = (threads + 255) / 256;
+  const uint64_t blocks = (nq + kQueryWaves - 1) / kQueryWaves;
   if (blocks > 0x7FFFFFFFull) return fail(HSDL_EINVAL, "query batch too large for one launch");
-  hipLaunchKernelGGL(k_query, dim3((uint32_t)blocks), dim3(256), 0, st, idx->n,
+  hipLaunchKernelGGL(k_query, dim3((uint32_t)blocks), dim3(kQueryWaves * kWave), 0, st, idx->n,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
                      (uint64_t)nq, d_s, d_t, d_k, d_out);
   HSDL_HIP(hipGetLastError());

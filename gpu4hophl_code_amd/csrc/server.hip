@@ -25,13 +25,14 @@
 
 namespace hsdl {
 
+// A slot's requests travel as one packed block [s: 4n B][t: 4n B][k: n B]
+// (one host-to-device copy per batch).
 struct Slot {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
-  uint32_t *h_s = nullptr, *h_t = nullptr;
-  int8_t *h_k = nullptr;
+  uint8_t *h_req = nullptr;
   int32_t *h_out = nullptr;
-  DevBuf d_s, d_t, d_k, d_out;
+  DevBuf d_req, d_out;
   std::vector<uint64_t> ids, arrival;
   size_t count = 0;
   bool busy = false;
@@ -70,11 +71,10 @@ namespace hsdl {
 
 static void launch_slot(hsdl_server *srv, Slot &sl) {
   const size_t n = sl.count;
-  (void)hipMemcpyAsync(sl.d_s.p, sl.h_s, n * 4, hipMemcpyHostToDevice, sl.st);
-  (void)hipMemcpyAsync(sl.d_t.p, sl.h_t, n * 4, hipMemcpyHostToDevice, sl.st);
-  (void)hipMemcpyAsync(sl.d_k.p, sl.h_k, n, hipMemcpyHostToDevice, sl.st);
-  if (launch_query(srv->idx, n, sl.d_s.as<uint32_t>(), sl.d_t.as<uint32_t>(), sl.d_k.as<int8_t>(),
-                   sl.d_out.as<int32_t>(), sl.st) != HSDL_OK)
+  (void)hipMemcpyAsync(sl.d_req.p, sl.h_req, n * 9, hipMemcpyHostToDevice, sl.st);
+  const uint8_t *d = sl.d_req.as<uint8_t>();
+  if (launch_query(srv->idx, n, (const uint32_t *)d, (const uint32_t *)(d + 4 * n),
+                   (const int8_t *)(d + 8 * n), sl.d_out.as<int32_t>(), sl.st) != HSDL_OK)
     srv->error = HSDL_EDEVICE;
   (void)hipMemcpyAsync(sl.h_out, sl.d_out.p, n * 4, hipMemcpyDeviceToHost, sl.st);
   (void)hipEventRecord(sl.done, sl.st);
@@ -110,10 +110,15 @@ static void worker_loop(hsdl_server *srv) {
       sl.count = 0;
       progressed = true;
     }
-    // 2. admit a batch into a free slot
+    // 2. admit a batch into a free slot.  A batch closes when it is full,
+    // when its oldest request has waited max_wait, or at once while fewer
+    // than two batches are on the device (an idle GPU does not wait).
     Slot *free_slot = nullptr;
-    for (Slot &sl : srv->slots)
-      if (!sl.busy) { free_slot = &sl; break; }
+    uint32_t busy = 0;
+    for (Slot &sl : srv->slots) {
+      if (sl.busy) ++busy;
+      else if (!free_slot) free_slot = &sl;
+    }
     if (free_slot) {
       std::unique_lock<std::mutex> g(srv->mu);
       const uint64_t pending = srv->q_tail - srv->q_head;
@@ -122,16 +127,19 @@ static void worker_loop(hsdl_server *srv) {
       else if (pending > 0) {
         const uint64_t oldest = srv->q_arr[srv->q_head % cap];
         const uint64_t now = now_ns();
-        if (now >= oldest + wait_ns || srv->stop.load()) close = true;
+        if (busy < 2 || now >= oldest + wait_ns || srv->stop.load()) close = true;
       }
       if (close) {
         const size_t n = (size_t)std::min<uint64_t>(pending, srv->o.max_batch);
         Slot &sl = *free_slot;
+        uint32_t *hs = (uint32_t *)sl.h_req;
+        uint32_t *ht = (uint32_t *)(sl.h_req + 4 * n);
+        int8_t *hk = (int8_t *)(sl.h_req + 8 * n);
         for (size_t i = 0; i < n; ++i) {
           const uint64_t p = (srv->q_head + i) % cap;
-          sl.h_s[i] = srv->q_s[p];
-          sl.h_t[i] = srv->q_t[p];
-          sl.h_k[i] = srv->q_k[p];
+          hs[i] = srv->q_s[p];
+          ht[i] = srv->q_t[p];
+          hk[i] = srv->q_k[p];
           sl.ids[i] = srv->q_head + i;
           sl.arrival[i] = srv->q_arr[p];
         }
@@ -192,16 +200,13 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
   for (Slot &sl : srv->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_s, o.max_batch * 4ull, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_t, o.max_batch * 4ull, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_k, o.max_batch * 1ull, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_req, o.max_batch * 9ull, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_out, o.max_batch * 4ull, hipHostMallocDefault);
     if (e != hipSuccess) {
       hsdl_server_destroy(srv);
       return hip_fail(e, "server slot setup", __FILE__, __LINE__);
     }
-    if ((rc = sl.d_s.alloc(o.max_batch * 4ull)) || (rc = sl.d_t.alloc(o.max_batch * 4ull)) ||
-        (rc = sl.d_k.alloc(o.max_batch * 1ull)) || (rc = sl.d_out.alloc(o.max_batch * 4ull))) {
+    if ((rc = sl.d_req.alloc(o.max_batch * 9ull)) || (rc = sl.d_out.alloc(o.max_batch * 4ull))) {
       hsdl_server_destroy(srv);
       return rc;
     }
@@ -282,9 +287,7 @@ void hsdl_server_destroy(hsdl_server *srv) {
     if (sl.st) (void)hipStreamSynchronize(sl.st);
     if (sl.done) (void)hipEventDestroy(sl.done);
     if (sl.st) (void)hipStreamDestroy(sl.st);
-    if (sl.h_s) (void)hipHostFree(sl.h_s);
-    if (sl.h_t) (void)hipHostFree(sl.h_t);
-    if (sl.h_k) (void)hipHostFree(sl.h_k);
+    if (sl.h_req) (void)hipHostFree(sl.h_req);
     if (sl.h_out) (void)hipHostFree(sl.h_out);
   }
   delete srv;

@@ -157,6 +157,7 @@ typedef struct hsdl_stats {
   uint64_t prune_launches;
   uint64_t prune_label_reads;       /* labels of L(y) streamed by the prune */
   uint64_t prune_tile_reads;        /* labels of L(s) staged into tiles     */
+  uint64_t clean_seq_vertices;      /* vertices the clean walked in order   */
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 
