@@ -52,7 +52,8 @@ class hsdl_opts(ctypes.Structure):
                 ("tile", ctypes.c_uint32), ("hbm_budget", ctypes.c_uint64),
                 ("deadline_s", ctypes.c_double), ("device", ctypes.c_int32),
                 ("pool_init_labels", ctypes.c_uint64), ("arena_init_words", ctypes.c_uint64),
-                ("kernel_timing", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+                ("kernel_timing", ctypes.c_uint32), ("lds_tile_slots", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class hsdl_stats(ctypes.Structure):
@@ -316,10 +317,11 @@ class Index:
 
 
 def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, device=-1,
-          pool_init_labels=0, arena_init_words=0, kernel_timing=False):
+          pool_init_labels=0, arena_init_words=0, kernel_timing=False, lds_tile_slots=0):
     o = hsdl_opts()
     lib().hsdl_opts_init(ctypes.byref(o))
     o.kernel_timing = 1 if kernel_timing else 0
+    o.lds_tile_slots = lds_tile_slots
     o.pool_init_labels = pool_init_labels
     o.arena_init_words = arena_init_words
     o.mode = mode
@@ -334,11 +336,12 @@ def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, 
 def hop_constrained_two_hop_labels_generation(graph, K, mode=MODE_BATCHED, hub_batch=0, tile=0,
                                               hbm_budget=0, deadline_s=0.0, device=-1,
                                               clean=False, pool_init_labels=0,
-                                              arena_init_words=0, kernel_timing=False):
+                                              arena_init_words=0, kernel_timing=False,
+                                              lds_tile_slots=0):
     """Generates the K-hop label index of `graph` on the GPU (uncleaned unless
     clean=True, which also applies hop_constrained_clean_L)."""
     o = _opts(mode, hub_batch, tile, hbm_budget, deadline_s, device, pool_init_labels,
-              arena_init_words, kernel_timing)
+              arena_init_words, kernel_timing, lds_tile_slots)
     h = _vp()
     _check(lib().hsdl_generate(graph.h, K, ctypes.byref(o), ctypes.byref(h)))
     idx = Index(h)

@@ -16,11 +16,14 @@
 //     wavefront takes all hub groups of v at once: L(v)'s hubs sit in LDS,
 //     64-label windows of L(v) yield up to 64 group heads (ballot on hub
 //     changes), the heads' lists L(u) are load-balanced over the lanes by a
-//     register prefix sum, and every element (w,h1,d1) of L(u) binary-searches
-//     w in the LDS hubs and tests its pairs against the group's labels.
-//  2. k_clean_seq: vertices where pass 1 would remove something (and the few
-//     whose lists exceed the LDS copy) are walked group by group in order
-//     with keep bits in LDS, exactly as the reference does.
+//     register prefix sum, and every element (w,h1,d1) of L(u) that could
+//     still remove a group label (h1 + 1 <= the group's largest hop and
+//     d1 <= its largest dist; hop 0 belongs to the self label only) binary-
+//     searches w in the LDS hubs (in global memory for lists longer than the
+//     LDS copy) and tests its pairs against the group's labels.
+//  2. k_clean_seq: vertices where pass 1 would remove something are walked
+//     group by group in order with keep bits in LDS, exactly as the
+//     reference does.
 // The reference's GPU clean (Hybrid_Generation_Clean/src/label_clean.cu:73-164)
 // uses a dense per-vertex hash of all labels instead; the kept-set form is
 // the normative one.
@@ -63,98 +66,123 @@ __device__ inline uint32_t lb_u32(const uint32_t *h, uint32_t lo, uint32_t hi, u
 }
 
 // ---------------------------------------------------------------- pass 1
-__global__ void __launch_bounds__(256) k_clean_all(uint32_t n, uint32_t K,
-                                                   const unsigned long long *off,
+// L(v)'s hubs are found through an LDS skip index: samp[i] = hub of label
+// i*stride (stride 1 = every label for lists up to kAllHubsPerWave).  A lower
+// bound searches LDS for the block and then at most log2(stride) labels in
+// global memory.
+__device__ inline uint32_t lb_skip(const uint32_t *samp, uint32_t stride,
+                                   const unsigned long long *Lv, uint32_t end, uint32_t w) {
+  const uint32_t ns = (end + stride - 1) / stride;  // samples below `end`
+  const uint32_t si = lb_u32(samp, 0, ns, w);
+  const uint32_t lo = si == 0 ? 0 : (si - 1) * stride + 1;
+  const uint32_t hi = si < ns ? si * stride : end;
+  return lb_hub(Lv, lo, hi, w);
+}
+
+// Returns true when the all-earlier predicate removes some label of v.
+__device__ bool any_removal(const unsigned long long *off, const unsigned long long *lab,
+                            const unsigned long long *Lv, uint32_t len, const uint32_t *samp,
+                            uint32_t stride) {
+  const int lane = lane_id();
+  bool removed = false;
+  for (uint32_t p0 = 0; p0 < len && !removed; p0 += kWave) {
+    // group heads in this window of L(v)
+    const uint32_t p = p0 + lane;
+    const bool head = p < len && (p == 0 || label_hub(Lv[p]) != label_hub(Lv[p - 1]));
+    uint32_t u = 0, glen = 0, maxh = 0, maxd = 0;
+    unsigned long long ub = 0;
+    uint32_t ulen = 0;
+    if (head) {
+      u = label_hub(Lv[p]);
+      while (p + glen < len && label_hub(Lv[p + glen]) == u) {
+        const unsigned long long g = Lv[p + glen];
+        maxh = max(maxh, label_hop(g));
+        maxd = max(maxd, label_dist(g));
+        ++glen;
+      }
+      ub = off[u];
+      ulen = (uint32_t)(off[u + 1] - ub);
+    }
+    const uint32_t incl = wave_incl_scan_u32(ulen);
+    const uint32_t excl = incl - ulen;
+    const uint32_t total = __shfl(incl, kWave - 1, kWave);
+    bool hit = false;
+    for (uint32_t jb = 0; jb < total; jb += kWave) {
+      const uint32_t j = jb + lane;
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1) {
+        const uint32_t e = __shfl(excl, k + step, kWave);
+        if (e <= j) k += step;
+      }
+      const uint32_t ku = __shfl(u, k, kWave);
+      const uint32_t kglen = __shfl(glen, k, kWave);
+      const uint32_t kmaxh = __shfl(maxh, k, kWave);
+      const uint32_t kmaxd = __shfl(maxd, k, kWave);
+      const uint32_t kex = __shfl(excl, k, kWave);
+      const unsigned long long kub = __shfl(ub, k, kWave);
+      const uint32_t gs = p0 + (uint32_t)k;  // group start in L(v)
+      if (j >= total) continue;
+      const unsigned long long l = lab[kub + (j - kex)];
+      const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
+      if (w < ku) {
+        // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
+        if (h1 + 1 > kmaxh || d1 > kmaxd) continue;
+        for (uint32_t q = lb_skip(samp, stride, Lv, gs, w); q < gs; ++q) {
+          const unsigned long long e2 = Lv[q];
+          if (label_hub(e2) != w) break;
+          const uint32_t x = h1 + label_hop(e2), sd = d1 + label_dist(e2);
+          for (uint32_t t = 0; t < kglen; ++t) {
+            const unsigned long long g = Lv[gs + t];
+            if (x <= label_hop(g) && sd <= label_dist(g)) hit = true;
+          }
+        }
+      } else if (w == ku) {
+        // through hub u: earlier labels of the group against later ones
+        for (uint32_t t1 = 0; t1 < kglen; ++t1) {
+          const unsigned long long g1 = Lv[gs + t1];
+          for (uint32_t t2 = t1 + 1; t2 < kglen; ++t2) {
+            const unsigned long long g2 = Lv[gs + t2];
+            if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
+              hit = true;
+          }
+        }
+      }
+    }
+    removed = __ballot(hit) != 0;
+  }
+  return removed;
+}
+
+__global__ void __launch_bounds__(256) k_clean_all(uint32_t n, const unsigned long long *off,
                                                    const unsigned long long *lab, uint8_t *keep,
                                                    uint32_t *new_len, uint32_t *redo,
-                                                   uint32_t *redo_cnt) {
+                                                   uint32_t *redo_cnt, uint32_t lds_hubs) {
   __shared__ uint32_t hubs_all[kCleanWaves * kAllHubsPerWave];
-  __shared__ uint32_t rm_all[kCleanWaves];
   const int lane = lane_id();
   const uint32_t wv = threadIdx.x / kWave;
   uint32_t *hubs = hubs_all + wv * kAllHubsPerWave;
-  uint32_t *rm = rm_all + wv;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t wave = blockIdx.x * kCleanWaves + wv;
   const uint32_t nwaves = gridDim.x * kCleanWaves;
   for (uint32_t v = wave; v < n; v += nwaves) {
     const unsigned long long vb = off[v];
     const uint32_t len = (uint32_t)(off[v + 1] - vb);
     const unsigned long long *Lv = lab + vb;
-    if (len > kAllHubsPerWave) {
-      if (lane == 0) redo[atomicAdd(redo_cnt, 1u)] = v;
+    if (len == 0) {
+      if (lane == 0) new_len[v] = 0;
       continue;
     }
-    for (uint32_t i = lane; i < len; i += kWave) hubs[i] = label_hub(Lv[i]);
-    if (lane == 0) *rm = 0;
+    const uint32_t stride = (len + lds_hubs - 1) / lds_hubs;
+    for (uint32_t i = lane; i * stride < len; i += kWave) hubs[i] = label_hub(Lv[(uint64_t)i * stride]);
     wave_lds_sync();
-    for (uint32_t p0 = 0; p0 < len; p0 += kWave) {
-      // group heads in this window of L(v)
-      const uint32_t p = p0 + lane;
-      const bool head = p < len && (p == 0 || hubs[p] != hubs[p - 1]);
-      uint32_t u = 0, glen = 0;
-      unsigned long long ub = 0;
-      uint32_t ulen = 0;
-      if (head) {
-        u = hubs[p];
-        glen = 1;
-        while (p + glen < len && hubs[p + glen] == u) ++glen;
-        ub = off[u];
-        ulen = (uint32_t)(off[u + 1] - ub);
-      }
-      const uint32_t incl = wave_incl_scan_u32(ulen);
-      const uint32_t excl = incl - ulen;
-      const uint32_t total = __shfl(incl, kWave - 1, kWave);
-      for (uint32_t jb = 0; jb < total; jb += kWave) {
-        const uint32_t j = jb + lane;
-        int k = 0;
-#pragma unroll
-        for (int step = 32; step > 0; step >>= 1) {
-          const uint32_t e = __shfl(excl, k + step, kWave);
-          if (e <= j) k += step;
-        }
-        const uint32_t ku = __shfl(u, k, kWave);
-        const uint32_t kglen = __shfl(glen, k, kWave);
-        const uint32_t kex = __shfl(excl, k, kWave);
-        const unsigned long long kub = __shfl(ub, k, kWave);
-        const uint32_t gs = p0 + (uint32_t)k;  // group start in L(v)
-        if (j >= total) continue;
-        const unsigned long long l = lab[kub + (j - kex)];
-        const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
-        uint32_t hit = 0;  // bits of group labels this element removes
-        if (w < ku) {
-          for (uint32_t q = lb_u32(hubs, 0, gs, w); q < gs && hubs[q] == w; ++q) {
-            const unsigned long long e2 = Lv[q];
-            const uint32_t x = h1 + label_hop(e2), s = d1 + label_dist(e2);
-            for (uint32_t t = 0; t < kglen; ++t) {
-              const unsigned long long g = Lv[gs + t];
-              if (x <= label_hop(g) && s <= label_dist(g)) hit |= 1u << t;
-            }
-          }
-        } else if (w == ku) {
-          // through hub u: earlier labels of the group against later ones
-          for (uint32_t t1 = 0; t1 < kglen; ++t1) {
-            const unsigned long long g1 = Lv[gs + t1];
-            for (uint32_t t2 = t1 + 1; t2 < kglen; ++t2) {
-              const unsigned long long g2 = Lv[gs + t2];
-              if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
-                hit |= 1u << t2;
-            }
-          }
-        }
-        if (hit) atomicOr(rm, 1u);
-      }
-    }
+    const bool removed = any_removal(off, lab, Lv, len, hubs, stride);
     wave_lds_sync();
-    const bool removed = *rm != 0;
     if (removed) {
       if (lane == 0) redo[atomicAdd(redo_cnt, 1u)] = v;
     } else {
       for (uint32_t i = lane; i < len; i += kWave) keep[vb + i] = 1;
       if (lane == 0) new_len[v] = len;
     }
-    (void)lt_mask;
-    wave_lds_sync();
   }
 }
 
@@ -308,9 +336,10 @@ int clean_index(hsdl_index *idx) {
   const int grid = std::max(1, cus * 8);
   uint32_t *cnt = ctr.as<uint32_t>();
   hipLaunchKernelGGL(k_clean_all, dim3(grid), dim3(kCleanWaves * kWave), 0, st, n,
-                     (uint32_t)idx->K, idx->off.as<unsigned long long>(),
+                     idx->off.as<unsigned long long>(),
                      idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
-                     new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt);
+                     new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
+                     env_u32("HSDL_CLEAN_LDS_HUBS", kAllHubsPerWave, 1, kAllHubsPerWave));
   hipLaunchKernelGGL(k_clean_seq, dim3(grid), dim3(kCleanWaves * kWave), 0, st, (uint32_t)idx->K,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
                      keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,

@@ -45,6 +45,14 @@ int hip_fail(hipError_t e, const char *what, const char *file, int line);
 // when no GPU is visible.
 int use_device(int32_t device, int *out_device);
 
+// Integer tuning knob from the environment, clamped to [lo, hi].
+inline uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
+  const char *v = getenv(name);
+  if (!v || !*v) return dflt;
+  const unsigned long x = strtoul(v, nullptr, 10);
+  return (uint32_t)(x < lo ? lo : (x > hi ? hi : x));
+}
+
 // Device buffer owned by RAII.
 struct DevBuf {
   void *p = nullptr;

@@ -323,68 +323,84 @@ __global__ void __launch_bounds__(256) k_group(GenArgs a) {
 }
 
 // ----------------------------------------------------------------- prune
-__device__ inline uint32_t hash_slot(uint32_t key, uint32_t mask) {
-  return ((key * 0x9E3779B1u) >> 7) & mask;
+// The tile of L(s) is an open-addressing hash: keys[c] = hub and
+// vals[c*(K+1)+x] = min dist over labels (hub, hb, db) of L(s) with hb <= x.
+// V is u16 when every label distance fits (K * max weight < 65535), halving
+// the tile; the empty value is V's maximum.  Capacities need not be powers of
+// two: the home slot is the high word of (mixed key) * cap.
+template <class V> struct TileVal;
+template <> struct TileVal<uint16_t> { static constexpr uint32_t kEmpty = 0xFFFFu; };
+template <> struct TileVal<uint32_t> { static constexpr uint32_t kEmpty = 0xFFFFFFFFu; };
+
+__device__ inline uint32_t hash_slot(uint32_t key, uint32_t cap) {
+  return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * cap) >> 32);
 }
 
 __device__ inline uint32_t tile_cap_for(uint32_t len) {
-  const uint32_t want = len + (len >> 1) + 1;  // load factor <= 2/3
-  uint32_t c = 64;
-  while (c < want) c <<= 1;
-  return c;
+  return len + (len >> 1) + 1;  // load factor <= 2/3
 }
 
-// Build the hash tile of L(s): keys[c] = hub, vals[c*(K+1)+x] = min dist over
-// labels (hub, hb, db) of L(s) with hb <= x.
-__device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, uint32_t *vals,
-                           uint32_t cap) {
+// u32 words a tile of `cap` slots takes (keys + values, value rows padded)
+template <class V>
+__device__ __host__ inline uint64_t tile_words(uint32_t cap, uint32_t K1) {
+  return (uint64_t)cap + ((uint64_t)cap * K1 * sizeof(V) + 3) / 4;
+}
+
+// LDS atomicMin on 16-bit values: CAS on the containing word.
+__device__ inline void atomic_min_v(uint16_t *p, uint32_t v) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(reinterpret_cast<uintptr_t>(p) & ~(uintptr_t)3);
+  const uint32_t sh = (reinterpret_cast<uintptr_t>(p) & 2) ? 16 : 0;
+  uint32_t old = *w;
+  for (;;) {
+    const uint32_t cur = (old >> sh) & 0xFFFFu;
+    if (cur <= v) return;
+    const uint32_t nw = (old & ~(0xFFFFu << sh)) | (v << sh);
+    const uint32_t got = atomicCAS(w, old, nw);
+    if (got == old) return;
+    old = got;
+  }
+}
+__device__ inline void atomic_min_v(uint32_t *p, uint32_t v) { atomicMin(p, v); }
+
+template <class V>
+__device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals, uint32_t cap) {
   const uint32_t K1 = a.K + 1;
   for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) keys[i] = kInf32;
-  for (uint32_t i = threadIdx.x; i < cap * K1; i += blockDim.x) vals[i] = kInf32;
+  for (uint32_t i = threadIdx.x; i < cap * K1; i += blockDim.x) vals[i] = (V)TileVal<V>::kEmpty;
   __syncthreads();
   const unsigned long long st = a.seg_start[s];
   const uint32_t len = a.seg_len[s];
-  const uint32_t mask = cap - 1;
   for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
     const unsigned long long l = a.pool[st + i];
     const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
-    uint32_t slot = hash_slot(hub, mask);
+    uint32_t slot = hash_slot(hub, cap);
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
       if (prev == kInf32 || prev == hub) break;
-      slot = (slot + 1) & mask;
+      slot = (slot + 1 == cap) ? 0 : slot + 1;
     }
-    for (uint32_t x = hb; x < K1; ++x) atomicMin(&vals[slot * K1 + x], db);
+    for (uint32_t x = hb; x < K1; ++x) atomic_min_v(&vals[slot * K1 + x], db);
   }
   __syncthreads();
 }
 
-// kGlobal: the tile lives in the global arena; its slots were filled by
-// atomics, so reads are agent-scope atomic loads (coherent in L2).
-template <bool kGlobal>
-__device__ inline uint32_t tile_ld(const uint32_t *p) {
-  if (kGlobal) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return *p;
-}
-
-template <bool kGlobal>
-__device__ inline uint32_t tile_lookup(const uint32_t *keys, const uint32_t *vals, uint32_t cap,
+template <class V>
+__device__ inline uint32_t tile_lookup(const uint32_t *keys, const V *vals, uint32_t cap,
                                        uint32_t K1, uint32_t hub, uint32_t x) {
-  const uint32_t mask = cap - 1;
-  uint32_t slot = hash_slot(hub, mask);
+  uint32_t slot = hash_slot(hub, cap);
   for (;;) {
-    const uint32_t k = tile_ld<kGlobal>(&keys[slot]);
-    if (k == hub) return tile_ld<kGlobal>(&vals[slot * K1 + x]);
-    if (k == kInf32) return kInf32;
-    slot = (slot + 1) & mask;
+    const uint32_t k = keys[slot];
+    if (k == hub) return vals[slot * K1 + x];
+    if (k == kInf32) return TileVal<V>::kEmpty;
+    slot = (slot + 1 == cap) ? 0 : slot + 1;
   }
 }
 
 // True when some common hub gives da + db <= dy (candidate pruned); `reads`
 // receives the number of labels of L(y) loaded.
-template <bool kGlobal>
+template <class V>
 __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
-                               const uint32_t *vals, uint32_t cap, uint32_t &reads) {
+                               const V *vals, uint32_t cap, uint32_t &reads) {
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
   const unsigned long long st = a.seg_start[y];
@@ -395,8 +411,8 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
       const unsigned long long l = a.pool[st + j + lane];
       const uint32_t ha = label_hop(l);
       if (ha <= a.h) {
-        const uint32_t v = tile_lookup<kGlobal>(keys, vals, cap, K1, label_hub(l), a.h - ha);
-        if (v != kInf32 && label_dist(l) + v <= dy) f = true;
+        const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, label_hub(l), a.h - ha);
+        if (v != TileVal<V>::kEmpty && label_dist(l) + v <= dy) f = true;
       }
     }
     if (__ballot(f)) {
@@ -408,6 +424,7 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
   return false;
 }
 
+template <class V>
 __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base;
@@ -441,18 +458,19 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
     const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
     const uint32_t *cands = a.cgroup + a.hub_off[b];
     const uint32_t cap = tile_cap_for(a.seg_len[s]);
-    uint32_t *keys, *vals;
+    uint32_t *keys;
+    V *vals;
     const bool in_lds = cap <= a.lds_slots;
     if (in_lds) {
       keys = lds;
-      vals = lds + cap;
+      vals = reinterpret_cast<V *>(lds + cap);
       if (tile_b != b) {
-        build_tile(a, s, keys, vals, cap);
+        build_tile<V>(a, s, keys, vals, cap);
         tile_b = b;
         tile_reads += a.seg_len[s];
       }
     } else {
-      if ((uint64_t)cap * (K1 + 1) > gwords) {
+      if (tile_words<V>(cap, K1) > gwords) {
         // arena too small: the host rolls the batch back, grows the arena
         // and runs the batch again
         if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, kOvfArena);
@@ -460,8 +478,11 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
         continue;
       }
       keys = gkeys;
-      vals = gkeys + cap;
-      build_tile(a, s, keys, vals, cap);
+      vals = reinterpret_cast<V *>(gkeys + cap);
+      build_tile<V>(a, s, keys, vals, cap);
+      // the tile was written through L2 (atomics); drop stale L1 lines of
+      // this workgroup's arena slice before plain loads read it
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tile_b = kInf32;
       tile_reads += a.seg_len[s];
     }
@@ -470,8 +491,7 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
       const uint64_t slot = (uint64_t)b * a.n + y;
       const uint32_t dy = a.cand_d[slot];
       uint32_t reads = 0;
-      const bool pruned = in_lds ? witness_exists<false>(a, y, dy, keys, vals, cap, reads)
-                                 : witness_exists<true>(a, y, dy, keys, vals, cap, reads);
+      const bool pruned = witness_exists<V>(a, y, dy, keys, vals, cap, reads);
       wave_reads += reads;
       if (lane == 0) {
         a.cand_d[slot] = kInf32;  // slot ready for the next round
@@ -640,6 +660,7 @@ struct Gen {
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
   size_t lds_bytes = 0;
+  bool small_vals = false;  // 16-bit tile values
   hipStream_t st = nullptr;
   bool verbose = false;
   // kernel timing (events around every expand and prune launch of a batch)
@@ -745,7 +766,10 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
-    hipLaunchKernelGGL(k_prune, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
+    if (g.small_vals)
+      hipLaunchKernelGGL(k_prune<uint16_t>, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
+    else
+      hipLaunchKernelGGL(k_prune<uint32_t>, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
@@ -831,11 +855,16 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))))
     return rc;
 
-  // LDS tile per workgroup
+  // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
   const uint32_t K1 = K + 1;
+  g.small_vals = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1) < 0xFFFFull;
+  const uint64_t lds_words = 60 * 1024 / 4;  // + ~3 KB of static LDS
   uint32_t lds_slots = 64;
-  while ((lds_slots * 2) * (K1 + 1) * 4 <= 56 * 1024) lds_slots *= 2;
-  g.lds_bytes = (size_t)lds_slots * (K1 + 1) * 4;
+  while ((g.small_vals ? tile_words<uint16_t>(lds_slots + 64, K1)
+                       : tile_words<uint32_t>(lds_slots + 64, K1)) <= lds_words)
+    lds_slots += 64;
+  if (opts->lds_tile_slots) lds_slots = std::min(lds_slots, opts->lds_tile_slots);
+  g.lds_bytes = (size_t)lds_words * 4;
 
   GenArgs &a = g.a;
   a.n = n;

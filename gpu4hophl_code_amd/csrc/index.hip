@@ -145,6 +145,9 @@ int hsdl_index_from_labels(uint32_t n, uint8_t K, const uint64_t *off, const uin
     for (uint64_t i = off[v] + 1; i < off[v + 1]; ++i)
       if ((lab[i] >> kDistBits) == (lab[i - 1] >> kDistBits))
         return fail(HSDL_EINVAL, "two labels with the same (hub, hop) at one vertex");
+    for (uint64_t i = off[v]; i < off[v + 1]; ++i)
+      if (label_hop(lab[i]) == 0 && (label_hub(lab[i]) != v || label_dist(lab[i]) != 0))
+        return fail(HSDL_EINVAL, "a hop-0 label must be the vertex's self label (v,0,0)");
   }
   int dev = 0;
   int rc = use_device(device, &dev);

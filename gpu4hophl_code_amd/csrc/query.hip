@@ -9,13 +9,14 @@
 // kernels per query; here one launch answers a whole batch.
 //
 // One wavefront per query.  The hubs of the longer list are staged in LDS
-// with coalesced loads (64 labels per instruction); the lanes then take the
-// shorter list 64 labels at a time and binary-search each hub in LDS, walk
-// that hub's run (<= K+1 labels, hops ascending) while h1 + h2 <= k, and keep
-// a running minimum; a wave min-reduction gives the answer.  A query's
+// with coalesced loads (64 labels per instruction) -- all of them for lists
+// up to 2048 labels, every stride-th one (a skip index) for longer lists; the
+// lanes then take the shorter list 64 labels at a time, binary-search each
+// hub in LDS (plus at most log2(stride) global probes inside the block),
+// walk that hub's run (<= K+1 labels, hops ascending) while h1 + h2 <= k and
+// keep a running minimum; a wave min-reduction gives the answer.  A query's
 // critical path is a few coalesced loads plus LDS probes, which keeps the
-// latency of small serving batches low; a longer list than the LDS slot holds
-// is searched in global memory instead.
+// latency of small serving batches low.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -41,7 +42,7 @@ __device__ inline uint32_t lower_bound_by(Getter get, uint32_t len, uint32_t w) 
 __global__ void __launch_bounds__(256) k_query(uint32_t n, const unsigned long long *off,
                                                const unsigned long long *lab, uint64_t nq,
                                                const uint32_t *qs, const uint32_t *qt,
-                                               const int8_t *qk, int32_t *out) {
+                                               const int8_t *qk, int32_t *out, uint32_t lds_hubs) {
   __shared__ uint32_t hubs_all[kQueryWaves * kQueryLdsHubs];
   const int lane = lane_id();
   const uint32_t wv = threadIdx.x / kWave;
@@ -61,25 +62,33 @@ __global__ void __launch_bounds__(256) k_query(uint32_t n, const unsigned long l
     const unsigned long long *T = A; A = B; B = T;
     const uint32_t tl = ls; ls = lt; lt = tl;
   }
-  const uint32_t kk = (uint32_t)k;
-  const bool staged = lt <= kQueryLdsHubs;
-  if (staged) {
-    for (uint32_t i = lane; i < lt; i += kWave) hubs[i] = label_hub(B[i]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (ls == 0) {  // an empty list shares no hub
+    if (lane == 0) out[q] = INT32_MAX;
+    return;
   }
+  const uint32_t kk = (uint32_t)k;
+  // LDS skip index of B: every stride-th hub (stride 1 = all of B for lists
+  // up to kQueryLdsHubs).  A lookup searches LDS for the block, then at most
+  // log2(stride) labels of B in global memory.
+  const uint32_t stride = (lt + lds_hubs - 1) / lds_hubs;
+  const uint32_t ns = (lt + stride - 1) / stride;
+  for (uint32_t i = lane; i < ns; i += kWave) hubs[i] = label_hub(B[(uint64_t)i * stride]);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   uint32_t best = kInf32;
   for (uint32_t j = lane; j < ls; j += kWave) {
     const unsigned long long a = A[j];
     const uint32_t w = label_hub(a), h1 = label_hop(a), d1 = label_dist(a);
     if (h1 > kk) continue;
-    uint32_t p = staged ? lower_bound_by([&](uint32_t i) { return hubs[i]; }, lt, w)
-                        : lower_bound_by([&](uint32_t i) { return label_hub(B[i]); }, lt, w);
+    // first sample >= w, then the first position >= w inside its block
+    const uint32_t si = lower_bound_by([&](uint32_t i) { return hubs[i]; }, ns, w);
+    const uint32_t lo = si == 0 ? 0 : (si - 1) * stride + 1;
+    const uint32_t hi = si < ns ? si * stride : lt;
+    uint32_t p = lo + lower_bound_by([&](uint32_t i) { return label_hub(B[lo + i]); }, hi - lo, w);
     for (; p < lt; ++p) {
-      if ((staged ? hubs[p] : label_hub(B[p])) != w) break;
       const unsigned long long b = B[p];
-      if (h1 + label_hop(b) > kk) break;  // hops ascend within a hub
+      if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend within a hub
       const uint32_t dsum = d1 + label_dist(b);
       best = dsum < best ? dsum : best;
     }
@@ -93,9 +102,11 @@ int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const ui
   if (nq == 0) return HSDL_OK;
   const uint64_t blocks = (nq + kQueryWaves - 1) / kQueryWaves;
   if (blocks > 0x7FFFFFFFull) return fail(HSDL_EINVAL, "query batch too large for one launch");
+  // LDS skip-index capacity per query (tuning knob HSDL_QUERY_LDS_HUBS, <= 2048)
+  const uint32_t lds_hubs = env_u32("HSDL_QUERY_LDS_HUBS", kQueryLdsHubs, 1, kQueryLdsHubs);
   hipLaunchKernelGGL(k_query, dim3((uint32_t)blocks), dim3(kQueryWaves * kWave), 0, st, idx->n,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
-                     (uint64_t)nq, d_s, d_t, d_k, d_out);
+                     (uint64_t)nq, d_s, d_t, d_k, d_out, lds_hubs);
   HSDL_HIP(hipGetLastError());
   return HSDL_OK;
 }

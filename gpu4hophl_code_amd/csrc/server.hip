@@ -212,6 +212,19 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
     }
     sl.ids.resize(o.max_batch);
     sl.arrival.resize(o.max_batch);
+    // warm the slot: the first copy and launch on a fresh stream pay one-time
+    // costs that would otherwise land on the first requests' latency
+    std::memset(sl.h_req, 0, 9);
+    sl.count = 1;
+    launch_slot(srv, sl);
+    e = hipStreamSynchronize(sl.st);
+    sl.busy = false;
+    sl.count = 0;
+    if (e != hipSuccess || srv->error) {
+      hsdl_server_destroy(srv);
+      return hip_fail(e != hipSuccess ? e : hipErrorUnknown, "server slot warm-up", __FILE__,
+                      __LINE__);
+    }
   }
   srv->worker = std::thread(worker_loop, srv);
   *out = srv;

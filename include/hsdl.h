@@ -121,7 +121,10 @@ typedef struct hsdl_opts {
                                  whose label tile exceeds LDS, 0 = automatic */
   uint32_t kernel_timing;     /* 1: time the expand and prune launches with
                                  HIP events on the build stream (stats)     */
-  uint32_t reserved[3];
+  uint32_t lds_tile_slots;    /* cap on the hub tile slots kept in LDS, 0 =
+                                 as many as 60 KB hold; larger tiles use the
+                                 global arena                               */
+  uint32_t reserved[2];
 } hsdl_opts;
 
 /* Fills opts with the defaults above. */

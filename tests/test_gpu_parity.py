@@ -58,6 +58,27 @@ def test_c1_pool_and_arena_growth_roll_back_and_redo(c1):
     same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
 
 
+def test_c1_global_arena_tiles(c1):
+    """Tiles capped at 64 LDS slots: most hubs' tiles go to the global arena
+    (which also starts too small and grows)."""
+    g, z = c1
+    idx = H.hop_constrained_two_hop_labels_generation(g, 4, lds_tile_slots=64,
+                                                      arena_init_words=1)
+    same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
+
+
+def test_wide_weights_use_32bit_tiles():
+    """K * max weight >= 65535 selects 32-bit tile values."""
+    g = H.Graph.rmat(10, 8, 4242, 1, 40000)
+    c = csr_of(g)
+    for slots in (0, 64):
+        idx = H.hop_constrained_two_hop_labels_generation(g, 3, lds_tile_slots=slots)
+        want = O.kround(c, 3)
+        same_labels(idx, want.arrays())
+        idx.clean()
+        same_labels(idx, O.clean(want).arrays())
+
+
 def test_deadline_reports_timeout(c1):
     g, _ = c1
     with pytest.raises(H.HSDLError) as e:
@@ -142,6 +163,29 @@ def test_queries_special_cases_and_bf():
             q = idx.query(np.full(c.n, src, np.uint32), np.arange(c.n, dtype=np.uint32),
                           np.full(c.n, kk, np.int8))
             assert np.array_equal(q, d)
+
+
+@pytest.mark.parametrize("lds_hubs", ["7", "64", "2048"])
+def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs):
+    """Small LDS skip indexes force strides > 1 on short lists: the clean and
+    the queries must not change."""
+    monkeypatch.setenv("HSDL_CLEAN_LDS_HUBS", lds_hubs)
+    monkeypatch.setenv("HSDL_QUERY_LDS_HUBS", lds_hubs)
+    g = H.Graph.rmat(10, 8, 99, 1, 20)
+    c = csr_of(g)
+    red = O.kround_grouped(c, 4, 32)
+    off, hub, hop, dist = red.arrays()
+    idx = H.Index.from_labels(c.n, 4, off.astype(np.uint64), hub, hop, dist)
+    idx.clean()
+    want = O.clean(red)
+    same_labels(idx, want.arrays())
+    canon = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
+    same_labels(canon, want.arrays())
+    rng = np.random.default_rng(int(lds_hubs))
+    s = rng.integers(0, c.n, 20000).astype(np.uint32)
+    t = rng.integers(0, c.n, 20000).astype(np.uint32)
+    k = rng.integers(-1, 6, 20000).astype(np.int8)
+    assert np.array_equal(canon.query(s, t, k), O.query(want, s, t, k))
 
 
 def test_export_import_save_load_roundtrip(tmp_path, c1):
