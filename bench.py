@@ -218,6 +218,17 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "k_prune", "bytes_per_launch": pr_bytes / launches,
                 "avg_launch_us": pr_t / launches * 1e6, "launches": launches}
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes on this workload (profiles/k_prune_<config>.md)
+    tpath = os.path.join(ROOT, "profiles", f"k_prune_{args.config}_k{K}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            roofline["traffic"] = json.load(f)["traffic_bytes_per_launch"]
+    # achievable copy bandwidth on this GPU (STREAM-style, read + write)
+    import ctypes
+    gbps = ctypes.c_double()
+    if H.lib().hsdl_device_copy_bench(1 << 30, 5, ctypes.byref(gbps)) == 0:
+        roofline["copy_gbps"] = round(gbps.value, 1)
 
     # queries on the device-resident cleaned index
     lens, _ = label_lengths(idx)

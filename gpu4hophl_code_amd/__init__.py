@@ -114,6 +114,7 @@ _SIGNATURES = [
     ("hsdl_server_stats_get", ctypes.c_int, [_vp, ctypes.POINTER(hsdl_server_stats)]),
     ("hsdl_server_destroy", None, [_vp]),
     ("hsdl_server_replay", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _u64p, _i32p, _u64p]),
+    ("hsdl_device_copy_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     ("hsdl_strerror", ctypes.c_char_p, [ctypes.c_int]),
     ("hsdl_last_error", ctypes.c_char_p, []),
     ("hsdl_now_ns", ctypes.c_uint64, []),

@@ -248,6 +248,13 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
 
 /* -------------------------------------------------------------- misc */
 
+/* Device copy benchmark (no counterpart in the reference): copies `bytes`
+ * between two device buffers `iters` times with 16 B vector accesses and
+ * reports read+write GB/s -- the achievable HBM bandwidth the kernels are
+ * compared with, and the known-traffic kernel PMC byte counters are
+ * calibrated against. */
+int hsdl_device_copy_bench(uint64_t bytes, uint32_t iters, double *gbps);
+
 const char *hsdl_strerror(int code);
 const char *hsdl_last_error(void);
 uint64_t hsdl_now_ns(void);

@@ -1,0 +1,355 @@
+// hsdl_compat.hpp -- the reference's C++ entry points over the HSDL C ABI.
+//
+// A driver written against rucdatascience/GPU4HopHL_code's headers calls
+//   graph_v_of_v<int>::add_edge / txt_read / txt_save
+//       (CPU_HopHL/include/graph_v_of_v/graph_v_of_v.h:92-108,215-277)
+//   graph_v_of_v_update_vertexIDs_by_degrees_large_to_small
+//       (CPU_HopHL/include/graph_v_of_v/graph_v_of_v_update_vertexIDs_by_degrees_large_to_small.h:14-40)
+//   hop_constrained_two_hop_labels_generation(graph, case_info)
+//       (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:513-632)
+//   hop_constrained_clean_L(case_info, thread_num)            (same file :422-511)
+//   hop_constrained_extract_distance(L, s, t, k)
+//       (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:144-222)
+//   gpu_query(...)                       (Query_mdnd/src/gpu_query.cu:84)
+// Including this header instead (namespace hsdl_compat) keeps those calls;
+// the work runs in libhsdl.so on the GPU.  Differences a caller can see:
+//   - parent_vertex of exported labels is 0 (the reference's uncleaned GPU
+//     export writes 0 as well, Hybrid_Generation_Clean/include/HBPLL/
+//     hop_constrained_two_hop_labels_generation.h:1483);
+//   - the degree renaming breaks degree ties by old id (the reference's
+//     std::sort leaves them unordered);
+//   - errors throw hsdl_compat::error instead of printing and exiting.
+// Link with -lhsdl (gpu4hophl_code_amd/libhsdl.so).
+#ifndef HSDL_COMPAT_HPP
+#define HSDL_COMPAT_HPP
+
+#include <climits>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "hsdl.h"
+
+namespace hsdl_compat {
+
+struct error : std::runtime_error {
+  int code;
+  error(int c, const std::string &what)
+      : std::runtime_error(what + ": " + hsdl_strerror(c) + " (" + hsdl_last_error() + ")"),
+        code(c) {}
+};
+
+inline void check(int rc, const char *what) {
+  if (rc != HSDL_OK) throw error(rc, what);
+}
+
+// ----------------------------------------------------------------- labels
+class hop_constrained_two_hop_label {
+ public:
+  int hub_vertex = 0, parent_vertex = 0, hop = 0, distance = 0;
+  hop_constrained_two_hop_label() = default;
+  hop_constrained_two_hop_label(int hv, int pv, int h, int d)
+      : hub_vertex(hv), parent_vertex(pv), hop(h), distance(d) {}
+};
+
+// ------------------------------------------------------------------ graph
+// Adjacency lists sorted by neighbour id; add_edge overwrites an existing
+// edge's weight (the reference's rule).  Converted to the library's graph on
+// demand.
+template <typename weight_type = int>
+class graph_v_of_v {
+ public:
+  std::vector<std::vector<std::pair<int, weight_type>>> ADJs;
+  graph_v_of_v() = default;
+  explicit graph_v_of_v(int n) : ADJs(n) {}
+  int size() { return (int)ADJs.size(); }
+  std::vector<std::pair<int, weight_type>> &operator[](int i) { return ADJs[i]; }
+
+  void add_edge(int e1, int e2, weight_type ec) {
+    insert(ADJs[e1], e2, ec);
+    insert(ADJs[e2], e1, ec);
+  }
+  long long edge_number() {
+    long long m = 0;
+    for (int i = 0; i < size(); ++i)
+      for (auto &e : ADJs[i])
+        if (i <= e.first) ++m;
+    return m;
+  }
+  void clear() { ADJs.clear(); }
+
+  void txt_read(const std::string &path) {
+    hsdl_graph *g = nullptr;
+    check(hsdl_graph_load_txt(path.c_str(), &g), "hsdl_graph_load_txt");
+    from_handle(g);
+    hsdl_free_graph(g);
+  }
+  void txt_save(const std::string &path) {
+    hsdl_graph *g = to_handle();
+    int rc = hsdl_graph_save_txt(g, path.c_str());
+    hsdl_free_graph(g);
+    check(rc, "hsdl_graph_save_txt");
+  }
+
+  // library graph with the same vertices and edges (caller frees)
+  hsdl_graph *to_handle() const {
+    std::vector<uint32_t> u, v;
+    std::vector<int32_t> w;
+    for (size_t i = 0; i < ADJs.size(); ++i)
+      for (auto &e : ADJs[i])
+        if ((int)i <= e.first) {
+          u.push_back((uint32_t)i);
+          v.push_back((uint32_t)e.first);
+          w.push_back((int32_t)e.second);
+        }
+    hsdl_graph *g = nullptr;
+    check(hsdl_graph_from_edges((uint32_t)ADJs.size(), u.size(), u.data(), v.data(), w.data(), 0,
+                                &g),
+          "hsdl_graph_from_edges");
+    return g;
+  }
+  void from_handle(const hsdl_graph *g) {
+    uint32_t n = 0;
+    uint64_t nnz = 0;
+    check(hsdl_graph_info(g, &n, &nnz), "hsdl_graph_info");
+    std::vector<uint64_t> off(n + 1ull);
+    std::vector<uint32_t> col(nnz);
+    std::vector<int32_t> w(nnz);
+    check(hsdl_graph_csr(g, off.data(), col.data(), w.data()), "hsdl_graph_csr");
+    ADJs.assign(n, {});
+    for (uint32_t i = 0; i < n; ++i)
+      for (uint64_t e = off[i]; e < off[i + 1]; ++e)
+        ADJs[i].push_back({(int)col[e], (weight_type)w[e]});
+  }
+
+ private:
+  static void insert(std::vector<std::pair<int, weight_type>> &a, int key, weight_type w) {
+    size_t lo = 0, hi = a.size();
+    while (lo < hi) {
+      size_t mid = (lo + hi) / 2;
+      if (a[mid].first < key) lo = mid + 1; else hi = mid;
+    }
+    if (lo < a.size() && a[lo].first == key)
+      a[lo].second = w;
+    else
+      a.insert(a.begin() + lo, {key, w});
+  }
+};
+
+template <typename weight_type>
+graph_v_of_v<weight_type> graph_v_of_v_update_vertexIDs_by_degrees_large_to_small(
+    graph_v_of_v<weight_type> &input_graph) {
+  hsdl_graph *g = input_graph.to_handle();
+  int rc = hsdl_graph_reorder_by_degree(g, nullptr);
+  graph_v_of_v<weight_type> out;
+  if (rc == HSDL_OK) out.from_handle(g);
+  hsdl_free_graph(g);
+  check(rc, "hsdl_graph_reorder_by_degree");
+  return out;
+}
+
+// ------------------------------------------------------------- case info
+// Fields of the reference's hop_constrained_case_info
+// (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:54-142) that the
+// GPU path honours; `thread_num` and the 2023WWW switch have no GPU meaning.
+class hop_constrained_case_info {
+ public:
+  int thread_num = 1;
+  int upper_k = 0;
+  bool use_rank_prune = true;
+  bool use_2023WWW_generation = false;
+  bool use_canonical_repair = true;
+  double time_initialization = 0, time_generate_labels = 0, time_sortL = 0,
+         time_canonical_repair = 0, time_total = 0;
+  long long max_bit_size = (long long)1e12;
+  double max_run_time_seconds = 1e12;
+  std::vector<std::vector<hop_constrained_two_hop_label>> L;
+  double label_size_before_canonical_repair = 0, label_size_after_canonical_repair = 0,
+         canonical_repair_remove_label_ratio = 0;
+  // the index stays resident in HBM for queries through this case_info
+  hsdl_index *index = nullptr;
+
+  hop_constrained_case_info() = default;
+  hop_constrained_case_info(const hop_constrained_case_info &) = delete;
+  hop_constrained_case_info &operator=(const hop_constrained_case_info &) = delete;
+  ~hop_constrained_case_info() { clear_labels(); }
+
+  void clear_labels() {
+    std::vector<std::vector<hop_constrained_two_hop_label>>().swap(L);
+    if (index) hsdl_free_index(index);
+    index = nullptr;
+  }
+  long long compute_label_bit_size() {
+    long long s = 0;
+    for (auto &x : L) s += (long long)(x.size() * sizeof(hop_constrained_two_hop_label));
+    return s;
+  }
+};
+
+// Copies an index's labels into vector-of-vectors form (parent_vertex = 0).
+inline void export_labels(const hsdl_index *idx,
+                          std::vector<std::vector<hop_constrained_two_hop_label>> &L) {
+  uint32_t n = 0;
+  uint8_t K = 0;
+  uint64_t total = 0;
+  check(hsdl_index_info(idx, &n, &K), "hsdl_index_info");
+  check(hsdl_label_count(idx, &total), "hsdl_label_count");
+  std::vector<uint64_t> off(n + 1ull);
+  std::vector<uint32_t> hub(total);
+  std::vector<uint8_t> hop(total);
+  std::vector<int32_t> dist(total);
+  check(hsdl_export_labels(idx, off.data(), hub.data(), hop.data(), dist.data()),
+        "hsdl_export_labels");
+  L.assign(n, {});
+  for (uint32_t v = 0; v < n; ++v) {
+    L[v].reserve(off[v + 1] - off[v]);
+    for (uint64_t i = off[v]; i < off[v + 1]; ++i)
+      L[v].emplace_back((int)hub[i], 0, (int)hop[i], (int)dist[i]);
+  }
+}
+
+// Index from vector-of-vectors labels (any per-vertex order).
+inline hsdl_index *import_labels(const std::vector<std::vector<hop_constrained_two_hop_label>> &L,
+                                 int K) {
+  std::vector<uint64_t> off(1, 0);
+  std::vector<uint32_t> hub;
+  std::vector<uint8_t> hop;
+  std::vector<int32_t> dist;
+  for (auto &lst : L) {
+    for (auto &x : lst) {
+      hub.push_back((uint32_t)x.hub_vertex);
+      hop.push_back((uint8_t)x.hop);
+      dist.push_back(x.distance);
+    }
+    off.push_back(hub.size());
+  }
+  hsdl_index *idx = nullptr;
+  check(hsdl_index_from_labels((uint32_t)L.size(), (uint8_t)K, off.data(), hub.data(), hop.data(),
+                               dist.data(), -1, &idx),
+        "hsdl_index_from_labels");
+  return idx;
+}
+
+// Generation (+ canonical repair when case_info.use_canonical_repair), on the
+// GPU, results in case_info.L sorted by (hub, hop, dist) and resident in HBM.
+template <typename weight_type>
+void hop_constrained_two_hop_labels_generation(graph_v_of_v<weight_type> &input_graph,
+                                               hop_constrained_case_info &case_info) {
+  if (case_info.upper_k <= 0 || case_info.upper_k > 15)
+    throw error(HSDL_EINVAL, "upper_k must be in 1..15 on the GPU path");
+  case_info.clear_labels();
+  hsdl_graph *g = input_graph.to_handle();
+  hsdl_opts o;
+  hsdl_opts_init(&o);
+  if (case_info.max_run_time_seconds < 1e11) o.deadline_s = case_info.max_run_time_seconds;
+  hsdl_index *idx = nullptr;
+  int rc = hsdl_generate(g, (uint8_t)case_info.upper_k, &o, &idx);
+  hsdl_free_graph(g);
+  check(rc, "hsdl_generate");
+  hsdl_stats st;
+  hsdl_index_stats(idx, &st);
+  case_info.time_generate_labels = st.t_generate_s - st.t_sort_s;
+  case_info.time_sortL = st.t_sort_s;
+  case_info.label_size_before_canonical_repair = (double)st.labels_uncleaned;
+  if (case_info.use_canonical_repair) {
+    rc = hsdl_clean(idx);
+    if (rc != HSDL_OK) {
+      hsdl_free_index(idx);
+      check(rc, "hsdl_clean");
+    }
+    hsdl_index_stats(idx, &st);
+    case_info.time_canonical_repair = st.t_clean_s;
+  }
+  uint64_t total = 0;
+  hsdl_label_count(idx, &total);
+  case_info.label_size_after_canonical_repair = (double)total;
+  case_info.canonical_repair_remove_label_ratio =
+      st.labels_uncleaned ? (double)(st.labels_uncleaned - total) / st.labels_uncleaned : 0.0;
+  case_info.time_total =
+      case_info.time_generate_labels + case_info.time_sortL + case_info.time_canonical_repair;
+  case_info.index = idx;
+  export_labels(idx, case_info.L);
+}
+
+// Canonical repair of case_info.L (any labels, e.g. from another generator).
+inline void hop_constrained_clean_L(hop_constrained_case_info &case_info, int /*thread_num*/) {
+  int K = 0;
+  for (auto &lst : case_info.L)
+    for (auto &x : lst) K = x.hop > K ? x.hop : K;
+  if (K < 1) K = 1;
+  hsdl_index *idx = import_labels(case_info.L, K);
+  int rc = hsdl_clean(idx);
+  if (rc != HSDL_OK) {
+    hsdl_free_index(idx);
+    check(rc, "hsdl_clean");
+  }
+  hsdl_stats st;
+  hsdl_index_stats(idx, &st);
+  case_info.label_size_before_canonical_repair = (double)st.labels_uncleaned;
+  case_info.label_size_after_canonical_repair = (double)st.labels_cleaned;
+  case_info.canonical_repair_remove_label_ratio =
+      st.labels_uncleaned
+          ? (double)(st.labels_uncleaned - st.labels_cleaned) / st.labels_uncleaned
+          : 0.0;
+  case_info.time_canonical_repair = st.t_clean_s;
+  if (case_info.index) hsdl_free_index(case_info.index);
+  case_info.index = idx;
+  export_labels(idx, case_info.L);
+}
+
+// Distance query on the resident index of case_info (fast path).
+inline int hop_constrained_extract_distance(hop_constrained_case_info &case_info, int source,
+                                            int terminal, int hop_cst) {
+  if (!case_info.index) throw error(HSDL_ESTATE, "case_info has no resident index");
+  const uint32_t s = (uint32_t)source, t = (uint32_t)terminal;
+  const int8_t k = (int8_t)(hop_cst > 127 ? 127 : (hop_cst < -1 ? -1 : hop_cst));
+  int32_t out = INT_MAX;
+  check(hsdl_query(case_info.index, 1, &s, &t, &k, &out), "hsdl_query");
+  return out;
+}
+
+// The reference's signature on host labels: builds a temporary device index
+// per call (correct but slow; batch queries through gpu_query instead).
+inline int hop_constrained_extract_distance(
+    std::vector<std::vector<hop_constrained_two_hop_label>> &L, int source, int terminal,
+    int hop_cst) {
+  if (hop_cst < 0) return INT_MAX;
+  if (source == terminal) return 0;
+  if (hop_cst == 0) return INT_MAX;
+  int K = 1;
+  for (auto &lst : L)
+    for (auto &x : lst) K = x.hop > K ? x.hop : K;
+  hsdl_index *idx = import_labels(L, K);
+  const uint32_t s = (uint32_t)source, t = (uint32_t)terminal;
+  const int8_t k = (int8_t)(hop_cst > 127 ? 127 : hop_cst);
+  int32_t out = INT_MAX;
+  int rc = hsdl_query(idx, 1, &s, &t, &k, &out);
+  hsdl_free_index(idx);
+  check(rc, "hsdl_query");
+  return out;
+}
+
+// Batched queries (Query_mdnd/src/gpu_query.cu:84: query_info{s,t,h} ->
+// ans[]), on the resident index of case_info.
+struct query_info {
+  int s, t, h;
+};
+inline void gpu_query(hop_constrained_case_info &case_info, int n, const query_info *que,
+                      int *ans) {
+  if (!case_info.index) throw error(HSDL_ESTATE, "case_info has no resident index");
+  std::vector<uint32_t> s(n), t(n);
+  std::vector<int8_t> k(n);
+  for (int i = 0; i < n; ++i) {
+    s[i] = (uint32_t)que[i].s;
+    t[i] = (uint32_t)que[i].t;
+    k[i] = (int8_t)(que[i].h > 127 ? 127 : (que[i].h < -1 ? -1 : que[i].h));
+  }
+  check(hsdl_query(case_info.index, (size_t)n, s.data(), t.data(), k.data(), (int32_t *)ans),
+        "hsdl_query");
+}
+
+}  // namespace hsdl_compat
+
+#endif  // HSDL_COMPAT_HPP

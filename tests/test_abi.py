@@ -144,6 +144,23 @@ def test_grid_generator():
     assert (deg == 4).sum() == 14 * 18 and (deg == 2).sum() == 4
 
 
+def test_compat_driver_is_built_and_links():
+    """The C++ shim driver exists, links libhsdl.so and, without a GPU, stops
+    at the first compute call with HSDL_ENODEV (no CPU fallback)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "compat_driver")
+    assert os.path.exists(exe), "run make -C gpu4hophl_code_amd/csrc"
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present: the GPU tests run the driver")
+    except ImportError:
+        pass
+    out = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "euroroad2.txt"), "3", "5"],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 3 and "no usable GPU" in out.stdout
+
+
 def test_generation_needs_gpu_no_cpu_fallback():
     import ctypes
     L = H.lib()

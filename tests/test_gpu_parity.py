@@ -199,6 +199,17 @@ def test_export_import_save_load_roundtrip(tmp_path, c1):
     assert np.array_equal(got, z["ans"])
 
 
+@pytest.mark.parametrize("K", [2, 4])
+def test_cpp_compat_driver(golden_dir, K):
+    """The reference-style C++ driver on include/hsdl_compat.hpp."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(golden_dir), "compat_driver")
+    out = subprocess.run([exe, os.path.join(golden_dir, "simple_iterative_tests.txt"), str(K), "200"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "check pass" in out.stdout
+
+
 def test_server_results_same_alone_and_in_mix(c1):
     g, z = c1
     idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
