@@ -72,7 +72,8 @@ class hsdl_stats(ctypes.Structure):
 class hsdl_server_opts(ctypes.Structure):
     _fields_ = [("max_batch", ctypes.c_uint32), ("max_wait_us", ctypes.c_uint32),
                 ("inflight", ctypes.c_uint32), ("queue_capacity", ctypes.c_uint64),
-                ("hbm_budget", ctypes.c_uint64), ("reserved", ctypes.c_uint32 * 8)]
+                ("hbm_budget", ctypes.c_uint64), ("staging", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 7)]
 
 
 class hsdl_server_stats(ctypes.Structure):
@@ -369,8 +370,10 @@ gpu_query = hop_constrained_extract_distance
 class Server:
     """Continuous-batching query server over a resident index."""
 
+    STAGING = {"mapped": 0, "copy": 1}
+
     def __init__(self, index, max_batch=65536, max_wait_us=200, inflight=4,
-                 queue_capacity=1 << 24, hbm_budget=0):
+                 queue_capacity=1 << 24, hbm_budget=0, staging="mapped"):
         o = hsdl_server_opts()
         lib().hsdl_server_opts_init(ctypes.byref(o))
         o.max_batch = max_batch
@@ -378,6 +381,7 @@ class Server:
         o.inflight = inflight
         o.queue_capacity = queue_capacity
         o.hbm_budget = hbm_budget
+        o.staging = self.STAGING[staging]
         self.index = index  # keeps the index alive
         self.h = _vp()
         _check(lib().hsdl_server_create(index.h, ctypes.byref(o), ctypes.byref(self.h)))

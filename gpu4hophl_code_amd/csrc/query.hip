@@ -8,17 +8,29 @@
 // The reference's GPU query (Query_mdnd/src/gpu_query.cu:5-177) launches three
 // kernels per query; here one launch answers a whole batch.
 //
-// One wavefront per query.  The hubs of the longer list are staged in LDS
-// with coalesced loads (64 labels per instruction) -- all of them for lists
-// up to 2048 labels, every stride-th one (a skip index) for longer lists; the
-// lanes then take the shorter list 64 labels at a time, binary-search each
-// hub in LDS (plus at most log2(stride) global probes inside the block),
-// walk that hub's run (<= K+1 labels, hops ascending) while h1 + h2 <= k and
-// keep a running minimum; a wave min-reduction gives the answer.  A query's
-// critical path is a few coalesced loads plus LDS probes, which keeps the
-// latency of small serving batches low.
+// One workgroup of four wavefronts per query, in a persistent loop over the
+// batch (workgroup g takes queries g, g+G, g+2G, ...).  Two paths:
+//  * merge path (|L(s)| + |L(t)| <= 4094): both lists are staged in LDS as
+//    packed labels, then the 256 lanes split the merged hub order evenly
+//    (merge path: each lane binary-searches its start on the diagonal and
+//    walks ceil(total/256) merged positions).  The shorter list A is the
+//    probing side: an element of A, met at B position j (= first B label
+//    with hub >= its hub, since A goes first on equal hubs), walks B's run of
+//    that hub (<= K+1 labels, hops ascending) while h1 + h2 <= k.
+//  * long path (longer pairs): the hubs of the longer list B are staged in
+//    LDS -- all of them up to 8188, else every stride-th one (a skip index);
+//    lanes take A 256 labels at a time, binary-search each hub in LDS (plus
+//    at most log2(stride) global probes) and walk B's run in global memory.
+// Memory latency is hidden by software pipelining across the workgroup's
+// queries: while query i is joined out of LDS, the labels of query i+1
+// (16 per lane in registers), the label offsets of query i+2 and the
+// (s,t,k) of query i+3 are in flight.  The prefetches are vector loads by
+// lane 0 of each wave (broadcast with readfirstlane), so they never hold up
+// the LDS waits of the join.  A min-reduction over the four waves gives the
+// answer.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <vector>
 
@@ -26,8 +38,12 @@
 
 namespace hsdl {
 
-constexpr uint32_t kQueryWaves = 4;        // queries (wavefronts) per workgroup
-constexpr uint32_t kQueryLdsHubs = 2048;   // hubs of the longer list per wave
+constexpr uint32_t kQThreads = 256;                 // four wavefronts per query
+constexpr uint32_t kQWaves = kQThreads / kWave;
+constexpr uint32_t kStageWords = 4096;              // 32 KB of LDS per workgroup
+constexpr uint32_t kMergeCap = kStageWords - 2;     // last 16 B hold the wave minima
+constexpr uint32_t kLongHubs = 2 * kMergeCap;       // hubs of B staged on the long path
+constexpr uint32_t kLoadPerLane = kStageWords / kQThreads;  // 16 labels in flight per lane
 
 template <class Getter>
 __device__ inline uint32_t lower_bound_by(Getter get, uint32_t len, uint32_t w) {
@@ -39,74 +55,233 @@ __device__ inline uint32_t lower_bound_by(Getter get, uint32_t len, uint32_t w) 
   return lo;
 }
 
-__global__ void __launch_bounds__(256) k_query(uint32_t n, const unsigned long long *off,
-                                               const unsigned long long *lab, uint64_t nq,
-                                               const uint32_t *qs, const uint32_t *qt,
-                                               const int8_t *qk, int32_t *out, uint32_t lds_hubs) {
-  __shared__ uint32_t hubs_all[kQueryWaves * kQueryLdsHubs];
-  const int lane = lane_id();
-  const uint32_t wv = threadIdx.x / kWave;
-  uint32_t *hubs = hubs_all + wv * kQueryLdsHubs;
-  const uint64_t q = (uint64_t)blockIdx.x * kQueryWaves + wv;
-  if (q >= nq) return;  // whole wave leaves together
-  const uint32_t s = qs[q], t = qt[q];
-  const int k = qk[q];
-  if (k < 0 || s >= n || t >= n || s == t || k == 0) {
-    if (lane == 0) out[q] = (k >= 0 && s == t && s < n) ? 0 : INT32_MAX;
-    return;
+__device__ inline uint32_t lane_u32(uint32_t v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline uint64_t lane_u64(uint64_t v, int l) {
+  return ((uint64_t)lane_u32((uint32_t)(v >> 32), l) << 32) | lane_u32((uint32_t)v, l);
+}
+
+// Prefetch loads.  Lanes 0-2 load (s, t, k) of a query and lanes 0-3 its two
+// label ranges; the per-lane addresses keep them vector loads (a scalar load
+// would share the wait counter of the LDS traffic and stall the join).
+__device__ inline uint32_t load_params(uint32_t lane, uint64_t x, const uint32_t *qs,
+                                       const uint32_t *qt, const int8_t *qk) {
+  uint32_t v = 0;
+  if (lane < 2) v = (lane == 0 ? qs : qt)[x];
+  else if (lane == 2) v = (uint32_t)(int32_t)qk[x];
+  return v;
+}
+__device__ inline uint64_t load_ranges(uint32_t lane, const unsigned long long *off, uint32_t s,
+                                       uint32_t t) {
+  uint64_t v = 0;
+  if (lane < 4) v = off[((lane & 2) ? t : s) + (lane & 1)];
+  return v;
+}
+
+// A query after its (s,t,k) and label offsets are known; uniform over the
+// workgroup.  fixed >= 0 is an answer that needs no join.
+struct QJob {
+  const unsigned long long *A, *B;
+  uint32_t ls, lt, kk;
+  int32_t fixed;
+};
+
+__device__ inline bool needs_join(uint32_t n, uint32_t s, uint32_t t, int k) {
+  return k > 0 && s < n && t < n && s != t;
+}
+
+__device__ inline QJob make_job(uint32_t n, const unsigned long long *lab, uint32_t s, uint32_t t,
+                                int k, uint64_t sb, uint64_t se, uint64_t tb, uint64_t te) {
+  QJob j;
+  j.A = j.B = lab;
+  j.ls = j.lt = 0;
+  j.kk = 0;
+  if (!needs_join(n, s, t, k)) {
+    j.fixed = (k >= 0 && s == t && s < n) ? 0 : INT32_MAX;
+    return j;
   }
-  const unsigned long long sb = off[s], tb = off[t];
-  uint32_t ls = (uint32_t)(off[s + 1] - sb), lt = (uint32_t)(off[t + 1] - tb);
+  uint32_t ls = (uint32_t)(se - sb), lt = (uint32_t)(te - tb);
   const unsigned long long *A = lab + sb, *B = lab + tb;
   if (ls > lt) {  // A = shorter list, B = longer list
     const unsigned long long *T = A; A = B; B = T;
     const uint32_t tl = ls; ls = lt; lt = tl;
   }
-  if (ls == 0) {  // an empty list shares no hub
-    if (lane == 0) out[q] = INT32_MAX;
-    return;
-  }
-  const uint32_t kk = (uint32_t)k;
-  // LDS skip index of B: every stride-th hub (stride 1 = all of B for lists
-  // up to kQueryLdsHubs).  A lookup searches LDS for the block, then at most
-  // log2(stride) labels of B in global memory.
-  const uint32_t stride = (lt + lds_hubs - 1) / lds_hubs;
-  const uint32_t ns = (lt + stride - 1) / stride;
-  for (uint32_t i = lane; i < ns; i += kWave) hubs[i] = label_hub(B[(uint64_t)i * stride]);
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  uint32_t best = kInf32;
-  for (uint32_t j = lane; j < ls; j += kWave) {
-    const unsigned long long a = A[j];
-    const uint32_t w = label_hub(a), h1 = label_hop(a), d1 = label_dist(a);
-    if (h1 > kk) continue;
-    // first sample >= w, then the first position >= w inside its block
-    const uint32_t si = lower_bound_by([&](uint32_t i) { return hubs[i]; }, ns, w);
-    const uint32_t lo = si == 0 ? 0 : (si - 1) * stride + 1;
-    const uint32_t hi = si < ns ? si * stride : lt;
-    uint32_t p = lo + lower_bound_by([&](uint32_t i) { return label_hub(B[lo + i]); }, hi - lo, w);
-    for (; p < lt; ++p) {
-      const unsigned long long b = B[p];
-      if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend within a hub
-      const uint32_t dsum = d1 + label_dist(b);
-      best = dsum < best ? dsum : best;
+  j.A = A; j.B = B; j.ls = ls; j.lt = lt; j.kk = (uint32_t)k;
+  j.fixed = ls == 0 ? INT32_MAX : -1;  // an empty list shares no hub
+  return j;
+}
+
+__device__ inline bool merge_job(const QJob &j, uint32_t merge_cap) {
+  return j.fixed < 0 && j.ls + j.lt <= merge_cap;
+}
+
+__global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned long long *off,
+                                                     const unsigned long long *lab, uint64_t nq,
+                                                     const uint32_t *qs, const uint32_t *qt,
+                                                     const int8_t *qk, int32_t *out,
+                                                     uint32_t merge_cap, uint32_t lds_hubs) {
+  __shared__ unsigned long long stage[kStageWords];
+  uint32_t *wave_min = reinterpret_cast<uint32_t *>(stage + kMergeCap);
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const uint64_t G = gridDim.x;
+  uint64_t q = blockIdx.x;
+  if (q >= nq) return;  // uniform over the workgroup
+
+  // ---- prologue: job of q (synchronous), ranges of q+G, (s,t,k) of q+2G
+  uint32_t pv = load_params(lane, q, qs, qt, qk);
+  uint32_t s = lane_u32(pv, 0), t = lane_u32(pv, 1);
+  int k = (int)lane_u32(pv, 2);
+  uint64_t rv = needs_join(n, s, t, k) ? load_ranges(lane, off, s, t) : 0;
+  QJob cur = make_job(n, lab, s, t, k, lane_u64(rv, 0), lane_u64(rv, 1), lane_u64(rv, 2),
+                      lane_u64(rv, 3));
+  pv = q + G < nq ? load_params(lane, q + G, qs, qt, qk) : 0;
+  uint32_t s1 = lane_u32(pv, 0), t1 = lane_u32(pv, 1);
+  int k1 = q + G < nq ? (int)lane_u32(pv, 2) : -1;
+  if (needs_join(n, s1, t1, k1)) rv = load_ranges(lane, off, s1, t1);
+  if (q + 2 * G < nq) pv = load_params(lane, q + 2 * G, qs, qt, qk);
+  // labels of q
+  unsigned long long v[kLoadPerLane];
+  if (merge_job(cur, merge_cap)) {
+    const uint32_t total = cur.ls + cur.lt;
+#pragma unroll
+    for (uint32_t u = 0; u < kLoadPerLane; ++u) {
+      const uint32_t i = u * kQThreads + tid;
+      const uint32_t ic = i < total ? i : total - 1;  // lanes past the end reload the last label
+      v[u] = ic < cur.ls ? cur.A[ic] : cur.B[ic - cur.ls];
     }
   }
-  best = wave_min_u32(best);
-  if (lane == 0) out[q] = (best == kInf32) ? INT32_MAX : (int32_t)best;
+
+  for (; q < nq; q += G) {
+    // In flight now: labels of q (v), ranges of q+G (rv, lanes 0-3) and
+    // (s,t,k) of q+2G (pv, lanes 0-2).
+    const bool is_merge = merge_job(cur, merge_cap);
+    const uint32_t total = cur.ls + cur.lt;
+    if (is_merge) {
+#pragma unroll
+      for (uint32_t u = 0; u < kLoadPerLane; ++u) {
+        const uint32_t i = u * kQThreads + tid;
+        if (i < total) stage[i] = v[u];
+      }
+    }
+    __syncthreads();
+    // ---- prefetch: labels of q+G, ranges of q+2G, (s,t,k) of q+3G
+    const bool has1 = q + G < nq;
+    const QJob nxt = make_job(n, lab, s1, t1, k1, lane_u64(rv, 0), lane_u64(rv, 1),
+                              lane_u64(rv, 2), lane_u64(rv, 3));
+    const bool has2 = q + 2 * G < nq;
+    const uint32_t s2 = lane_u32(pv, 0), t2 = lane_u32(pv, 1);
+    const int k2 = has2 ? (int)lane_u32(pv, 2) : -1;
+    if (has1 && merge_job(nxt, merge_cap)) {
+      const uint32_t ntot = nxt.ls + nxt.lt;
+#pragma unroll
+      for (uint32_t u = 0; u < kLoadPerLane; ++u) {
+        const uint32_t i = u * kQThreads + tid;
+        const uint32_t ic = i < ntot ? i : ntot - 1;
+        v[u] = ic < nxt.ls ? nxt.A[ic] : nxt.B[ic - nxt.ls];
+      }
+    }
+    if (needs_join(n, s2, t2, k2)) rv = load_ranges(lane, off, s2, t2);
+    if (q + 3 * G < nq) pv = load_params(lane, q + 3 * G, qs, qt, qk);
+
+    // ---- join of q
+    uint32_t best = kInf32;
+    const uint32_t kk = cur.kk;
+    if (is_merge) {
+      const unsigned long long *SA = stage, *SB = stage + cur.ls;
+      const uint32_t ls = cur.ls, lt = cur.lt;
+      const uint32_t per = (total + kQThreads - 1) / kQThreads;
+      const uint32_t d0 = min(tid * per, total), d1 = min(d0 + per, total);
+      uint32_t lo = d0 > lt ? d0 - lt : 0, hi = min(d0, ls);
+      while (lo < hi) {  // first i whose A label does not precede B[d0-1-i]
+        const uint32_t mid = (lo + hi) >> 1;
+        if (label_hub(SA[mid]) <= label_hub(SB[d0 - 1 - mid])) lo = mid + 1; else hi = mid;
+      }
+      uint32_t i = lo, j = d0 - lo;
+      for (uint32_t d = d0; d < d1; ++d) {
+        const bool takeA = i < ls && (j >= lt || label_hub(SA[i]) <= label_hub(SB[j]));
+        if (takeA) {
+          const unsigned long long a = SA[i];
+          const uint32_t w = label_hub(a), h1 = label_hop(a), da = label_dist(a);
+          if (h1 <= kk) {
+            for (uint32_t p = j; p < lt; ++p) {
+              const unsigned long long b = SB[p];
+              if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend in a run
+              const uint32_t dsum = da + label_dist(b);
+              best = dsum < best ? dsum : best;
+            }
+          }
+          ++i;
+        } else {
+          ++j;
+        }
+      }
+    } else if (cur.fixed < 0) {
+      // long path: LDS skip index of B's hubs (stride 1 up to lds_hubs)
+      const unsigned long long *A = cur.A, *B = cur.B;
+      const uint32_t ls = cur.ls, lt = cur.lt;
+      uint32_t *hubs = reinterpret_cast<uint32_t *>(stage);
+      const uint32_t stride = (lt + lds_hubs - 1) / lds_hubs;
+      const uint32_t nsm = (lt + stride - 1) / stride;
+      for (uint32_t i = tid; i < nsm; i += kQThreads) hubs[i] = label_hub(B[(uint64_t)i * stride]);
+      __syncthreads();
+      for (uint32_t j = tid; j < ls; j += kQThreads) {
+        const unsigned long long a = A[j];
+        const uint32_t w = label_hub(a), h1 = label_hop(a), d1 = label_dist(a);
+        if (h1 > kk) continue;
+        // first sample >= w, then the first position >= w inside its block
+        const uint32_t si = lower_bound_by([&](uint32_t x) { return hubs[x]; }, nsm, w);
+        const uint32_t lo = si == 0 ? 0 : (si - 1) * stride + 1;
+        const uint32_t hi = si < nsm ? si * stride : lt;
+        uint32_t p = lo + lower_bound_by([&](uint32_t x) { return label_hub(B[lo + x]); }, hi - lo, w);
+        for (; p < lt; ++p) {
+          const unsigned long long b = B[p];
+          if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend within a hub
+          const uint32_t dsum = d1 + label_dist(b);
+          best = dsum < best ? dsum : best;
+        }
+      }
+    }
+    best = wave_min_u32(best);
+    if (lane == 0) wave_min[wv] = best;
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t m = wave_min[0];
+      for (uint32_t x = 1; x < kQWaves; ++x) m = wave_min[x] < m ? wave_min[x] : m;
+      out[q] = cur.fixed >= 0 ? cur.fixed : (m == kInf32 ? INT32_MAX : (int32_t)m);
+    }
+    // advance the pipeline
+    cur = nxt;
+    s1 = s2; t1 = t2; k1 = k2;
+  }
+}
+
+// Workgroups resident at once on a device (the persistent grid), cached.
+static uint32_t query_grid(int dev) {
+  static int cached_dev = -1;
+  static uint32_t cached = 0;
+  if (dev == cached_dev && cached) return cached;
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_query, kQThreads, 0) != hipSuccess ||
+      per_cu <= 0)
+    per_cu = 4;
+  cached = (uint32_t)(cus * per_cu);
+  cached_dev = dev;
+  return cached;
 }
 
 int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,
                  const int8_t *d_k, int32_t *d_out, hipStream_t st) {
   if (nq == 0) return HSDL_OK;
-  const uint64_t blocks = (nq + kQueryWaves - 1) / kQueryWaves;
-  if (blocks > 0x7FFFFFFFull) return fail(HSDL_EINVAL, "query batch too large for one launch");
-  // LDS skip-index capacity per query (tuning knob HSDL_QUERY_LDS_HUBS, <= 2048)
-  const uint32_t lds_hubs = env_u32("HSDL_QUERY_LDS_HUBS", kQueryLdsHubs, 1, kQueryLdsHubs);
-  hipLaunchKernelGGL(k_query, dim3((uint32_t)blocks), dim3(kQueryWaves * kWave), 0, st, idx->n,
+  // tuning knobs: labels staged for the merge path (<= 4094; 0 sends every
+  // query down the long path) and LDS skip-index capacity of the long path
+  const uint32_t merge_cap = env_u32("HSDL_QUERY_MERGE_CAP", kMergeCap, 0, kMergeCap);
+  const uint32_t lds_hubs = env_u32("HSDL_QUERY_LDS_HUBS", kLongHubs, 1, kLongHubs);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(nq, query_grid(idx->device));
+  hipLaunchKernelGGL(k_query, dim3(grid), dim3(kQThreads), 0, st, idx->n,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
-                     (uint64_t)nq, d_s, d_t, d_k, d_out, lds_hubs);
+                     (uint64_t)nq, d_s, d_t, d_k, d_out, merge_cap, lds_hubs);
   HSDL_HIP(hipGetLastError());
   return HSDL_OK;
 }

@@ -4,10 +4,11 @@
 // between (Query_mdnd/src/gpu_query.cu:139-155).  Here requests go into a host
 // queue; a server thread closes a batch when it holds max_batch requests or
 // its oldest request has waited max_wait_us, and launches it into one of
-// `inflight` slots.  A slot owns pinned host staging, device buffers and its
-// own stream (default priority), so batches that arrive while others run are
-// copied and answered concurrently; a batch waits only for a free slot, and
-// the slot count is capped by the device-memory budget.  Answers are the same
+// `inflight` slots.  A slot owns pinned host staging (device-mapped, read and
+// written by the query kernel directly; or, with copy staging, device buffers
+// too) and its own stream (default priority), so batches that arrive while
+// others run are answered concurrently; a batch waits only for a free slot,
+// and the slot count is capped by the staging-memory budget.  Answers are the same
 // kernel's as hsdl_query: a request's result does not depend on its batch.
 #include <hip/hip_runtime.h>
 
@@ -18,6 +19,8 @@
 #include <cstring>
 #include <deque>
 #include <mutex>
+#include <string>
+#include <cstdio>
 #include <thread>
 #include <vector>
 
@@ -36,12 +39,24 @@ struct Slot {
   std::vector<uint64_t> ids, arrival;
   size_t count = 0;
   bool busy = false;
+  uint64_t t_close = 0, t_h2d = 0, t_kernel = 0, t_d2h = 0, t_launched = 0;
 };
 
 struct Result {
   uint64_t id;
   int32_t ans;
   uint64_t lat;
+};
+
+// Optional event trace (HSDL_SERVER_TRACE=<path>, appended at destroy), times
+// in CLOCK_MONOTONIC ns:
+//   B slot n oldest_arrival closed after_h2d after_launch after_d2h launched retired
+//   G at gap      (worker loop iteration gap over 500 us)
+//   R at lag      (replay submission later than its arrival by over 500 us)
+struct TraceEvent {
+  char kind;
+  uint32_t slot;
+  uint64_t v[8];
 };
 
 }  // namespace hsdl
@@ -65,19 +80,42 @@ struct hsdl_server {
   std::thread worker;
   int device = 0;
   int error = 0;
+  bool tracing = false;
+  std::string trace_path;
+  std::mutex tmu;
+  std::vector<hsdl::TraceEvent> trace;
+  void note(const hsdl::TraceEvent &ev) {
+    if (!tracing) return;
+    std::lock_guard<std::mutex> g(tmu);
+    if (trace.size() < (1u << 22)) trace.push_back(ev);
+  }
 };
 
 namespace hsdl {
 
+// Mapped staging: the kernel reads the packed requests from and writes the
+// answers to pinned fine-grained host memory, so a batch costs one launch and
+// one event record.  Copy staging moves them through device buffers.
 static void launch_slot(hsdl_server *srv, Slot &sl) {
   const size_t n = sl.count;
-  (void)hipMemcpyAsync(sl.d_req.p, sl.h_req, n * 9, hipMemcpyHostToDevice, sl.st);
-  const uint8_t *d = sl.d_req.as<uint8_t>();
+  const bool copy = srv->o.staging == HSDL_STAGE_COPY;
+  if (copy) {
+    if (hipMemcpyAsync(sl.d_req.p, sl.h_req, n * 9, hipMemcpyHostToDevice, sl.st) != hipSuccess)
+      srv->error = HSDL_EDEVICE;
+  }
+  sl.t_h2d = srv->tracing ? now_ns() : 0;
+  const uint8_t *d = copy ? sl.d_req.as<uint8_t>() : sl.h_req;
+  int32_t *out = copy ? sl.d_out.as<int32_t>() : sl.h_out;
   if (launch_query(srv->idx, n, (const uint32_t *)d, (const uint32_t *)(d + 4 * n),
-                   (const int8_t *)(d + 8 * n), sl.d_out.as<int32_t>(), sl.st) != HSDL_OK)
+                   (const int8_t *)(d + 8 * n), out, sl.st) != HSDL_OK)
     srv->error = HSDL_EDEVICE;
-  (void)hipMemcpyAsync(sl.h_out, sl.d_out.p, n * 4, hipMemcpyDeviceToHost, sl.st);
-  (void)hipEventRecord(sl.done, sl.st);
+  sl.t_kernel = srv->tracing ? now_ns() : 0;
+  if (copy) {
+    if (hipMemcpyAsync(sl.h_out, sl.d_out.p, n * 4, hipMemcpyDeviceToHost, sl.st) != hipSuccess)
+      srv->error = HSDL_EDEVICE;
+  }
+  sl.t_d2h = srv->tracing ? now_ns() : 0;
+  if (hipEventRecord(sl.done, sl.st) != hipSuccess) srv->error = HSDL_EDEVICE;
   sl.busy = true;
 }
 
@@ -86,8 +124,14 @@ static void worker_loop(hsdl_server *srv) {
   const uint64_t cap = srv->o.queue_capacity;
   const uint64_t wait_ns = (uint64_t)srv->o.max_wait_us * 1000ull;
   std::vector<Result> done;
+  uint64_t last_iter = now_ns();
   while (true) {
     bool progressed = false;
+    if (srv->tracing) {
+      const uint64_t t = now_ns();
+      if (t - last_iter > 500000) srv->note(TraceEvent{'G', 0, {t, t - last_iter}});
+      last_iter = t;
+    }
     // 1. retire finished slots
     for (Slot &sl : srv->slots) {
       if (!sl.busy) continue;
@@ -98,6 +142,9 @@ static void worker_loop(hsdl_server *srv) {
         for (size_t i = 0; i < sl.count; ++i) sl.h_out[i] = INT32_MAX;
       }
       const uint64_t t = now_ns();
+      srv->note(TraceEvent{'B', (uint32_t)(&sl - srv->slots.data()),
+                           {sl.count, sl.count ? sl.arrival[0] : 0, sl.t_close, sl.t_h2d,
+                            sl.t_kernel, sl.t_d2h, sl.t_launched, t}});
       done.clear();
       for (size_t i = 0; i < sl.count; ++i)
         done.push_back(Result{sl.ids[i], sl.h_out[i], t > sl.arrival[i] ? t - sl.arrival[i] : 0});
@@ -147,7 +194,9 @@ static void worker_loop(hsdl_server *srv) {
         g.unlock();
         srv->cv.notify_all();
         sl.count = n;
+        sl.t_close = now_ns();
         launch_slot(srv, sl);
+        sl.t_launched = now_ns();
         srv->batches += 1;
         srv->batched_items += n;
         progressed = true;
@@ -175,9 +224,12 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
   if (opts) o = *opts; else hsdl_server_opts_init(&o);
   if (o.max_batch == 0 || o.inflight == 0 || o.queue_capacity == 0)
     return fail(HSDL_EINVAL, "max_batch, inflight and queue_capacity must be positive");
+  if (o.staging != HSDL_STAGE_MAPPED && o.staging != HSDL_STAGE_COPY)
+    return fail(HSDL_EINVAL, "unknown staging mode");
   int rc = use_device(idx->device, nullptr);
   if (rc) return rc;
-  // admission by device memory: each slot holds 13 B per request
+  // admission by memory: each slot stages 13 B per request (device buffers
+  // with copy staging; the same budget caps mapped host staging)
   const uint64_t per_slot = (uint64_t)o.max_batch * 13 + 4096;
   uint64_t budget = o.hbm_budget;
   if (!budget) {
@@ -200,13 +252,19 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
   for (Slot &sl : srv->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_req, o.max_batch * 9ull, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_out, o.max_batch * 4ull, hipHostMallocDefault);
+    // mapped staging needs fine-grained (coherent) host memory: the kernel's
+    // reads and writes go straight to host memory, never to a stale GPU cache
+    const unsigned hflags = o.staging == HSDL_STAGE_MAPPED
+                                ? (hipHostMallocMapped | hipHostMallocCoherent)
+                                : hipHostMallocDefault;
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_req, o.max_batch * 9ull, hflags);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sl.h_out, o.max_batch * 4ull, hflags);
     if (e != hipSuccess) {
       hsdl_server_destroy(srv);
       return hip_fail(e, "server slot setup", __FILE__, __LINE__);
     }
-    if ((rc = sl.d_req.alloc(o.max_batch * 9ull)) || (rc = sl.d_out.alloc(o.max_batch * 4ull))) {
+    if (o.staging == HSDL_STAGE_COPY &&
+        ((rc = sl.d_req.alloc(o.max_batch * 9ull)) || (rc = sl.d_out.alloc(o.max_batch * 4ull)))) {
       hsdl_server_destroy(srv);
       return rc;
     }
@@ -225,6 +283,11 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
       return hip_fail(e != hipSuccess ? e : hipErrorUnknown, "server slot warm-up", __FILE__,
                       __LINE__);
     }
+  }
+  if (const char *tp = getenv("HSDL_SERVER_TRACE")) {
+    srv->tracing = true;
+    srv->trace_path = tp;
+    srv->trace.reserve(1u << 16);
   }
   srv->worker = std::thread(worker_loop, srv);
   *out = srv;
@@ -303,6 +366,21 @@ void hsdl_server_destroy(hsdl_server *srv) {
     if (sl.h_req) (void)hipHostFree(sl.h_req);
     if (sl.h_out) (void)hipHostFree(sl.h_out);
   }
+  if (srv->tracing && !srv->trace.empty()) {
+    if (FILE *f = fopen(srv->trace_path.c_str(), "a")) {
+      for (const TraceEvent &ev : srv->trace) {
+        if (ev.kind == 'B') {
+          fprintf(f, "B %u", ev.slot);
+          for (uint64_t x : ev.v) fprintf(f, " %llu", (unsigned long long)x);
+          fputc('\n', f);
+        } else {
+          fprintf(f, "%c %llu %llu\n", ev.kind, (unsigned long long)ev.v[0],
+                  (unsigned long long)ev.v[1]);
+        }
+      }
+      fclose(f);
+    }
+  }
   delete srv;
 }
 
@@ -332,6 +410,8 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
     size_t upto = sent;
     while (upto < n && start + arrival_offset_ns[upto] <= now) ++upto;
     if (upto > sent) {
+      if (srv->tracing && now > start + arrival_offset_ns[sent] + 500000)
+        srv->note(TraceEvent{'R', 0, {now, now - start - arrival_offset_ns[sent]}});
       arr.resize(upto - sent);
       for (size_t i = sent; i < upto; ++i) arr[i - sent] = start + arrival_offset_ns[i];
       int rc = hsdl_server_submit(srv, upto - sent, s + sent, t + sent, k + sent, arr.data(), nullptr);

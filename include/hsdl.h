@@ -214,8 +214,13 @@ typedef struct hsdl_server_opts {
   uint32_t inflight;       /* concurrent batches on the device (default 4)  */
   uint64_t queue_capacity; /* pending requests (default 1<<24)              */
   uint64_t hbm_budget;     /* device bytes for batch buffers; 0 = derive    */
-  uint32_t reserved[8];
+  uint32_t staging;        /* HSDL_STAGE_MAPPED (default) or HSDL_STAGE_COPY */
+  uint32_t reserved[7];
 } hsdl_server_opts;
+/* Where a batch's requests and answers live: MAPPED = pinned, device-mapped
+ * host memory read and written by the query kernel itself (one launch per
+ * batch, no DMA); COPY = device buffers filled and drained by async copies. */
+enum { HSDL_STAGE_MAPPED = 0, HSDL_STAGE_COPY = 1 };
 
 void hsdl_server_opts_init(hsdl_server_opts *o);
 int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *o, hsdl_server **out);

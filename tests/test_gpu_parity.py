@@ -168,9 +168,10 @@ def test_queries_special_cases_and_bf():
 @pytest.mark.parametrize("lds_hubs", ["7", "64", "2048"])
 def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs):
     """Small LDS skip indexes force strides > 1 on short lists: the clean and
-    the queries must not change."""
+    the queries (all sent down the long path) must not change."""
     monkeypatch.setenv("HSDL_CLEAN_LDS_HUBS", lds_hubs)
     monkeypatch.setenv("HSDL_QUERY_LDS_HUBS", lds_hubs)
+    monkeypatch.setenv("HSDL_QUERY_MERGE_CAP", "0")
     g = H.Graph.rmat(10, 8, 99, 1, 20)
     c = csr_of(g)
     red = O.kround_grouped(c, 4, 32)
@@ -186,6 +187,24 @@ def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs):
     t = rng.integers(0, c.n, 20000).astype(np.uint32)
     k = rng.integers(-1, 6, 20000).astype(np.int8)
     assert np.array_equal(canon.query(s, t, k), O.query(want, s, t, k))
+
+
+@pytest.mark.parametrize("merge_cap", ["0", "64", "150", "300", "4094"])
+def test_query_merge_and_long_paths(monkeypatch, c1, merge_cap):
+    """The merge path (both lists in LDS) and the long path (skip index) give
+    the golden answers; the caps split the config-1 queries between them
+    (lists of 1..256 labels)."""
+    monkeypatch.setenv("HSDL_QUERY_MERGE_CAP", merge_cap)
+    g, z = c1
+    idx = H.Index.from_labels(len(z["lab_off"]) - 1, 4, z["lab_off"].astype(np.uint64),
+                              z["hub"], z["hop"], z["dist"])
+    assert np.array_equal(idx.query(z["qs"], z["qt"], z["qk"]), z["ans"])
+    rng = np.random.default_rng(int(merge_cap) + 7)
+    s = rng.integers(0, 4096, 20000).astype(np.uint32)
+    t = rng.integers(0, 4096, 20000).astype(np.uint32)
+    k = rng.integers(-1, 9, 20000).astype(np.int8)
+    want = O.query(O.labels_from_arrays(4096, z["lab_off"], z["hub"], z["hop"], z["dist"]), s, t, k)
+    assert np.array_equal(idx.query(s, t, k), want)
 
 
 def test_export_import_save_load_roundtrip(tmp_path, c1):
