@@ -91,8 +91,7 @@ def query_throughput(H, idx, n, nq, torch, reps=5):
 
 
 def label_lengths(idx):
-    off, hub, hop, dist = idx.labels()
-    return np.diff(off.astype(np.int64)), (off, hub, hop, dist)
+    return np.diff(idx.offsets().astype(np.int64))
 
 
 def serving(H, idx, n, rates, nq):
@@ -231,7 +230,7 @@ def main():
         roofline["copy_gbps"] = round(gbps.value, 1)
 
     # queries on the device-resident cleaned index
-    lens, _ = label_lengths(idx)
+    lens = label_lengths(idx)
     s, t, k, ans, qsec = query_throughput(H, idx, n, args.queries, torch)
     qbytes = float((9 * (lens[s] + lens[t]) + 13)[s != t].sum() + 13 * (s == t).sum())
     query = {"queries": args.queries, "qps": args.queries / qsec, "kernel_ms": qsec * 1e3,

@@ -101,7 +101,7 @@ _SIGNATURES = [
     ("hsdl_query", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
     ("hsdl_query_device", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp]),
     ("hsdl_label_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
-    ("hsdl_export_labels", ctypes.c_int, [_vp, _u64p, _u32p, _u8p, _i32p]),
+    ("hsdl_export_labels", ctypes.c_int, [_vp, _u64p, _vp, _vp, _vp]),
     ("hsdl_index_from_labels", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint8, _u64p, _u32p, _u8p, _i32p, ctypes.c_int32, ctypes.POINTER(_vp)]),
     ("hsdl_index_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint8)]),
     ("hsdl_save_labels", ctypes.c_int, [_vp, ctypes.c_char_p]),
@@ -295,8 +295,16 @@ class Index:
         hub = np.zeros(tot, np.uint32)
         hop = np.zeros(tot, np.uint8)
         dist = np.zeros(tot, np.int32)
-        _check(lib().hsdl_export_labels(self.h, off, hub, hop, dist))
+        _check(lib().hsdl_export_labels(self.h, off, hub.ctypes.data_as(_vp),
+                                        hop.ctypes.data_as(_vp), dist.ctypes.data_as(_vp)))
         return off, hub, hop, dist
+
+    def offsets(self):
+        """off[n+1] only (label list of v = [off[v], off[v+1]))."""
+        n, _ = self.info()
+        off = np.zeros(n + 1, np.uint64)
+        _check(lib().hsdl_export_labels(self.h, off, None, None, None))
+        return off
 
     def clean(self):
         _check(lib().hsdl_clean(self.h))

@@ -323,6 +323,10 @@ __global__ void __launch_bounds__(256) k_group(GenArgs a) {
 }
 
 // ----------------------------------------------------------------- prune
+// Sixteen wavefronts share one tile: with a ~60 KB tile two workgroups fit a
+// CU, which keeps 32 waves (the maximum) streaming label lists per CU.
+constexpr uint32_t kPruneThreads = 1024;
+
 // The tile of L(s) is an open-addressing hash: keys[c] = hub and
 // vals[c*(K+1)+x] = min dist over labels (hub, hb, db) of L(s) with hb <= x.
 // V is u16 when every label distance fits (K * max weight < 65535), halving
@@ -397,41 +401,56 @@ __device__ inline uint32_t tile_lookup(const uint32_t *keys, const V *vals, uint
 }
 
 // True when some common hub gives da + db <= dy (candidate pruned); `reads`
-// receives the number of labels of L(y) loaded.
+// receives the number of labels of L(y) examined (algorithmic: up to the
+// 128-label step that holds the witness).  L(y) is streamed 128 labels per
+// step (two per lane) with the next step's loads already in flight, so a wave
+// waits about one memory round trip per 128 labels, not per 64.
+template <class V>
+__device__ inline bool probe_label(const GenArgs &a, unsigned long long l, bool valid,
+                                   uint32_t dy, const uint32_t *keys, const V *vals,
+                                   uint32_t cap, uint32_t K1) {
+  const uint32_t ha = label_hop(l);
+  if (!valid || ha > a.h) return false;
+  const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, label_hub(l), a.h - ha);
+  return v != TileVal<V>::kEmpty && label_dist(l) + v <= dy;
+}
+
 template <class V>
 __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
                                const V *vals, uint32_t cap, uint32_t &reads) {
-  const int lane = lane_id();
+  const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
-  const unsigned long long st = a.seg_start[y];
+  const unsigned long long *L = a.pool + a.seg_start[y];
   const uint32_t len = a.seg_len[y];
-  for (uint32_t j = 0; j < len; j += kWave) {
-    bool f = false;
-    if (j + lane < len) {
-      const unsigned long long l = a.pool[st + j + lane];
-      const uint32_t ha = label_hop(l);
-      if (ha <= a.h) {
-        const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, label_hub(l), a.h - ha);
-        if (v != TileVal<V>::kEmpty && label_dist(l) + v <= dy) f = true;
-      }
-    }
+  constexpr uint32_t kStep = 2 * kWave;
+  unsigned long long c0 = 0, c1 = 0;
+  if (lane < len) c0 = L[lane];
+  if (kWave + lane < len) c1 = L[kWave + lane];
+  for (uint32_t j = 0; j < len; j += kStep) {
+    const uint32_t jn = j + kStep;
+    unsigned long long n0 = 0, n1 = 0;
+    if (jn + lane < len) n0 = L[jn + lane];
+    if (jn + kWave + lane < len) n1 = L[jn + kWave + lane];
+    const bool f = probe_label<V>(a, c0, j + lane < len, dy, keys, vals, cap, K1) ||
+                   probe_label<V>(a, c1, j + kWave + lane < len, dy, keys, vals, cap, K1);
     if (__ballot(f)) {
-      reads = min(len, j + kWave);
+      reads = min(len, jn);
       return true;
     }
+    c0 = n0;
+    c1 = n1;
   }
   reads = len;
   return false;
 }
 
 template <class V>
-__global__ void __launch_bounds__(256) k_prune(GenArgs a) {
+__global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_prune(GenArgs a) {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base;
+  __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next;
   __shared__ Item sh_acc[256];
   const int lane = lane_id();
-  const uint32_t wv = threadIdx.x / kWave;
-  const uint32_t nwv = blockDim.x / kWave;
   const uint32_t K1 = a.K + 1;
   Item *Fn = frontier(a, a.cur ^ 1);
   uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
@@ -442,6 +461,7 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
     if (threadIdx.x == 0) {
       sh_w = atomicAdd(&a.ctrl->work_ctr, 1u);
       sh_acc_n = 0;
+      sh_next = 0;
     }
     __syncthreads();
     const uint32_t w = sh_w;
@@ -486,7 +506,12 @@ __global__ void __launch_bounds__(256) k_prune(GenArgs a) {
       tile_b = kInf32;
       tile_reads += a.seg_len[s];
     }
-    for (uint32_t c = c_beg + wv; c < c_end; c += nwv) {
+    // waves take the item's candidates one at a time (L(y) lengths vary widely)
+    for (;;) {
+      uint32_t c = 0;
+      if (lane == 0) c = c_beg + atomicAdd(&sh_next, 1u);
+      c = __shfl(c, 0, kWave);
+      if (c >= c_end) break;
       const uint32_t y = cands[c];
       const uint64_t slot = (uint64_t)b * a.n + y;
       const uint32_t dy = a.cand_d[slot];
@@ -659,6 +684,7 @@ struct Gen {
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
+  int prune_grid = 0;  // resident k_prune workgroups (two per CU)
   size_t lds_bytes = 0;
   bool small_vals = false;  // 16-bit tile values
   hipStream_t st = nullptr;
@@ -767,9 +793,9 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
     if (g.small_vals)
-      hipLaunchKernelGGL(k_prune<uint16_t>, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
+      hipLaunchKernelGGL(k_prune<uint16_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
     else
-      hipLaunchKernelGGL(k_prune<uint32_t>, dim3(g.grid), dim3(256), g.lds_bytes, g.st, a);
+      hipLaunchKernelGGL(k_prune<uint32_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
@@ -813,6 +839,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     ~StreamGuard() { if (s) (void)hipStreamDestroy(s); }
   } sg{g.st};
   g.grid = grid_for_device(dev);
+  g.prune_grid = g.grid / 2;
 
   // device graph (kept with the graph handle)
   if ((rc = upload_graph(const_cast<hsdl_graph *>(hg), dev))) return rc;

@@ -112,7 +112,7 @@ int hsdl_export_labels(const hsdl_index *idx, uint64_t *off, uint32_t *hub, uint
   int rc = use_device(idx->device, nullptr);
   if (rc) return rc;
   HSDL_HIP(hipMemcpy(off, idx->off.p, sizeof(uint64_t) * (idx->n + 1ull), hipMemcpyDeviceToHost));
-  if (idx->total == 0) return HSDL_OK;
+  if (idx->total == 0 || (!hub && !hop && !dist)) return HSDL_OK;
   std::vector<uint64_t> lab(idx->total);
   HSDL_HIP(hipMemcpy(lab.data(), idx->lab.p, sizeof(uint64_t) * idx->total, hipMemcpyDeviceToHost));
   for (uint64_t i = 0; i < idx->total; ++i) {

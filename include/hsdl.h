@@ -183,7 +183,8 @@ int hsdl_query_device(const hsdl_index *idx, size_t n, const uint32_t *d_s,
 /* ---------------------------------------------------- labels in and out */
 
 int hsdl_label_count(const hsdl_index *idx, uint64_t *total);
-/* off[n+1]; hub/hop/dist[total]; hubs are vertex ids of the graph. */
+/* off[n+1]; hub/hop/dist[total]; hubs are vertex ids of the graph.  Any of
+ * hub/hop/dist may be NULL (all three NULL: only the offsets are copied). */
 int hsdl_export_labels(const hsdl_index *idx, uint64_t *off, uint32_t *hub, uint8_t *hop,
                        int32_t *dist);
 /* Index from host label arrays (any per-vertex order; sorted on import).  Lets
