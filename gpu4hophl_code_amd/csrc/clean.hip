@@ -9,21 +9,27 @@
 // oracle's tests require both readings to give the same output.
 //
 // Two passes, exact in every case:
-//  1. k_clean_all: the predicate with `kept` replaced by every earlier label
+//  1. k_clean_tile: the predicate with `kept` replaced by every earlier label
 //     of v.  That set contains the sequential kept set, so its minima are
 //     never larger: when it removes nothing from L(v), the sequential walk
-//     keeps every label of v as well.  Pass 1 has no order dependence, so one
-//     wavefront takes all hub groups of v at once: L(v)'s hubs sit in LDS,
-//     64-label windows of L(v) yield up to 64 group heads (ballot on hub
-//     changes), the heads' lists L(u) are load-balanced over the lanes by a
-//     register prefix sum, and every element (w,h1,d1) of L(u) that could
-//     still remove a group label (h1 + 1 <= the group's largest hop and
-//     d1 <= its largest dist; hop 0 belongs to the self label only) binary-
-//     searches w in the LDS hubs (in global memory for lists longer than the
-//     LDS copy) and tests its pairs against the group's labels.
+//     keeps every label of v as well.  Pass 1 has no order dependence.  One
+//     workgroup (16 wavefronts) takes a vertex v: it builds a hash tile of
+//     L(v) (hub -> per-hop prefix minimum distance; in LDS, or in the
+//     workgroup's slice of a global arena for long lists), then the waves take
+//     16-label windows of L(v) in turn, find the windows' group heads (hub u),
+//     load-balance the heads' lists L(u) over the lanes by a register prefix
+//     sum, and every element (w,h1,d1) of L(u) with w < u that could still
+//     remove a group label (h1 + 1 <= the group's largest hop and d1 <= its
+//     largest dist) looks w up once in the tile: a group label (u,h,d) falls
+//     when d1 + tile[w][h - h1] <= d.  Pairs through u itself (earlier labels
+//     of the same group) are checked directly.
 //  2. k_clean_seq: vertices where pass 1 would remove something are walked
 //     group by group in order with keep bits in LDS, exactly as the
 //     reference does.
+// A K-round, rank-ordered generation already satisfies the predicate (its
+// acceptance test is the all-earlier test over the same final lists), so on
+// this engine's own output pass 1 removes nothing and pass 2 never runs; the
+// clean still verifies every label.
 // The reference's GPU clean (Hybrid_Generation_Clean/src/label_clean.cu:73-164)
 // uses a dense per-vertex hash of all labels instead; the kept-set form is
 // the normative one.
@@ -36,10 +42,12 @@
 
 namespace hsdl {
 
-constexpr uint32_t kCleanWaves = 4;            // waves per workgroup
+constexpr uint32_t kCleanWaves = 4;            // waves per workgroup (pass 2)
 constexpr uint32_t kSeqBitsPerWave = 65536;    // max labels per vertex (pass 2)
 constexpr uint32_t kSeqWordsPerWave = kSeqBitsPerWave / 32;
-constexpr uint32_t kAllHubsPerWave = 2048;     // L(v) hubs staged in LDS (pass 1)
+constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vertex
+constexpr uint32_t kTileWindow = 16;           // labels of L(v) per work unit
+constexpr uint32_t kTileLdsBytes = 60 * 1024;  // LDS tile budget (two workgroups per CU)
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -57,132 +65,201 @@ __device__ inline uint32_t lb_hub(const unsigned long long *lab, uint32_t lo, ui
   return lo;
 }
 
-__device__ inline uint32_t lb_u32(const uint32_t *h, uint32_t lo, uint32_t hi, uint32_t w) {
-  while (lo < hi) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (h[mid] < w) lo = mid + 1; else hi = mid;
-  }
-  return lo;
-}
-
 // ---------------------------------------------------------------- pass 1
-// L(v)'s hubs are found through an LDS skip index: samp[i] = hub of label
-// i*stride (stride 1 = every label for lists up to kAllHubsPerWave).  A lower
-// bound searches LDS for the block and then at most log2(stride) labels in
-// global memory.
-__device__ inline uint32_t lb_skip(const uint32_t *samp, uint32_t stride,
-                                   const unsigned long long *Lv, uint32_t end, uint32_t w) {
-  const uint32_t ns = (end + stride - 1) / stride;  // samples below `end`
-  const uint32_t si = lb_u32(samp, 0, ns, w);
-  const uint32_t lo = si == 0 ? 0 : (si - 1) * stride + 1;
-  const uint32_t hi = si < ns ? si * stride : end;
-  return lb_hub(Lv, lo, hi, w);
-}
-
-// Returns true when the all-earlier predicate removes some label of v.
-__device__ bool any_removal(const unsigned long long *off, const unsigned long long *lab,
-                            const unsigned long long *Lv, uint32_t len, const uint32_t *samp,
-                            uint32_t stride) {
-  const int lane = lane_id();
-  bool removed = false;
-  for (uint32_t p0 = 0; p0 < len && !removed; p0 += kWave) {
-    // group heads in this window of L(v)
-    const uint32_t p = p0 + lane;
-    const bool head = p < len && (p == 0 || label_hub(Lv[p]) != label_hub(Lv[p - 1]));
-    uint32_t u = 0, glen = 0, maxh = 0, maxd = 0;
-    unsigned long long ub = 0;
-    uint32_t ulen = 0;
-    if (head) {
-      u = label_hub(Lv[p]);
-      while (p + glen < len && label_hub(Lv[p + glen]) == u) {
-        const unsigned long long g = Lv[p + glen];
-        maxh = max(maxh, label_hop(g));
-        maxd = max(maxd, label_dist(g));
-        ++glen;
-      }
-      ub = off[u];
-      ulen = (uint32_t)(off[u + 1] - ub);
-    }
-    const uint32_t incl = wave_incl_scan_u32(ulen);
-    const uint32_t excl = incl - ulen;
-    const uint32_t total = __shfl(incl, kWave - 1, kWave);
-    bool hit = false;
-    for (uint32_t jb = 0; jb < total; jb += kWave) {
-      const uint32_t j = jb + lane;
-      int k = 0;
-#pragma unroll
-      for (int step = 32; step > 0; step >>= 1) {
-        const uint32_t e = __shfl(excl, k + step, kWave);
-        if (e <= j) k += step;
-      }
-      const uint32_t ku = __shfl(u, k, kWave);
-      const uint32_t kglen = __shfl(glen, k, kWave);
-      const uint32_t kmaxh = __shfl(maxh, k, kWave);
-      const uint32_t kmaxd = __shfl(maxd, k, kWave);
-      const uint32_t kex = __shfl(excl, k, kWave);
-      const unsigned long long kub = __shfl(ub, k, kWave);
-      const uint32_t gs = p0 + (uint32_t)k;  // group start in L(v)
-      if (j >= total) continue;
-      const unsigned long long l = lab[kub + (j - kex)];
-      const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
-      if (w < ku) {
-        // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
-        if (h1 + 1 > kmaxh || d1 > kmaxd) continue;
-        for (uint32_t q = lb_skip(samp, stride, Lv, gs, w); q < gs; ++q) {
-          const unsigned long long e2 = Lv[q];
-          if (label_hub(e2) != w) break;
-          const uint32_t x = h1 + label_hop(e2), sd = d1 + label_dist(e2);
-          for (uint32_t t = 0; t < kglen; ++t) {
-            const unsigned long long g = Lv[gs + t];
-            if (x <= label_hop(g) && sd <= label_dist(g)) hit = true;
-          }
-        }
-      } else if (w == ku) {
-        // through hub u: earlier labels of the group against later ones
-        for (uint32_t t1 = 0; t1 < kglen; ++t1) {
-          const unsigned long long g1 = Lv[gs + t1];
-          for (uint32_t t2 = t1 + 1; t2 < kglen; ++t2) {
-            const unsigned long long g2 = Lv[gs + t2];
-            if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
-              hit = true;
-          }
-        }
-      }
-    }
-    removed = __ballot(hit) != 0;
+// Slot of `hub` in a tile, or kInf32 when absent.
+__device__ inline uint32_t tile_find(const uint32_t *keys, uint32_t cap, uint32_t hub) {
+  uint32_t slot = hash_slot(hub, cap);
+  for (;;) {
+    const uint32_t k = keys[slot];
+    if (k == hub) return slot;
+    if (k == kInf32) return kInf32;
+    slot = (slot + 1 == cap) ? 0 : slot + 1;
   }
-  return removed;
 }
 
-__global__ void __launch_bounds__(256) k_clean_all(uint32_t n, const unsigned long long *off,
-                                                   const unsigned long long *lab, uint8_t *keep,
-                                                   uint32_t *new_len, uint32_t *redo,
-                                                   uint32_t *redo_cnt, uint32_t lds_hubs) {
-  __shared__ uint32_t hubs_all[kCleanWaves * kAllHubsPerWave];
-  const int lane = lane_id();
-  const uint32_t wv = threadIdx.x / kWave;
-  uint32_t *hubs = hubs_all + wv * kAllHubsPerWave;
-  const uint32_t wave = blockIdx.x * kCleanWaves + wv;
-  const uint32_t nwaves = gridDim.x * kCleanWaves;
-  for (uint32_t v = wave; v < n; v += nwaves) {
+// True when the all-earlier predicate removes a label of a group whose head
+// lies in L(v)[p0, p0 + kTileWindow).
+template <class V>
+__device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long *off,
+                               const unsigned long long *lab, const unsigned long long *Lv,
+                               uint32_t len, uint32_t p0, const uint32_t *keys, const V *vals,
+                               uint32_t cap) {
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint32_t p = p0 + lane;
+  // the self label (hub v, last in L(v)) is always kept: no head for it
+  bool head = lane < kTileWindow && p < len && (p == 0 || label_hub(Lv[p]) != label_hub(Lv[p - 1]));
+  uint32_t u = 0, glen = 0, maxh = 0, maxd = 0, ulen = 0;
+  unsigned long long ub = 0;
+  if (head) {
+    u = label_hub(Lv[p]);
+    if (u == v) head = false;
+  }
+  if (head) {
+    while (p + glen < len && label_hub(Lv[p + glen]) == u) {
+      const unsigned long long g = Lv[p + glen];
+      maxh = max(maxh, label_hop(g));
+      maxd = max(maxd, label_dist(g));
+      ++glen;
+    }
+    ub = off[u];
+    ulen = (uint32_t)(off[u + 1] - ub);
+  }
+  const uint32_t incl = wave_incl_scan_u32(ulen);
+  const uint32_t excl = incl - ulen;
+  const uint32_t total = __shfl(incl, kWave - 1, kWave);
+  bool hit = false;
+  for (uint32_t jb = 0; jb < total; jb += kWave) {
+    const uint32_t j = jb + lane;
+    // owning head k: largest k with excl[k] <= j (binary lifting over lanes)
+    int k = 0;
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1) {
+      const uint32_t e = __shfl(excl, k + step, kWave);
+      if (e <= j) k += step;
+    }
+    const uint32_t ku = __shfl(u, k, kWave);
+    const uint32_t kglen = __shfl(glen, k, kWave);
+    const uint32_t kmaxh = __shfl(maxh, k, kWave);
+    const uint32_t kmaxd = __shfl(maxd, k, kWave);
+    const uint32_t kex = __shfl(excl, k, kWave);
+    const unsigned long long kub = __shfl(ub, k, kWave);
+    const uint32_t gs = p0 + (uint32_t)k;  // group start in L(v)
+    if (j >= total) continue;
+    const unsigned long long l = lab[kub + (j - kex)];
+    const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
+    if (w < ku) {
+      // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
+      if (h1 + 1 > kmaxh || d1 > kmaxd) continue;
+      const uint32_t slot = tile_find(keys, cap, w);
+      if (slot == kInf32) continue;
+      for (uint32_t t = 0; t < kglen; ++t) {
+        const unsigned long long g = Lv[gs + t];
+        const uint32_t hg = label_hop(g);
+        if (hg <= h1) continue;
+        const uint32_t m = vals[slot * K1 + (hg - h1)];
+        if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
+      }
+    } else if (w == ku) {
+      // through hub u: earlier labels of the group against later ones
+      for (uint32_t t1 = 0; t1 < kglen; ++t1) {
+        const unsigned long long g1 = Lv[gs + t1];
+        for (uint32_t t2 = t1 + 1; t2 < kglen; ++t2) {
+          const unsigned long long g2 = Lv[gs + t2];
+          if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
+            hit = true;
+        }
+      }
+    }
+  }
+  return __ballot(hit) != 0;
+}
+
+// ctr[0]: redo count, ctr[2]: next vertex.  gtile: per-workgroup arena slices
+// of gwords words for tiles above lds_slots slots.
+template <class V>
+__global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *off, const unsigned long long *lab,
+             uint8_t *keep, uint32_t *new_len, uint32_t *redo, uint32_t *ctr, uint32_t lds_slots,
+             uint32_t *gtile, uint64_t gwords) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t sh_v, sh_win, sh_heads, sh_removed;
+  const uint32_t K1 = K + 1;
+  const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id();
+  uint32_t *gkeys = gtile + (uint64_t)blockIdx.x * gwords;
+  for (;;) {
+    if (tid == 0) {
+      sh_v = atomicAdd(&ctr[2], 1u);
+      sh_win = 0;
+      sh_heads = 0;
+      sh_removed = 0;
+    }
+    __syncthreads();
+    const uint32_t v = sh_v;
+    if (v >= n) break;
     const unsigned long long vb = off[v];
     const uint32_t len = (uint32_t)(off[v + 1] - vb);
     const unsigned long long *Lv = lab + vb;
-    if (len == 0) {
-      if (lane == 0) new_len[v] = 0;
-      continue;
+    if (len > 1) {
+      // distinct hubs of L(v) size the tile
+      uint32_t mine = 0;
+      for (uint32_t i = tid; i < len; i += kTileThreads)
+        mine += (i == 0 || label_hub(Lv[i]) != label_hub(Lv[i - 1])) ? 1u : 0u;
+      mine = wave_sum_u32(mine);
+      if (lane == 0 && mine) atomicAdd(&sh_heads, mine);
+      __syncthreads();
+      const uint32_t cap = tile_cap_for(sh_heads);
+      const bool in_lds = cap <= lds_slots;
+      uint32_t *keys = in_lds ? lds : gkeys;
+      V *vals = reinterpret_cast<V *>(keys + cap);
+      for (uint32_t i = tid; i < cap; i += kTileThreads) keys[i] = kInf32;
+      __syncthreads();
+      // one thread per group: claim the hub's slot, write its per-hop prefix minima
+      for (uint32_t i = tid; i < len; i += kTileThreads) {
+        const uint32_t w = label_hub(Lv[i]);
+        if (i > 0 && label_hub(Lv[i - 1]) == w) continue;
+        uint32_t slot = hash_slot(w, cap);
+        while (atomicCAS(&keys[slot], kInf32, w) != kInf32) slot = (slot + 1 == cap) ? 0 : slot + 1;
+        uint32_t m[kMaxK + 1];
+#pragma unroll
+        for (uint32_t x = 0; x <= kMaxK; ++x) m[x] = kInf32;
+        for (uint32_t q = i; q < len && label_hub(Lv[q]) == w; ++q) {
+          const uint32_t h = label_hop(Lv[q]), d = label_dist(Lv[q]);
+#pragma unroll
+          for (uint32_t x = 0; x <= kMaxK; ++x)
+            if (x == h) m[x] = min(m[x], d);
+        }
+        uint32_t run = kInf32;
+#pragma unroll
+        for (uint32_t x = 0; x <= kMaxK; ++x) {
+          run = min(run, m[x]);
+          if (x < K1) vals[slot * K1 + x] = (V)min(run, (uint32_t)TileVal<V>::kEmpty);
+        }
+      }
+      // a global tile was written through L2: drop stale L1 lines before reading
+      if (!in_lds) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __syncthreads();
+      if (!in_lds) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      for (;;) {
+        uint32_t p0 = 0;
+        if (lane == 0) p0 = atomicAdd(&sh_win, kTileWindow);
+        p0 = __shfl(p0, 0, kWave);
+        if (p0 >= len) break;
+        if (window_removes<V>(v, K1, off, lab, Lv, len, p0, keys, vals, cap)) {
+          if (lane == 0) sh_removed = 1;
+          break;
+        }
+      }
+      __syncthreads();
     }
-    const uint32_t stride = (len + lds_hubs - 1) / lds_hubs;
-    for (uint32_t i = lane; i * stride < len; i += kWave) hubs[i] = label_hub(Lv[(uint64_t)i * stride]);
-    wave_lds_sync();
-    const bool removed = any_removal(off, lab, Lv, len, hubs, stride);
-    wave_lds_sync();
-    if (removed) {
-      if (lane == 0) redo[atomicAdd(redo_cnt, 1u)] = v;
+    if (sh_removed) {
+      if (tid == 0) redo[atomicAdd(&ctr[0], 1u)] = v;
     } else {
-      for (uint32_t i = lane; i < len; i += kWave) keep[vb + i] = 1;
-      if (lane == 0) new_len[v] = len;
+      for (uint32_t i = tid; i < len; i += kTileThreads) keep[vb + i] = 1;
+      if (tid == 0) new_len[v] = len;
     }
+    __syncthreads();  // shared state and the tile are reused for the next vertex
+  }
+}
+
+// Largest list length and largest label distance of an index (out[0], out[1]).
+__global__ void __launch_bounds__(256) k_index_maxima(uint32_t n, const unsigned long long *off,
+                                                      const unsigned long long *lab,
+                                                      uint32_t *out) {
+  uint32_t ml = 0, md = 0;
+  const uint32_t nth = gridDim.x * blockDim.x;
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += nth) {
+    const unsigned long long b = off[v], e = off[v + 1];
+    ml = max(ml, (uint32_t)(e - b));
+  }
+  const unsigned long long total = off[n];
+  for (unsigned long long i = blockIdx.x * (unsigned long long)blockDim.x + threadIdx.x; i < total;
+       i += nth)
+    md = max(md, label_dist(lab[i]));
+  ml = wave_max_u32(ml);
+  md = wave_max_u32(md);
+  if (lane_id() == 0) {
+    atomicMax(&out[0], ml);
+    atomicMax(&out[1], md);
   }
 }
 
@@ -328,18 +405,51 @@ int clean_index(hsdl_index *idx) {
   DevBuf keep, new_len, noff, nlab, redo, ctr;
   if ((rc = keep.alloc(std::max<uint64_t>(idx->total, 1))) || (rc = new_len.alloc(n * 4ull)) ||
       (rc = noff.alloc((n + 1ull) * 8)) || (rc = redo.alloc(n * 4ull + 4)) ||
-      (rc = ctr.alloc(8)))
+      (rc = ctr.alloc(16)))
     return rc;
-  HSDL_HIP(hipMemsetAsync(ctr.p, 0, 8, st));
+  HSDL_HIP(hipMemsetAsync(ctr.p, 0, 16, st));
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, idx->device);
   const int grid = std::max(1, cus * 8);
   uint32_t *cnt = ctr.as<uint32_t>();
-  hipLaunchKernelGGL(k_clean_all, dim3(grid), dim3(kCleanWaves * kWave), 0, st, n,
-                     idx->off.as<unsigned long long>(),
-                     idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
-                     new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
-                     env_u32("HSDL_CLEAN_LDS_HUBS", kAllHubsPerWave, 1, kAllHubsPerWave));
+  // list-length and distance maxima size the global tile arena and pick the
+  // tile's value width
+  DevBuf maxima;
+  if ((rc = maxima.alloc(8))) return rc;
+  HSDL_HIP(hipMemsetAsync(maxima.p, 0, 8, st));
+  hipLaunchKernelGGL(k_index_maxima, dim3(cus * 4), dim3(256), 0, st, n,
+                     idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
+                     maxima.as<uint32_t>());
+  HSDL_HIP(hipGetLastError());
+  uint32_t mx[2] = {0, 0};
+  HSDL_HIP(hipMemcpyAsync(mx, maxima.p, 8, hipMemcpyDeviceToHost, st));
+  HSDL_HIP(hipStreamSynchronize(st));
+  const uint32_t K1 = (uint32_t)idx->K + 1;
+  const bool small_vals = mx[1] < 0xFFFFu;
+  const uint32_t slot_bytes = 4 + K1 * (small_vals ? 2 : 4);
+  // tuning knob HSDL_CLEAN_LDS_SLOTS caps the LDS tile (small values send
+  // every vertex to the global arena)
+  uint32_t lds_slots = (kTileLdsBytes - 16) / slot_bytes;
+  lds_slots = env_u32("HSDL_CLEAN_LDS_SLOTS", lds_slots, 1, lds_slots);
+  const size_t lds_bytes = (size_t)lds_slots * slot_bytes + 16;
+  const int tgrid = std::max(1, cus * 2);
+  const uint32_t gcap = tile_cap_for(mx[0]);
+  const uint64_t gwords = small_vals ? tile_words<uint16_t>(gcap, K1) : tile_words<uint32_t>(gcap, K1);
+  DevBuf gtile;
+  if ((rc = gtile.alloc((uint64_t)tgrid * gwords * 4))) return rc;
+  if (small_vals)
+    hipLaunchKernelGGL(k_clean_tile<uint16_t>, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n,
+                       (uint32_t)idx->K, idx->off.as<unsigned long long>(),
+                       idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
+                       new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, lds_slots,
+                       gtile.as<uint32_t>(), gwords);
+  else
+    hipLaunchKernelGGL(k_clean_tile<uint32_t>, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n,
+                       (uint32_t)idx->K, idx->off.as<unsigned long long>(),
+                       idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
+                       new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, lds_slots,
+                       gtile.as<uint32_t>(), gwords);
+  HSDL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_clean_seq, dim3(grid), dim3(kCleanWaves * kWave), 0, st, (uint32_t)idx->K,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
                      keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,

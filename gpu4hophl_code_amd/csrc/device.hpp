@@ -53,6 +53,43 @@ inline uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t h
   return (uint32_t)(x < lo ? lo : (x > hi ? hi : x));
 }
 
+// Hash tile of a label list (the prune keeps L(s) in one, the clean L(v)): an
+// open-addressing hash with keys[c] = hub and vals[c*(K+1)+x] = min dist over
+// the list's labels (hub, hb, db) with hb <= x.
+// V is u16 when every label distance fits (K * max weight < 65535), halving
+// the tile; the empty value is V's maximum.  Capacities need not be powers of
+// two: the home slot is the high word of (mixed key) * cap.
+template <class V> struct TileVal;
+template <> struct TileVal<uint16_t> { static constexpr uint32_t kEmpty = 0xFFFFu; };
+template <> struct TileVal<uint32_t> { static constexpr uint32_t kEmpty = 0xFFFFFFFFu; };
+
+__device__ inline uint32_t hash_slot(uint32_t key, uint32_t cap) {
+  return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * cap) >> 32);
+}
+
+__host__ __device__ inline uint32_t tile_cap_for(uint32_t len) {
+  return len + (len >> 1) + 1;  // load factor <= 2/3
+}
+
+// u32 words a tile of `cap` slots takes (keys + values, value rows padded)
+template <class V>
+__device__ __host__ inline uint64_t tile_words(uint32_t cap, uint32_t K1) {
+  return (uint64_t)cap + ((uint64_t)cap * K1 * sizeof(V) + 3) / 4;
+}
+
+// Value for (hub, hop budget x), or the empty value when the hub is absent.
+template <class V>
+__device__ inline uint32_t tile_lookup(const uint32_t *keys, const V *vals, uint32_t cap,
+                                       uint32_t K1, uint32_t hub, uint32_t x) {
+  uint32_t slot = hash_slot(hub, cap);
+  for (;;) {
+    const uint32_t k = keys[slot];
+    if (k == hub) return vals[slot * K1 + x];
+    if (k == kInf32) return TileVal<V>::kEmpty;
+    slot = (slot + 1 == cap) ? 0 : slot + 1;
+  }
+}
+
 // Device buffer owned by RAII.
 struct DevBuf {
   void *p = nullptr;
@@ -94,6 +131,15 @@ __device__ inline uint32_t wave_min_u32(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) {
     uint32_t t = __shfl_xor(v, o, kWave);
     v = t < v ? t : v;
+  }
+  return v;
+}
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t t = __shfl_xor(v, o, kWave);
+    v = t > v ? t : v;
   }
   return v;
 }

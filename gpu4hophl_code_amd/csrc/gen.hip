@@ -327,28 +327,8 @@ __global__ void __launch_bounds__(256) k_group(GenArgs a) {
 // CU, which keeps 32 waves (the maximum) streaming label lists per CU.
 constexpr uint32_t kPruneThreads = 1024;
 
-// The tile of L(s) is an open-addressing hash: keys[c] = hub and
-// vals[c*(K+1)+x] = min dist over labels (hub, hb, db) of L(s) with hb <= x.
-// V is u16 when every label distance fits (K * max weight < 65535), halving
-// the tile; the empty value is V's maximum.  Capacities need not be powers of
-// two: the home slot is the high word of (mixed key) * cap.
-template <class V> struct TileVal;
-template <> struct TileVal<uint16_t> { static constexpr uint32_t kEmpty = 0xFFFFu; };
-template <> struct TileVal<uint32_t> { static constexpr uint32_t kEmpty = 0xFFFFFFFFu; };
-
-__device__ inline uint32_t hash_slot(uint32_t key, uint32_t cap) {
-  return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * cap) >> 32);
-}
-
-__device__ inline uint32_t tile_cap_for(uint32_t len) {
-  return len + (len >> 1) + 1;  // load factor <= 2/3
-}
-
-// u32 words a tile of `cap` slots takes (keys + values, value rows padded)
-template <class V>
-__device__ __host__ inline uint64_t tile_words(uint32_t cap, uint32_t K1) {
-  return (uint64_t)cap + ((uint64_t)cap * K1 * sizeof(V) + 3) / 4;
-}
+// The tile of L(s) is the hash tile of device.hpp (hub -> per-hop prefix
+// minimum distance).
 
 // LDS atomicMin on 16-bit values: CAS on the containing word.
 __device__ inline void atomic_min_v(uint16_t *p, uint32_t v) {
@@ -388,17 +368,6 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals
   __syncthreads();
 }
 
-template <class V>
-__device__ inline uint32_t tile_lookup(const uint32_t *keys, const V *vals, uint32_t cap,
-                                       uint32_t K1, uint32_t hub, uint32_t x) {
-  uint32_t slot = hash_slot(hub, cap);
-  for (;;) {
-    const uint32_t k = keys[slot];
-    if (k == hub) return vals[slot * K1 + x];
-    if (k == kInf32) return TileVal<V>::kEmpty;
-    slot = (slot + 1 == cap) ? 0 : slot + 1;
-  }
-}
 
 // True when some common hub gives da + db <= dy (candidate pruned); `reads`
 // receives the number of labels of L(y) examined (algorithmic: up to the

@@ -167,9 +167,10 @@ def test_queries_special_cases_and_bf():
 
 @pytest.mark.parametrize("lds_hubs", ["7", "64", "2048"])
 def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs):
-    """Small LDS skip indexes force strides > 1 on short lists: the clean and
-    the queries (all sent down the long path) must not change."""
-    monkeypatch.setenv("HSDL_CLEAN_LDS_HUBS", lds_hubs)
+    """Small LDS capacities send the clean's tiles to the global arena and
+    force skip-index strides > 1 on the queries' long path: neither the clean
+    nor the queries may change."""
+    monkeypatch.setenv("HSDL_CLEAN_LDS_SLOTS", lds_hubs)
     monkeypatch.setenv("HSDL_QUERY_LDS_HUBS", lds_hubs)
     monkeypatch.setenv("HSDL_QUERY_MERGE_CAP", "0")
     g = H.Graph.rmat(10, 8, 99, 1, 20)
