@@ -106,7 +106,9 @@ struct GenArgs {
   uint32_t *add_cnt;
   uint32_t *add_cur;
   uint32_t *touched;
-  uint32_t *cand_d;           // dense slots [Bmax][n]
+  uint32_t *cand_d;           // dense slots [Bmax][n], tagged (slot_fresh / slot_stale)
+  uint32_t tag;               // epoch of this batch attempt (high bits of a slot)
+  uint32_t slot_shift;        // tag << slot_shift | dist << 1 | stale
   unsigned long long *cflat;  // (b << 32 | y), ccap entries
   uint32_t *cgroup;           // y grouped by hub, ccap entries
   uint32_t ccap;
@@ -123,6 +125,32 @@ struct GenArgs {
 };
 
 __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
+
+// Dense dedupe slots keep, per (hub slot, vertex), the smallest candidate
+// distance of every round of the current batch attempt:
+//   fresh  tag << shift | d << 1        a candidate of the running round
+//   stale  tag << shift | (2d - 1)      left by an earlier round (d >= 1)
+// Older attempts carry larger tags (tags count down; a cleared slot is all
+// ones), so any value of the running attempt is below them.  A round-h
+// candidate whose distance is not below an earlier round's candidate for the
+// same (s, y) is always pruned -- the earlier one was either accepted (then
+// (s,0,0) in L(s) with that label is a witness) or pruned by a witness that
+// still applies with h >= its hop count -- so the expansion drops it: with
+// stale = 2d - 1, a fresh value undercuts a stale one exactly when its
+// distance is smaller.
+__device__ inline uint32_t slot_fresh(const GenArgs &a, uint32_t d) {
+  return (a.tag << a.slot_shift) | (d << 1);
+}
+__device__ inline uint32_t slot_stale(const GenArgs &a, uint32_t d) {
+  return (a.tag << a.slot_shift) | (d ? 2 * d - 1 : 1u);
+}
+__device__ inline uint32_t slot_dist(const GenArgs &a, uint32_t v) {
+  return (v & ((1u << a.slot_shift) - 1u)) >> 1;
+}
+// first candidate of the running round at a slot whose previous value was `old`
+__device__ inline bool slot_opens(const GenArgs &a, uint32_t old) {
+  return (old & 1u) || (old >> a.slot_shift) != a.tag;
+}
 
 // ------------------------------------------------------------------ init
 __global__ void k_init_labels(GenArgs a, uint32_t c0) {
@@ -217,8 +245,9 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       if (j < total) {
         const uint2 ed = a.adj[kbeg + (j - kex)];
         y = ed.x;
-        const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], kd + ed.y);
-        first = (old == kInf32);
+        const uint32_t nv = slot_fresh(a, kd + ed.y);
+        const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], nv);
+        first = nv < old && slot_opens(a, old);
       }
       // first touches: one flat-list atomic per wave, one count atomic per hub
       const unsigned long long firsts = __ballot(first);
@@ -483,12 +512,12 @@ k_prune(GenArgs a) {
       if (c >= c_end) break;
       const uint32_t y = cands[c];
       const uint64_t slot = (uint64_t)b * a.n + y;
-      const uint32_t dy = a.cand_d[slot];
+      const uint32_t dy = slot_dist(a, a.cand_d[slot]);
       uint32_t reads = 0;
       const bool pruned = witness_exists<V>(a, y, dy, keys, vals, cap, reads);
       wave_reads += reads;
       if (lane == 0) {
-        a.cand_d[slot] = kInf32;  // slot ready for the next round
+        a.cand_d[slot] = slot_stale(a, dy);  // later rounds must undercut dy
         if (!pruned) {
           const uint32_t p = atomicAdd(&sh_acc_n, 1u);
           sh_acc[p] = Item{b, y, dy};
@@ -577,7 +606,8 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
 
 // Removes the labels of hubs [s0, s0+B) (the batch being rolled back): they
 // sit at the tail of every segment, after all labels of earlier batches; the
-// self label at position 0 stays.  Also re-arms the batch's dense slots.
+// self label at position 0 stays.  The redo runs under a new slot tag, so the
+// batch's dense slots need no refill.
 __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
     uint32_t len = a.seg_len[v];
@@ -591,10 +621,6 @@ __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
     a.add_cnt[v] = 0;
     a.add_cur[v] = 0;
   }
-  const uint64_t slots = (uint64_t)a.B * a.n;
-  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < slots;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    a.cand_d[i] = kInf32;
 }
 
 // --------------------------------------------------------------- export
@@ -879,6 +905,14 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
   a.cand_d = g.cand_d.as<uint32_t>();
+  {
+    // slot layout: candidate distances need bits for K * max weight; the
+    // rest of the word holds the attempt tag
+    const uint64_t dmax = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1);
+    uint32_t dist_bits = 1;
+    while ((1ull << dist_bits) <= dmax) ++dist_bits;
+    a.slot_shift = dist_bits + 1;  // <= 28: dmax < 2^27
+  }
   a.hub_ccnt = g.hub_ccnt.as<uint32_t>();
   a.hub_off = g.hub_off.as<uint32_t>();
   a.hub_cur = g.hub_cur.as<uint32_t>();
@@ -902,10 +936,24 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
   uint64_t batches = 0, redos = 0;
   uint32_t s0 = 0;
-  Ctrl c{};
+  Ctrl c{}, c_ok{};  // c_ok: counters after the last batch that completed
+  // attempt tags count down from tag_top; when they run out, the rows used
+  // since the last clear are reset to all ones (the oldest possible value)
+  const uint32_t tag_top = (uint32_t)((1ull << (32 - g.a.slot_shift)) - 2);
+  uint64_t tag_next = tag_top;
+  uint32_t rows_used = 0;
   while (s0 < n) {
     const uint32_t B = std::min<uint32_t>(tuner.B, n - s0);
     const uint64_t tb = now_ns();
+    if (tag_next > tag_top) {  // wrapped below zero
+      hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, g.a.cand_d,
+                         (uint64_t)rows_used * n, kInf32);
+      HSDL_HIP(hipGetLastError());
+      tag_next = tag_top;
+      rows_used = 0;
+    }
+    g.a.tag = (uint32_t)tag_next--;
+    rows_used = std::max(rows_used, B);
     if ((rc = enqueue_batch(g, s0, B, K))) return rc;
     const uint64_t te = now_ns();
     if ((rc = read_ctrl(g, c))) return rc;
@@ -940,9 +988,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
           return fail(HSDL_ENOMEM, "a round holds more than 2^32 candidates; lower hub_batch");
         if ((rc = alloc_lists(g, (uint32_t)nc, (uint32_t)nf))) return rc;
       }
+      // the counters restart from the last completed batch (the attempt's
+      // work is redone); the prune's read counters keep it, matching the
+      // kernel time that includes it
       Ctrl fix = c;
       fix.overflow = 0;
       fix.pool_top = used;
+      fix.cand_total = c_ok.cand_total;
+      fix.item_total = c_ok.item_total;
+      fix.edge_total = c_ok.edge_total;
       HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &fix, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
       HSDL_HIP(hipStreamSynchronize(g.st));
       g.h_grow += (now_ns() - tg) * 1e-9;
@@ -959,6 +1013,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       if ((rc = grow_pool(g, c.pool_top, 0))) return rc;
       g.h_grow += (now_ns() - tg) * 1e-9;
     }
+    c_ok = c;
     s0 += B;
     ++batches;
     const double dt = (now_ns() - tb) * 1e-9;

@@ -79,6 +79,19 @@ def test_wide_weights_use_32bit_tiles():
         same_labels(idx, O.clean(want).arrays())
 
 
+@pytest.mark.parametrize("mode,hub_batch", [("plain", 0), ("batched", 7), ("batched", 0)])
+def test_slot_tags_wrap_with_wide_weights(mode, hub_batch):
+    """K * max weight near 2^27 leaves four tag bits in a dense slot, so the
+    slot tags run out every 15 batch attempts and the used rows are cleared;
+    the labels must not change."""
+    g = H.Graph.rmat(9, 8, 777, 1, 1 << 24)
+    c = csr_of(g)
+    m = H.MODE_PLAIN if mode == "plain" else H.MODE_BATCHED
+    idx = H.hop_constrained_two_hop_labels_generation(g, 5, mode=m, hub_batch=hub_batch)
+    want = O.kround(c, 5)
+    same_labels(idx, want.arrays())
+
+
 def test_deadline_reports_timeout(c1):
     g, _ = c1
     with pytest.raises(H.HSDLError) as e:
