@@ -320,34 +320,68 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_group(GenArgs a) {
+// Scatter of the flat candidates into per-hub runs.  A workgroup takes 2048
+// consecutive entries, counts their hubs in an LDS hash, reserves each hub's
+// space with one global atomic and writes the entries: one global atomic per
+// (chunk, hub) instead of one per (wave, hub), so hubs with millions of
+// candidates do not serialize the scatter on their counters.
+constexpr uint32_t kGroupThreads = 256;
+constexpr uint32_t kGroupPer = 8;                                 // entries per thread
+constexpr uint32_t kGroupChunk = kGroupThreads * kGroupPer;       // 2048
+constexpr uint32_t kGroupSlots = 2 * kGroupChunk;                 // load factor <= 1/2
+static_assert(kGroupSlots == 4096, "k_group hashes hubs to 12 bits");
+
+__global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
+  __shared__ uint32_t hkey[kGroupSlots];
+  __shared__ uint32_t hcnt[kGroupSlots];
+  __shared__ uint32_t hbase[kGroupSlots];
   if (a.ctrl->overflow) return;
   const uint32_t cnt = a.ctrl->ccnt;
-  const int lane = lane_id();
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint64_t nthreads = (uint64_t)gridDim.x * blockDim.x;
-  // whole waves iterate together so the ballots see every lane
-  for (uint64_t base = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) - lane; base < cnt;
-       base += nthreads) {
-    const uint64_t i = base + lane;
-    const bool valid = i < cnt;
-    uint32_t b = 0, y = 0;
-    if (valid) {
-      const unsigned long long e = a.cflat[i];
-      b = (uint32_t)(e >> 32);
-      y = (uint32_t)e;
+  const uint32_t tid = threadIdx.x;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * kGroupChunk; c0 < cnt;
+       c0 += (uint64_t)gridDim.x * kGroupChunk) {
+    for (uint32_t i = tid; i < kGroupSlots; i += kGroupThreads) {
+      hkey[i] = kInf32;
+      hcnt[i] = 0;
     }
-    unsigned long long pend = __ballot(valid);
-    while (pend) {
-      const int leader = __ffsll((long long)pend) - 1;
-      const uint32_t bl = __shfl(b, leader, kWave);
-      const unsigned long long grp = __ballot(valid && b == bl);
-      uint32_t basepos = 0;
-      if (lane == leader) basepos = atomicAdd(&a.hub_cur[bl], (uint32_t)__popcll(grp));
-      basepos = __shfl(basepos, leader, kWave);
-      if (valid && b == bl) a.cgroup[a.hub_off[bl] + basepos + __popcll(grp & lt_mask)] = y;
-      pend &= ~grp;
+    __syncthreads();
+    uint32_t slot[kGroupPer], yv[kGroupPer];
+#pragma unroll
+    for (uint32_t r = 0; r < kGroupPer; ++r) {
+      const uint64_t i = c0 + r * kGroupThreads + tid;
+      slot[r] = kInf32;
+      yv[r] = 0;
+      if (i < cnt) {
+        const unsigned long long e = a.cflat[i];
+        const uint32_t b = (uint32_t)(e >> 32);
+        yv[r] = (uint32_t)e;
+        uint32_t sl = (b * 0x9E3779B1u) >> 20;  // 12 bits: kGroupSlots = 4096
+        for (;;) {
+          const uint32_t prev = atomicCAS(&hkey[sl], kInf32, b);
+          if (prev == kInf32 || prev == b) break;
+          sl = (sl + 1) & (kGroupSlots - 1);
+        }
+        slot[r] = sl;
+        atomicAdd(&hcnt[sl], 1u);
+      }
     }
+    __syncthreads();
+    for (uint32_t i = tid; i < kGroupSlots; i += kGroupThreads) {
+      const uint32_t c = hcnt[i];
+      if (c) {
+        const uint32_t b = hkey[i];
+        hbase[i] = a.hub_off[b] + atomicAdd(&a.hub_cur[b], c);
+        hcnt[i] = 0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kGroupPer; ++r) {
+      if (slot[r] == kInf32) continue;
+      const uint32_t rank = atomicAdd(&hcnt[slot[r]], 1u);
+      a.cgroup[hbase[slot[r]] + rank] = yv[r];
+    }
+    __syncthreads();
   }
 }
 
@@ -785,7 +819,7 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 1], g.st));
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
-    hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(256), 0, g.st, a);
+    hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
     if (g.small_vals)
       hipLaunchKernelGGL(k_prune<uint16_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
