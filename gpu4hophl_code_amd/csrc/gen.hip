@@ -29,14 +29,19 @@
 //                  remaining degrees in registers and walks the concatenated
 //                  edges 64 at a time (coalesced 8 B (col,w) loads);
 //                  candidates are deduplicated by atomicMin on a dense
-//                  per-(hub,vertex) slot and the first toucher appends
-//                  (hub,vertex) to a flat list
+//                  per-(hub,vertex) slot that keeps the batch's earlier
+//                  rounds' minima (tagged per batch attempt); the first
+//                  toucher that undercuts them appends (hub,vertex) to a flat
+//                  list and the per-hub counts are tallied per wave
 //   chunk_scan  -> per-hub offsets and work items of `tile` candidates
-//   group       -> flat candidates scattered into per-hub runs
-//   prune       -> persistent: a workgroup takes a work item, stages the hub's
-//                  labels L(s) in an LDS hash tile (hub -> prefix-min dist per
-//                  hop budget), then one wavefront per candidate streams L(y)
-//                  64 labels at a time and stops at the first witness (ballot)
+//   group       -> flat candidates scattered into per-hub runs (hubs tallied
+//                  per 2048-entry chunk in an LDS hash: one global
+//                  reservation per chunk and hub)
+//   prune       -> persistent: a workgroup of 16 waves takes a work item,
+//                  stages the hub's labels L(s) in an LDS hash tile (hub ->
+//                  prefix-min dist per hop budget), then the waves take its
+//                  candidates one at a time, stream L(y) 128 labels per step
+//                  with the next step in flight and stop at the first witness
 //   reserve     -> grows label segments that the round's accepted labels
 //                  overflow (bump allocation in a device pool; no locks)
 //   append      -> writes accepted labels; they also form the next frontier
