@@ -867,6 +867,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint64_t t_start = now_ns();
   Gen g;
   g.verbose = getenv("HSDL_VERBOSE") != nullptr;
+  // HSDL_PROGRESS=<seconds>: one line on stderr per interval (long builds)
+  const double progress_s = (double)env_u32("HSDL_PROGRESS", 0, 0, 86400);
+  uint64_t next_progress = 0;
   HSDL_HIP(hipStreamCreateWithFlags(&g.st, hipStreamNonBlocking));
   struct StreamGuard {
     hipStream_t s;
@@ -1062,6 +1065,16 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
               (unsigned long long)c.cand_total, (unsigned long long)c.pool_top,
               (unsigned long long)g.a.pool_cap);
     if (!plain && !opts->hub_batch) tuner.next(dt);
+    if (progress_s > 0) {
+      const uint64_t now = now_ns();
+      if (!next_progress) next_progress = t_csr + (uint64_t)(progress_s * 1e9);
+      if (now >= next_progress) {
+        fprintf(stderr, "[hsdl] progress: %u of %u hubs, %llu batches, %.1f s\n", s0, n,
+                (unsigned long long)batches, (now - t_csr) * 1e-9);
+        fflush(stderr);
+        next_progress = now + (uint64_t)(progress_s * 1e9);
+      }
+    }
     if (opts->deadline_s > 0 && (now_ns() - t_start) * 1e-9 > opts->deadline_s)
       return fail(HSDL_ETIMEDOUT, "generation passed its deadline");
   }

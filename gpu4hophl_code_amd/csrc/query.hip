@@ -155,12 +155,21 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
     // (s,t,k) of q+2G (pv, lanes 0-2).
     const bool is_merge = merge_job(cur, merge_cap);
     const uint32_t total = cur.ls + cur.lt;
+    const uint32_t ls = cur.ls;
+    // hash of A's hub groups: u16 indexes into the staged A, two per word,
+    // 2^tbits >= 2|A| slots (<= 4096 since |A| <= 2047), right after A
+    const uint32_t tbits = ls ? max(6u, 32u - (uint32_t)__builtin_clz(2 * ls - 1)) : 6u;
+    const uint32_t tmask = (1u << tbits) - 1u;
+    uint32_t *T = reinterpret_cast<uint32_t *>(stage + ls);
+    unsigned long long vc[kLoadPerLane];  // this query's labels; v takes the next one's
     if (is_merge) {
 #pragma unroll
       for (uint32_t u = 0; u < kLoadPerLane; ++u) {
+        vc[u] = v[u];
         const uint32_t i = u * kQThreads + tid;
-        if (i < total) stage[i] = v[u];
+        if (i < ls) stage[i] = vc[u];
       }
+      for (uint32_t j = tid; j < (1u << (tbits - 1)); j += kQThreads) T[j] = kInf32;
     }
     __syncthreads();
     // ---- prefetch: labels of q+G, ranges of q+2G, (s,t,k) of q+3G
@@ -186,32 +195,50 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
     uint32_t best = kInf32;
     const uint32_t kk = cur.kk;
     if (is_merge) {
-      const unsigned long long *SA = stage, *SB = stage + cur.ls;
-      const uint32_t ls = cur.ls, lt = cur.lt;
-      const uint32_t per = (total + kQThreads - 1) / kQThreads;
-      const uint32_t d0 = min(tid * per, total), d1 = min(d0 + per, total);
-      uint32_t lo = d0 > lt ? d0 - lt : 0, hi = min(d0, ls);
-      while (lo < hi) {  // first i whose A label does not precede B[d0-1-i]
-        const uint32_t mid = (lo + hi) >> 1;
-        if (label_hub(SA[mid]) <= label_hub(SB[d0 - 1 - mid])) lo = mid + 1; else hi = mid;
+      const unsigned long long *SA = stage;
+      // insert the group heads of A (hubs are unique among heads)
+      for (uint32_t i = tid; i < ls; i += kQThreads) {
+        const uint32_t w = label_hub(SA[i]);
+        if (i > 0 && label_hub(SA[i - 1]) == w) continue;
+        uint32_t slot = (w * 0x9E3779B1u) >> (32 - tbits);
+        for (;;) {
+          uint32_t *wp = &T[slot >> 1];
+          const uint32_t sh = (slot & 1u) << 4;
+          uint32_t word = *wp;
+          bool done = false;
+          while (((word >> sh) & 0xFFFFu) == 0xFFFFu) {
+            const uint32_t got = atomicCAS(wp, word, (word & ~(0xFFFFu << sh)) | (i << sh));
+            if (got == word) { done = true; break; }
+            word = got;
+          }
+          if (done) break;
+          slot = (slot + 1) & tmask;
+        }
       }
-      uint32_t i = lo, j = d0 - lo;
-      for (uint32_t d = d0; d < d1; ++d) {
-        const bool takeA = i < ls && (j >= lt || label_hub(SA[i]) <= label_hub(SB[j]));
-        if (takeA) {
-          const unsigned long long a = SA[i];
-          const uint32_t w = label_hub(a), h1 = label_hop(a), da = label_dist(a);
-          if (h1 <= kk) {
-            for (uint32_t p = j; p < lt; ++p) {
-              const unsigned long long b = SB[p];
-              if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend in a run
-              const uint32_t dsum = da + label_dist(b);
+      __syncthreads();
+      // each lane probes its labels of B (held in registers) and walks A's
+      // run of that hub (<= K+1 labels, hops ascending) while h1 + h2 <= k
+#pragma unroll
+      for (uint32_t u = 0; u < kLoadPerLane; ++u) {
+        const uint32_t i = u * kQThreads + tid;
+        if (i < ls || i >= total) continue;
+        const unsigned long long b = vc[u];
+        const uint32_t w = label_hub(b), h2 = label_hop(b), d2 = label_dist(b);
+        if (h2 > kk) continue;
+        uint32_t slot = (w * 0x9E3779B1u) >> (32 - tbits);
+        for (;;) {
+          const uint32_t e = (T[slot >> 1] >> ((slot & 1u) << 4)) & 0xFFFFu;
+          if (e == 0xFFFFu) break;
+          if (label_hub(SA[e]) == w) {
+            for (uint32_t p = e; p < ls; ++p) {
+              const unsigned long long a = SA[p];
+              if (label_hub(a) != w || label_hop(a) + h2 > kk) break;
+              const uint32_t dsum = label_dist(a) + d2;
               best = dsum < best ? dsum : best;
             }
+            break;
           }
-          ++i;
-        } else {
-          ++j;
+          slot = (slot + 1) & tmask;
         }
       }
     } else if (cur.fixed < 0) {
