@@ -150,6 +150,12 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
   return v;
 }
 
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
 // Inclusive prefix sum across the wave.
 __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
   const int l = lane_id();

@@ -90,6 +90,19 @@ struct Ctrl {
   uint32_t total_chunks;
   uint32_t overflow;
   uint32_t maxlen;
+  uint32_t expand_ctr;  // next 64-item chunk of the frontier (k_expand)
+  uint32_t heavy_cnt;   // pieces pushed to the heavy list this round
+  uint32_t heavy_ctr;   // next heavy piece (k_expand_heavy)
+};
+
+// A piece of a high-degree frontier item: up to kHeavyPiece consecutive
+// edges, relaxed by a whole workgroup.
+struct HeavyItem {
+  uint32_t b;
+  uint32_t d;
+  unsigned long long beg;
+  uint32_t cnt;
+  uint32_t pad;
 };
 
 struct GenArgs {
@@ -124,6 +137,8 @@ struct GenArgs {
   uint32_t *chunk_off;
   Item *F0;
   Item *F1;
+  HeavyItem *heavy;
+  uint32_t heavy_cap;
   uint32_t *gtile;
   unsigned long long gtile_cap;  // u32 words
   Ctrl *ctrl;
@@ -193,6 +208,9 @@ __global__ void k_round_prep(GenArgs a) {
     a.ctrl->ccnt = 0;
     a.ctrl->touched_cnt = 0;
     a.ctrl->work_ctr = 0;
+    a.ctrl->expand_ctr = 0;
+    a.ctrl->heavy_cnt = 0;
+    a.ctrl->heavy_ctr = 0;
   }
 }
 
@@ -207,15 +225,65 @@ __device__ inline unsigned long long first_above(const uint2 *adj, unsigned long
   return beg;
 }
 
+// Frontier items with more than kHeavyDeg edges above the hub are cut into
+// pieces of kHeavyPiece edges for k_expand_heavy, so no wave is left alone
+// with a hub-sized adjacency list while the rest of the GPU idles.
+constexpr uint32_t kHeavyDeg = 256;
+constexpr uint32_t kHeavyPiece = 1024;
+
+// Relaxes edge e of hub slot kb at distance kd (when valid): atomicMin on the
+// tagged slot; first touches that undercut earlier rounds go to the flat
+// list (one list atomic per wave, one count atomic per hub).  Called by all
+// lanes of a wave.
+__device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uint32_t kd,
+                                  unsigned long long e, unsigned long long lt_mask) {
+  const int lane = lane_id();
+  bool first = false;
+  uint32_t y = 0;
+  if (valid) {
+    const uint2 ed = a.adj[e];
+    y = ed.x;
+    const uint32_t nv = slot_fresh(a, kd + ed.y);
+    const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], nv);
+    first = nv < old && slot_opens(a, old);
+  }
+  const unsigned long long firsts = __ballot(first);
+  if (firsts) {
+    uint32_t fbase = 0;
+    const int lead = __ffsll((long long)firsts) - 1;
+    if (lane == lead) fbase = atomicAdd(&a.ctrl->ccnt, (uint32_t)__popcll(firsts));
+    fbase = __shfl(fbase, lead, kWave);
+    if (first) {
+      const uint32_t pos = fbase + (uint32_t)__popcll(firsts & lt_mask);
+      if (pos < a.ccap)
+        a.cflat[pos] = ((unsigned long long)kb << 32) | y;
+      else
+        atomicOr(&a.ctrl->overflow, kOvfCand);
+    }
+    unsigned long long pend = firsts;
+    while (pend) {
+      const int leader = __ffsll((long long)pend) - 1;
+      const uint32_t bl = __shfl(kb, leader, kWave);
+      const unsigned long long grp = __ballot(first && kb == bl);
+      if (lane == leader) atomicAdd(&a.hub_ccnt[bl], (uint32_t)__popcll(grp));
+      pend &= ~grp;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   const Item *F = frontier(a, a.cur);
   const uint32_t fcnt = min(a.ctrl->fcnt[a.cur], a.fcap);
   const int lane = lane_id();
-  const uint64_t wave = (blockIdx.x * (uint64_t)blockDim.x + threadIdx.x) / kWave;
-  const uint64_t nwaves = ((uint64_t)gridDim.x * blockDim.x) / kWave;
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t edges_seen = 0;
-  for (uint64_t base = wave * kWave; base < fcnt; base += nwaves * kWave) {
+  for (;;) {
+    // waves claim 64-item chunks of the frontier
+    uint32_t chunk = 0;
+    if (lane == 0) chunk = atomicAdd(&a.ctrl->expand_ctr, 1u);
+    chunk = __shfl(chunk, 0, kWave);
+    const uint64_t base = (uint64_t)chunk * kWave;
+    if (base >= fcnt) break;
     const uint64_t i = base + lane;
     uint32_t ib = 0, id = 0;
     unsigned long long beg = 0;
@@ -227,11 +295,27 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       const unsigned long long e0 = a.off[it.y], e1 = a.off[it.y + 1];
       beg = first_above(a.adj, e0, e1, a.s0 + it.b);  // rank pruning y > s
       deg = (uint32_t)(e1 - beg);
+      if (deg > kHeavyDeg) {
+        const uint32_t pieces = (deg + kHeavyPiece - 1) / kHeavyPiece;
+        const uint32_t pos = atomicAdd(&a.ctrl->heavy_cnt, pieces);
+        if (pos + pieces <= a.heavy_cap) {
+          for (uint32_t j = 0; j < pieces; ++j)
+            a.heavy[pos + j] = HeavyItem{ib, id, beg + (unsigned long long)j * kHeavyPiece,
+                                         min(kHeavyPiece, deg - j * kHeavyPiece), 0u};
+          edges_seen += deg;
+          deg = 0;
+        } else {
+          // no room: this wave keeps the item; reserved slots below the cap
+          // are written as empty pieces
+          for (uint32_t j = 0; j < pieces && pos + j < a.heavy_cap; ++j)
+            a.heavy[pos + j] = HeavyItem{0u, 0u, 0ull, 0u, 0u};
+        }
+      }
     }
     const uint32_t incl = wave_incl_scan_u32(deg);
     const uint32_t excl = incl - deg;
     const uint32_t total = __shfl(incl, kWave - 1, kWave);
-    edges_seen += total;
+    if (lane == 0) edges_seen += total;
     for (uint32_t jb = 0; jb < total; jb += kWave) {
       const uint32_t j = jb + lane;
       // largest k with excl_k <= j (items with no edges are skipped over)
@@ -245,41 +329,34 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       const uint32_t kd = __shfl(id, k, kWave);
       const uint32_t kex = __shfl(excl, k, kWave);
       const unsigned long long kbeg = __shfl(beg, k, kWave);
-      bool first = false;
-      uint32_t y = 0;
-      if (j < total) {
-        const uint2 ed = a.adj[kbeg + (j - kex)];
-        y = ed.x;
-        const uint32_t nv = slot_fresh(a, kd + ed.y);
-        const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], nv);
-        first = nv < old && slot_opens(a, old);
-      }
-      // first touches: one flat-list atomic per wave, one count atomic per hub
-      const unsigned long long firsts = __ballot(first);
-      if (firsts) {
-        uint32_t fbase = 0;
-        const int lead = __ffsll((long long)firsts) - 1;
-        if (lane == lead) fbase = atomicAdd(&a.ctrl->ccnt, (uint32_t)__popcll(firsts));
-        fbase = __shfl(fbase, lead, kWave);
-        if (first) {
-          const uint32_t pos = fbase + (uint32_t)__popcll(firsts & lt_mask);
-          if (pos < a.ccap)
-            a.cflat[pos] = ((unsigned long long)kb << 32) | y;
-          else
-            atomicOr(&a.ctrl->overflow, kOvfCand);
-        }
-        unsigned long long pend = firsts;
-        while (pend) {
-          const int leader = __ffsll((long long)pend) - 1;
-          const uint32_t bl = __shfl(kb, leader, kWave);
-          const unsigned long long grp = __ballot(first && kb == bl);
-          if (lane == leader) atomicAdd(&a.hub_ccnt[bl], (uint32_t)__popcll(grp));
-          pend &= ~grp;
-        }
-      }
+      relax_edge(a, j < total, kb, kd, kbeg + (j - kex), lt_mask);
     }
   }
+  edges_seen = wave_sum_u64(edges_seen);
   if (lane == 0 && edges_seen) atomicAdd(&a.ctrl->edge_total, (unsigned long long)edges_seen);
+}
+
+// Heavy pieces: a workgroup takes one piece at a time and spreads its edges
+// over its 256 lanes.
+__global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
+  __shared__ uint32_t sh_h;
+  const int lane = lane_id();
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint32_t wv = threadIdx.x / kWave;
+  const uint32_t cnt = min(a.ctrl->heavy_cnt, a.heavy_cap);
+  for (;;) {
+    if (threadIdx.x == 0) sh_h = atomicAdd(&a.ctrl->heavy_ctr, 1u);
+    __syncthreads();
+    const uint32_t h = sh_h;
+    __syncthreads();
+    if (h >= cnt) break;
+    const HeavyItem it = a.heavy[h];
+    // whole waves iterate together so the ballots see every lane
+    for (uint32_t eb = wv * kWave; eb < it.cnt; eb += 256) {
+      const uint32_t e = eb + (uint32_t)lane;
+      relax_edge(a, e < it.cnt, it.b, it.d, it.beg + e, lt_mask);
+    }
+  }
 }
 
 // ------------------------------------------------------------ chunk scan
@@ -714,7 +791,7 @@ static int upload_graph(hsdl_graph *g, int dev) {
 struct Gen {
   GenArgs a{};
   DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
-  DevBuf cand_d, cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl;
+  DevBuf cand_d, cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
@@ -822,6 +899,7 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_round_prep, dim3(small_grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 0], g.st));
     hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
+    hipLaunchKernelGGL(k_expand_heavy, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 1], g.st));
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
@@ -906,7 +984,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   if (plain) Bmax = 1;
   if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
   if (Bmax == 0) return fail(HSDL_ENOMEM, "HBM budget below one hub's dedupe slots");
-  g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20);
+  const uint32_t heavy_cap = 1u << 20;  // heavy pieces per round (24 MB)
+  g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20) +
+            (uint64_t)heavy_cap * sizeof(HeavyItem);
   if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 36)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
@@ -916,7 +996,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       (rc = g.touched.alloc(n * 4ull)) || (rc = g.cand_d.alloc((uint64_t)Bmax * n * 4)) ||
       (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
-      (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))))
+      (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))) ||
+      (rc = g.heavy.alloc((uint64_t)heavy_cap * sizeof(HeavyItem))))
     return rc;
 
   // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
@@ -962,6 +1043,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.gtile = g.gtile.as<uint32_t>();
   a.gtile_cap = gtile0;
   a.ctrl = g.ctrl.as<Ctrl>();
+  a.heavy = g.heavy.as<HeavyItem>();
+  a.heavy_cap = heavy_cap;
   if ((rc = alloc_lists(g, list0, list0))) return rc;
 
   {
