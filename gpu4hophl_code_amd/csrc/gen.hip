@@ -231,12 +231,29 @@ __device__ inline unsigned long long first_above(const uint2 *adj, unsigned long
 constexpr uint32_t kHeavyDeg = 256;
 constexpr uint32_t kHeavyPiece = 1024;
 
+// Flat-list space is reserved per wave in blocks of kListBlock entries (one
+// atomic on the list counter per block instead of one per step); unused
+// tails are filled with empty entries (hub slot kInf32) that the grouping
+// skips.
+constexpr uint32_t kListBlock = 512;
+constexpr unsigned long long kEmptyEntry = ~0ull;
+
+struct ListRes {
+  uint32_t base = 0, used = kListBlock;  // starts exhausted
+};
+
+// Writes empty entries over [base+used, base+kListBlock) (whole wave).
+__device__ inline void list_pad(const GenArgs &a, const ListRes &r) {
+  for (uint32_t i = r.used + (uint32_t)lane_id(); i < kListBlock; i += kWave)
+    if (r.base + i < a.ccap) a.cflat[r.base + i] = kEmptyEntry;
+}
+
 // Relaxes edge e of hub slot kb at distance kd (when valid): atomicMin on the
 // tagged slot; first touches that undercut earlier rounds go to the flat
-// list (one list atomic per wave, one count atomic per hub).  Called by all
+// list (the wave's reserved block) and are counted per hub.  Called by all
 // lanes of a wave.
 __device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uint32_t kd,
-                                  unsigned long long e, unsigned long long lt_mask) {
+                                  unsigned long long e, unsigned long long lt_mask, ListRes &res) {
   const int lane = lane_id();
   bool first = false;
   uint32_t y = 0;
@@ -249,17 +266,22 @@ __device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uin
   }
   const unsigned long long firsts = __ballot(first);
   if (firsts) {
-    uint32_t fbase = 0;
-    const int lead = __ffsll((long long)firsts) - 1;
-    if (lane == lead) fbase = atomicAdd(&a.ctrl->ccnt, (uint32_t)__popcll(firsts));
-    fbase = __shfl(fbase, lead, kWave);
+    const uint32_t cnt = (uint32_t)__popcll(firsts);
+    if (res.used + cnt > kListBlock) {
+      list_pad(a, res);
+      uint32_t nb = 0;
+      if (lane == 0) nb = atomicAdd(&a.ctrl->ccnt, kListBlock);
+      res.base = __shfl(nb, 0, kWave);
+      res.used = 0;
+    }
     if (first) {
-      const uint32_t pos = fbase + (uint32_t)__popcll(firsts & lt_mask);
+      const uint32_t pos = res.base + res.used + (uint32_t)__popcll(firsts & lt_mask);
       if (pos < a.ccap)
         a.cflat[pos] = ((unsigned long long)kb << 32) | y;
       else
         atomicOr(&a.ctrl->overflow, kOvfCand);
     }
+    res.used += cnt;
     unsigned long long pend = firsts;
     while (pend) {
       const int leader = __ffsll((long long)pend) - 1;
@@ -277,6 +299,7 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   const int lane = lane_id();
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t edges_seen = 0;
+  ListRes res;
   for (;;) {
     // waves claim 64-item chunks of the frontier
     uint32_t chunk = 0;
@@ -329,9 +352,10 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       const uint32_t kd = __shfl(id, k, kWave);
       const uint32_t kex = __shfl(excl, k, kWave);
       const unsigned long long kbeg = __shfl(beg, k, kWave);
-      relax_edge(a, j < total, kb, kd, kbeg + (j - kex), lt_mask);
+      relax_edge(a, j < total, kb, kd, kbeg + (j - kex), lt_mask, res);
     }
   }
+  list_pad(a, res);
   edges_seen = wave_sum_u64(edges_seen);
   if (lane == 0 && edges_seen) atomicAdd(&a.ctrl->edge_total, (unsigned long long)edges_seen);
 }
@@ -344,6 +368,7 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t wv = threadIdx.x / kWave;
   const uint32_t cnt = min(a.ctrl->heavy_cnt, a.heavy_cap);
+  ListRes res;
   for (;;) {
     if (threadIdx.x == 0) sh_h = atomicAdd(&a.ctrl->heavy_ctr, 1u);
     __syncthreads();
@@ -354,9 +379,10 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
     // whole waves iterate together so the ballots see every lane
     for (uint32_t eb = wv * kWave; eb < it.cnt; eb += 256) {
       const uint32_t e = eb + (uint32_t)lane;
-      relax_edge(a, e < it.cnt, it.b, it.d, it.beg + e, lt_mask);
+      relax_edge(a, e < it.cnt, it.b, it.d, it.beg + e, lt_mask, res);
     }
   }
+  list_pad(a, res);
 }
 
 // ------------------------------------------------------------ chunk scan
@@ -418,7 +444,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
   __shared__ uint32_t hcnt[kGroupSlots];
   __shared__ uint32_t hbase[kGroupSlots];
   if (a.ctrl->overflow) return;
-  const uint32_t cnt = a.ctrl->ccnt;
+  // reserved blocks may reach past the list's end; only empty entries lie there
+  const uint32_t cnt = min(a.ctrl->ccnt, a.ccap);
   const uint32_t tid = threadIdx.x;
   for (uint64_t c0 = (uint64_t)blockIdx.x * kGroupChunk; c0 < cnt;
        c0 += (uint64_t)gridDim.x * kGroupChunk) {
@@ -433,8 +460,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
       const uint64_t i = c0 + r * kGroupThreads + tid;
       slot[r] = kInf32;
       yv[r] = 0;
-      if (i < cnt) {
-        const unsigned long long e = a.cflat[i];
+      const unsigned long long e = i < cnt ? a.cflat[i] : kEmptyEntry;
+      if (e != kEmptyEntry) {
         const uint32_t b = (uint32_t)(e >> 32);
         yv[r] = (uint32_t)e;
         uint32_t sl = (b * 0x9E3779B1u) >> 20;  // 12 bits: kGroupSlots = 4096
