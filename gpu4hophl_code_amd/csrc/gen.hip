@@ -240,12 +240,19 @@ constexpr unsigned long long kEmptyEntry = ~0ull;
 
 struct ListRes {
   uint32_t base = 0, used = kListBlock;  // starts exhausted
+  uint32_t hub = kInf32, hub_cnt = 0;    // per-hub count not yet added to hub_ccnt
 };
 
 // Writes empty entries over [base+used, base+kListBlock) (whole wave).
 __device__ inline void list_pad(const GenArgs &a, const ListRes &r) {
   for (uint32_t i = r.used + (uint32_t)lane_id(); i < kListBlock; i += kWave)
     if (r.base + i < a.ccap) a.cflat[r.base + i] = kEmptyEntry;
+}
+
+// End of a kernel: pad the open block and add the pending hub count.
+__device__ inline void list_close(const GenArgs &a, const ListRes &r) {
+  list_pad(a, r);
+  if (lane_id() == 0 && r.hub_cnt) atomicAdd(&a.hub_ccnt[r.hub], r.hub_cnt);
 }
 
 // Relaxes edge e of hub slot kb at distance kd (when valid): atomicMin on the
@@ -282,12 +289,19 @@ __device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uin
         atomicOr(&a.ctrl->overflow, kOvfCand);
     }
     res.used += cnt;
+    // per-hub counts: a wave's steps mostly share one hub, so the count for
+    // the current hub stays in a register until the hub changes
     unsigned long long pend = firsts;
     while (pend) {
       const int leader = __ffsll((long long)pend) - 1;
       const uint32_t bl = __shfl(kb, leader, kWave);
       const unsigned long long grp = __ballot(first && kb == bl);
-      if (lane == leader) atomicAdd(&a.hub_ccnt[bl], (uint32_t)__popcll(grp));
+      if (bl != res.hub) {
+        if (lane == 0 && res.hub_cnt) atomicAdd(&a.hub_ccnt[res.hub], res.hub_cnt);
+        res.hub = bl;
+        res.hub_cnt = 0;
+      }
+      res.hub_cnt += (uint32_t)__popcll(grp);
       pend &= ~grp;
     }
   }
@@ -355,7 +369,7 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       relax_edge(a, j < total, kb, kd, kbeg + (j - kex), lt_mask, res);
     }
   }
-  list_pad(a, res);
+  list_close(a, res);
   edges_seen = wave_sum_u64(edges_seen);
   if (lane == 0 && edges_seen) atomicAdd(&a.ctrl->edge_total, (unsigned long long)edges_seen);
 }
@@ -382,7 +396,7 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
       relax_edge(a, e < it.cnt, it.b, it.d, it.beg + e, lt_mask, res);
     }
   }
-  list_pad(a, res);
+  list_close(a, res);
 }
 
 // ------------------------------------------------------------ chunk scan
