@@ -603,8 +603,9 @@ template <class V>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune(GenArgs a) {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next;
+  __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next, sh_touch_n, sh_touch_base;
   __shared__ Item sh_acc[256];
+  __shared__ uint32_t sh_touch[256];
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
   Item *Fn = frontier(a, a.cur ^ 1);
@@ -617,6 +618,7 @@ k_prune(GenArgs a) {
       sh_w = atomicAdd(&a.ctrl->work_ctr, 1u);
       sh_acc_n = 0;
       sh_next = 0;
+      sh_touch_n = 0;
     }
     __syncthreads();
     const uint32_t w = sh_w;
@@ -678,21 +680,31 @@ k_prune(GenArgs a) {
         if (!pruned) {
           const uint32_t p = atomicAdd(&sh_acc_n, 1u);
           sh_acc[p] = Item{b, y, dy};
-          if (atomicAdd(&a.add_cnt[y], 1u) == 0) {
-            const uint32_t t = atomicAdd(&a.ctrl->touched_cnt, 1u);
-            a.touched[t] = y;
-          }
         }
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0 && sh_acc_n) {
-      sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], sh_acc_n);
-      if (sh_acc_base + sh_acc_n > a.fcap) atomicOr(&a.ctrl->overflow, kOvfFront);
+    // the item's accepted labels: per-vertex add counts; vertices touched
+    // for the first time this round are gathered in LDS and published with
+    // one reservation per item (no per-label atomic on the shared counter)
+    const uint32_t nacc = sh_acc_n;
+    for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x) {
+      const uint32_t y = sh_acc[i].y;
+      if (atomicAdd(&a.add_cnt[y], 1u) == 0) sh_touch[atomicAdd(&sh_touch_n, 1u)] = y;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < sh_acc_n; i += blockDim.x)
+    if (threadIdx.x == 0) {
+      if (nacc) {
+        sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], nacc);
+        if (sh_acc_base + nacc > a.fcap) atomicOr(&a.ctrl->overflow, kOvfFront);
+      }
+      if (sh_touch_n) sh_touch_base = atomicAdd(&a.ctrl->touched_cnt, sh_touch_n);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x)
       if (sh_acc_base + i < a.fcap) Fn[sh_acc_base + i] = sh_acc[i];
+    for (uint32_t i = threadIdx.x; i < sh_touch_n; i += blockDim.x)
+      a.touched[sh_touch_base + i] = sh_touch[i];
     __syncthreads();
   }
   if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
