@@ -46,7 +46,8 @@ constexpr uint32_t kCleanWaves = 4;            // waves per workgroup (pass 2)
 constexpr uint32_t kSeqBitsPerWave = 65536;    // max labels per vertex (pass 2)
 constexpr uint32_t kSeqWordsPerWave = kSeqBitsPerWave / 32;
 constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vertex
-constexpr uint32_t kTileWindow = 16;           // labels of L(v) per work unit
+constexpr uint32_t kTileWindow = 8;            // labels of L(v) per work unit (default)
+constexpr uint32_t kMaxWindow = 64;            // largest window
 constexpr uint32_t kTileLdsBytes = 60 * 1024;  // LDS tile budget (two workgroups per CU)
 
 __device__ inline void wave_lds_sync() {
@@ -78,75 +79,69 @@ __device__ inline uint32_t tile_find(const uint32_t *keys, uint32_t cap, uint32_
 }
 
 // True when the all-earlier predicate removes a label of a group whose head
-// lies in L(v)[p0, p0 + kTileWindow).
+// lies in L(v)[p0, p0 + win).  The wave walks the window's heads in order
+// (uniform control flow: the group's labels are uniform scalar loads) and
+// streams each head's list L(u) 256 elements per step with coalesced loads;
+// only the filter and the tile probe are per lane.
 template <class V>
-__device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long *off,
-                               const unsigned long long *lab, const unsigned long long *Lv,
-                               uint32_t len, uint32_t p0, const uint32_t *keys, const V *vals,
+__device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off,
+                               const unsigned long long *__restrict__ lab,
+                               const unsigned long long *__restrict__ Lv, uint32_t len,
+                               uint32_t p0, uint32_t win, const uint32_t *keys, const V *vals,
                                uint32_t cap) {
   const uint32_t lane = (uint32_t)lane_id();
-  const uint32_t p = p0 + lane;
-  // the self label (hub v, last in L(v)) is always kept: no head for it
-  bool head = lane < kTileWindow && p < len && (p == 0 || label_hub(Lv[p]) != label_hub(Lv[p - 1]));
-  uint32_t u = 0, glen = 0, maxh = 0, maxd = 0, ulen = 0;
-  unsigned long long ub = 0;
-  if (head) {
-    u = label_hub(Lv[p]);
-    if (u == v) head = false;
-  }
-  if (head) {
+  bool hit = false;
+  const uint32_t pend = min(p0 + win, len);
+  for (uint32_t p = p0; p < pend; ++p) {
+    const uint32_t u = label_hub(Lv[p]);
+    if ((p > 0 && label_hub(Lv[p - 1]) == u) || u == v) continue;  // not a head / self label
+    uint32_t glen = 0, maxh = 0, maxd = 0;
     while (p + glen < len && label_hub(Lv[p + glen]) == u) {
       const unsigned long long g = Lv[p + glen];
       maxh = max(maxh, label_hop(g));
       maxd = max(maxd, label_dist(g));
       ++glen;
     }
-    ub = off[u];
-    ulen = (uint32_t)(off[u + 1] - ub);
-  }
-  const uint32_t incl = wave_incl_scan_u32(ulen);
-  const uint32_t excl = incl - ulen;
-  const uint32_t total = __shfl(incl, kWave - 1, kWave);
-  bool hit = false;
-  for (uint32_t jb = 0; jb < total; jb += kWave) {
-    const uint32_t j = jb + lane;
-    // owning head k: largest k with excl[k] <= j (binary lifting over lanes)
-    int k = 0;
+    const unsigned long long ub = off[u];
+    const uint32_t ulen = (uint32_t)(off[u + 1] - ub);
+    const unsigned long long *Lu = lab + ub;
+    constexpr int kSteps = 4;
+    for (uint32_t jb = 0; jb < ulen; jb += kSteps * kWave) {
+      unsigned long long el[kSteps];
 #pragma unroll
-    for (int step = 32; step > 0; step >>= 1) {
-      const uint32_t e = __shfl(excl, k + step, kWave);
-      if (e <= j) k += step;
-    }
-    const uint32_t ku = __shfl(u, k, kWave);
-    const uint32_t kglen = __shfl(glen, k, kWave);
-    const uint32_t kmaxh = __shfl(maxh, k, kWave);
-    const uint32_t kmaxd = __shfl(maxd, k, kWave);
-    const uint32_t kex = __shfl(excl, k, kWave);
-    const unsigned long long kub = __shfl(ub, k, kWave);
-    const uint32_t gs = p0 + (uint32_t)k;  // group start in L(v)
-    if (j >= total) continue;
-    const unsigned long long l = lab[kub + (j - kex)];
-    const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
-    if (w < ku) {
-      // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
-      if (h1 + 1 > kmaxh || d1 > kmaxd) continue;
-      const uint32_t slot = tile_find(keys, cap, w);
-      if (slot == kInf32) continue;
-      for (uint32_t t = 0; t < kglen; ++t) {
-        const unsigned long long g = Lv[gs + t];
-        const uint32_t hg = label_hop(g);
-        if (hg <= h1) continue;
-        const uint32_t m = vals[slot * K1 + (hg - h1)];
-        if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
+      for (int r = 0; r < kSteps; ++r) {
+        const uint32_t j = jb + r * kWave + lane;
+        el[r] = j < ulen ? Lu[j] : ~0ull;  // hub ~0u: never < u, never == u
       }
-    } else if (w == ku) {
-      // through hub u: earlier labels of the group against later ones
-      for (uint32_t t1 = 0; t1 < kglen; ++t1) {
-        const unsigned long long g1 = Lv[gs + t1];
-        for (uint32_t t2 = t1 + 1; t2 < kglen; ++t2) {
-          const unsigned long long g2 = Lv[gs + t2];
-          if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
-            hit = true;
+#pragma unroll
+      for (int r = 0; r < kSteps; ++r) {
+        const unsigned long long l = el[r];
+        const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
+        if (w < u) {
+          // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
+          if (h1 + 1 <= maxh && d1 <= maxd) {
+            const uint32_t slot = tile_find(keys, cap, w);
+            if (slot != kInf32) {
+              for (uint32_t t = 0; t < glen; ++t) {
+                const unsigned long long g = Lv[p + t];
+                const uint32_t hg = label_hop(g);
+                if (hg > h1) {
+                  const uint32_t m = vals[slot * K1 + (hg - h1)];
+                  if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
+                }
+              }
+            }
+          }
+        } else if (w == u) {
+          // through hub u: earlier labels of the group against later ones
+          for (uint32_t t1 = 0; t1 < glen; ++t1) {
+            const unsigned long long g1 = Lv[p + t1];
+            for (uint32_t t2 = t1 + 1; t2 < glen; ++t2) {
+              const unsigned long long g2 = Lv[p + t2];
+              if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
+                hit = true;
+            }
+          }
         }
       }
     }
@@ -158,9 +153,10 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
 // of gwords words for tiles above lds_slots slots.
 template <class V>
 __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8)))
-k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *off, const unsigned long long *lab,
+k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
+             const unsigned long long *__restrict__ lab,
              uint8_t *keep, uint32_t *new_len, uint32_t *redo, uint32_t *ctr, uint32_t lds_slots,
-             uint32_t *gtile, uint64_t gwords) {
+             uint32_t *gtile, uint64_t gwords, uint32_t win) {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_v, sh_win, sh_heads, sh_removed;
   const uint32_t K1 = K + 1;
@@ -174,7 +170,9 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *off, const unsign
       sh_removed = 0;
     }
     __syncthreads();
-    const uint32_t v = sh_v;
+    // wave-uniform values are made explicit, so the list offsets and hubs are
+    // scalar loads and the loops over them carry no per-lane masks
+    const uint32_t v = __builtin_amdgcn_readfirstlane(sh_v);
     if (v >= n) break;
     const unsigned long long vb = off[v];
     const uint32_t len = (uint32_t)(off[v + 1] - vb);
@@ -221,10 +219,10 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *off, const unsign
       if (!in_lds) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       for (;;) {
         uint32_t p0 = 0;
-        if (lane == 0) p0 = atomicAdd(&sh_win, kTileWindow);
-        p0 = __shfl(p0, 0, kWave);
+        if (lane == 0) p0 = atomicAdd(&sh_win, win);
+        p0 = __builtin_amdgcn_readfirstlane(__shfl(p0, 0, kWave));
         if (p0 >= len) break;
-        if (window_removes<V>(v, K1, off, lab, Lv, len, p0, keys, vals, cap)) {
+        if (window_removes<V>(v, K1, off, lab, Lv, len, p0, win, keys, vals, cap)) {
           if (lane == 0) sh_removed = 1;
           break;
         }
@@ -433,6 +431,8 @@ int clean_index(hsdl_index *idx) {
   lds_slots = env_u32("HSDL_CLEAN_LDS_SLOTS", lds_slots, 1, lds_slots);
   const size_t lds_bytes = (size_t)lds_slots * slot_bytes + 16;
   const int tgrid = std::max(1, cus * 2);
+  // tuning knob HSDL_CLEAN_WINDOW: labels of L(v) per wave work unit
+  const uint32_t win = env_u32("HSDL_CLEAN_WINDOW", kTileWindow, 1, kMaxWindow);
   const uint32_t gcap = tile_cap_for(mx[0]);
   const uint64_t gwords = small_vals ? tile_words<uint16_t>(gcap, K1) : tile_words<uint32_t>(gcap, K1);
   DevBuf gtile;
@@ -442,13 +442,13 @@ int clean_index(hsdl_index *idx) {
                        (uint32_t)idx->K, idx->off.as<unsigned long long>(),
                        idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
                        new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, lds_slots,
-                       gtile.as<uint32_t>(), gwords);
+                       gtile.as<uint32_t>(), gwords, win);
   else
     hipLaunchKernelGGL(k_clean_tile<uint32_t>, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n,
                        (uint32_t)idx->K, idx->off.as<unsigned long long>(),
                        idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
                        new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, lds_slots,
-                       gtile.as<uint32_t>(), gwords);
+                       gtile.as<uint32_t>(), gwords, win);
   HSDL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_clean_seq, dim3(grid), dim3(kCleanWaves * kWave), 0, st, (uint32_t)idx->K,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
