@@ -46,7 +46,7 @@ constexpr uint32_t kCleanWaves = 4;            // waves per workgroup (pass 2)
 constexpr uint32_t kSeqBitsPerWave = 65536;    // max labels per vertex (pass 2)
 constexpr uint32_t kSeqWordsPerWave = kSeqBitsPerWave / 32;
 constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vertex
-constexpr uint32_t kTileWindow = 8;            // labels of L(v) per work unit (default)
+constexpr uint32_t kTileWindow = 4;            // labels of L(v) per work unit (default)
 constexpr uint32_t kMaxWindow = 64;            // largest window
 constexpr uint32_t kTileLdsBytes = 60 * 1024;  // LDS tile budget (two workgroups per CU)
 
