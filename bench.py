@@ -149,14 +149,49 @@ def cpu_baseline(scale=15, K=5):
     g = H.Graph.rmat(scale, 16, 0x5EED0002, 1, 100)
     off, col, w = g.csr()
     c = O.CSR(len(off) - 1, off.astype(np.int64), col.astype(np.int32), w)
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     t0 = time.perf_counter()
     lab = O.kround(c, K, threads=threads)
     cl = O.clean(lab, 1, threads=threads)
     sec = time.perf_counter() - t0
     return {"value": cl.total() / sec, "unit": "labels/s", "cores": threads, "kind": "port",
             "sample": f"R-MAT scale {scale} (same generator, edge factor 16, K={K}), full "
-                      f"K-round build + clean by the oracle, {cl.total()} labels in {sec:.2f} s"}
+                      f"K-round build + clean by the oracle, {cl.total()} labels in {sec:.2f} s",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+
+
+def cpu_threads():
+    return min(16, os.cpu_count() or 1)  # the box's CPU share for one GPU
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_query_baseline(idx, n, K):
+    """The oracle's merge-join query (restated reference CPU query) on the same
+    cleaned index, one query per thread: 1 thread and the box's share."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    off, hub, hop, dist = idx.labels()
+    L = O.labels_from_arrays(n, off.astype(np.int64), hub, hop, dist)
+    rng = np.random.default_rng(0x5EED0207)
+    out = {"threads": cpu_threads(), "kind": "port"}
+    for nthr, nq in ((1, 100_000), (cpu_threads(), 1_000_000)):
+        s = rng.integers(0, n, nq).astype(np.uint32)
+        t = rng.integers(0, n, nq).astype(np.uint32)
+        k = rng.integers(1, K + 1, nq).astype(np.int32)
+        t0 = time.perf_counter()
+        O.query(L, s, t, k, threads=nthr)
+        out[f"qps_{nthr}_threads" if nthr > 1 else "qps_1_thread"] = nq / (time.perf_counter() - t0)
+    return out
 
 
 def main():
@@ -244,11 +279,28 @@ def main():
     top = max(rl, key=lambda x: rl[x]["seconds"])
     roofline = rl[top]
     roofline_other = {k: v for k, v in rl.items() if k != top}
-    # achievable copy bandwidth on this GPU (STREAM-style, read + write)
+    # achievable copy bandwidth on this GPU (STREAM-style, read + write) and
+    # the rate of random returning 32-bit atomicMin over 16 GB (the dense
+    # dedupe slots' operation: expansion is bound by it)
     import ctypes
     gbps = ctypes.c_double()
     if H.lib().hsdl_device_copy_bench(1 << 30, 5, ctypes.byref(gbps)) == 0:
         roofline["copy_gbps"] = round(gbps.value, 1)
+    gops = ctypes.c_double()
+    if H.lib().hsdl_device_atomic_bench(16 << 30, 1 << 30, ctypes.byref(gops)) == 0:
+        ex = rl["k_expand"]
+        ex["atomic_peak_gops"] = round(gops.value, 2)
+        ex["atomic_gops"] = round(stats["scanned_edges"] / stats["t_expand_s"] / 1e9, 2) \
+            if stats["t_expand_s"] > 0 else None
+    # the clean's first pass: 8 B per label of L(v) + 8 B per label of the
+    # heads' lists L(u) it streams (SURVEY's B_clean with stored label size)
+    if stats["t_clean_s"] > 0:
+        cb = 8 * stats["labels_uncleaned"] + 8 * stats["clean_label_reads"]
+        ca = cb / stats["t_clean_s"] / 1e9
+        roofline_other["k_clean_tile"] = {
+            "bound": "hbm", "achieved": round(ca, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ca / HBM_PEAK_GBS, 4), "traffic": None, "bytes": cb,
+            "seconds": stats["t_clean_s"], "note": "host-timed clean (both passes, compaction)"}
 
     # queries on the device-resident cleaned index
     lens = label_lengths(idx)
@@ -266,6 +318,10 @@ def main():
         serve = {"rates": serving(H, idx, n, rates, args.queries),
                  "plain_per_query": plain_per_query(H, idx, n, torch)}
         log(f"plain per-query launches: {serve['plain_per_query']}")
+
+    cpu_q = cpu_query_baseline(idx, n, K) if args.cpu_baseline == "on" else None
+    if cpu_q:
+        log(f"cpu query baseline: {cpu_q}")
 
     plain = None
     if args.plain == "on":
@@ -289,6 +345,7 @@ def main():
 
     cpu = cpu_baseline() if args.cpu_baseline == "on" else None
     if cpu:
+        cpu["query"] = cpu_q
         log(f"cpu baseline: {cpu}")
 
     line = {

@@ -66,7 +66,7 @@ class hsdl_stats(ctypes.Structure):
                 ("scanned_edges", ctypes.c_uint64), ("t_expand_s", ctypes.c_double),
                 ("t_prune_s", ctypes.c_double), ("prune_launches", ctypes.c_uint64),
                 ("prune_label_reads", ctypes.c_uint64), ("prune_tile_reads", ctypes.c_uint64),
-                ("clean_seq_vertices", ctypes.c_uint64)]
+                ("clean_seq_vertices", ctypes.c_uint64), ("clean_label_reads", ctypes.c_uint64)]
 
 
 class hsdl_server_opts(ctypes.Structure):
@@ -116,6 +116,7 @@ _SIGNATURES = [
     ("hsdl_server_destroy", None, [_vp]),
     ("hsdl_server_replay", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _u64p, _i32p, _u64p]),
     ("hsdl_device_copy_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    ("hsdl_device_atomic_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     ("hsdl_strerror", ctypes.c_char_p, [ctypes.c_int]),
     ("hsdl_last_error", ctypes.c_char_p, []),
     ("hsdl_now_ns", ctypes.c_uint64, []),

@@ -88,7 +88,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
                                const unsigned long long *__restrict__ lab,
                                const unsigned long long *__restrict__ Lv, uint32_t len,
                                uint32_t p0, uint32_t win, const uint32_t *keys, const V *vals,
-                               uint32_t cap) {
+                               uint32_t cap, uint64_t &streamed) {
   const uint32_t lane = (uint32_t)lane_id();
   bool hit = false;
   const uint32_t pend = min(p0 + win, len);
@@ -105,6 +105,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     const unsigned long long ub = off[u];
     const uint32_t ulen = (uint32_t)(off[u + 1] - ub);
     const unsigned long long *Lu = lab + ub;
+    streamed += ulen;
     constexpr int kSteps = 4;
     for (uint32_t jb = 0; jb < ulen; jb += kSteps * kWave) {
       unsigned long long el[kSteps];
@@ -149,7 +150,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
   return __ballot(hit) != 0;
 }
 
-// ctr[0]: redo count, ctr[2]: next vertex.  gtile: per-workgroup arena slices
+// ctr[0]: redo count, ctr[2]: next vertex, ctr[4..5]: labels of L(u) read (u64).  gtile: per-workgroup arena slices
 // of gwords words for tiles above lds_slots slots.
 template <class V>
 __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8)))
@@ -162,6 +163,7 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
   const uint32_t K1 = K + 1;
   const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id();
   uint32_t *gkeys = gtile + (uint64_t)blockIdx.x * gwords;
+  uint64_t streamed = 0;  // labels of L(u) lists this wave read
   for (;;) {
     if (tid == 0) {
       sh_v = atomicAdd(&ctr[2], 1u);
@@ -222,7 +224,7 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
         if (lane == 0) p0 = atomicAdd(&sh_win, win);
         p0 = __builtin_amdgcn_readfirstlane(__shfl(p0, 0, kWave));
         if (p0 >= len) break;
-        if (window_removes<V>(v, K1, off, lab, Lv, len, p0, win, keys, vals, cap)) {
+        if (window_removes<V>(v, K1, off, lab, Lv, len, p0, win, keys, vals, cap, streamed)) {
           if (lane == 0) sh_removed = 1;
           break;
         }
@@ -237,6 +239,8 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
     }
     __syncthreads();  // shared state and the tile are reused for the next vertex
   }
+  if (lane == 0 && streamed)
+    atomicAdd(reinterpret_cast<unsigned long long *>(ctr) + 2, (unsigned long long)streamed);
 }
 
 // Largest list length and largest label distance of an index (out[0], out[1]).
@@ -403,9 +407,9 @@ int clean_index(hsdl_index *idx) {
   DevBuf keep, new_len, noff, nlab, redo, ctr;
   if ((rc = keep.alloc(std::max<uint64_t>(idx->total, 1))) || (rc = new_len.alloc(n * 4ull)) ||
       (rc = noff.alloc((n + 1ull) * 8)) || (rc = redo.alloc(n * 4ull + 4)) ||
-      (rc = ctr.alloc(16)))
+      (rc = ctr.alloc(32)))
     return rc;
-  HSDL_HIP(hipMemsetAsync(ctr.p, 0, 16, st));
+  HSDL_HIP(hipMemsetAsync(ctr.p, 0, 32, st));
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, idx->device);
   const int grid = std::max(1, cus * 8);
@@ -455,8 +459,8 @@ int clean_index(hsdl_index *idx) {
                      keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
                      cnt + 1);
   HSDL_HIP(hipGetLastError());
-  uint32_t hc[2] = {0, 0};
-  HSDL_HIP(hipMemcpyAsync(hc, ctr.p, 8, hipMemcpyDeviceToHost, st));
+  uint32_t hc[6] = {0, 0, 0, 0, 0, 0};
+  HSDL_HIP(hipMemcpyAsync(hc, ctr.p, 24, hipMemcpyDeviceToHost, st));
   HSDL_HIP(hipStreamSynchronize(st));
   if (hc[1]) return fail(HSDL_ERANGE, "a vertex holds more than 65536 labels; clean not supported");
   if ((rc = scan_counts_u32(new_len.as<uint32_t>(), noff.as<uint64_t>(), n, st))) return rc;
@@ -475,6 +479,7 @@ int clean_index(hsdl_index *idx) {
   idx->stats.labels_cleaned = total;
   idx->stats.clean_passes = hc[0] ? 2 : 1;
   idx->stats.clean_seq_vertices = hc[0];
+  idx->stats.clean_label_reads = (uint64_t)hc[4] | ((uint64_t)hc[5] << 32);
   idx->stats.t_clean_s = (now_ns() - t0) * 1e-9;
   return HSDL_OK;
 }

@@ -578,10 +578,31 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
   const unsigned long long *L = a.pool + a.seg_start[y];
   const uint32_t len = a.seg_len[y];
   constexpr uint32_t kStep = 2 * kWave;
+  // witnesses are mostly top-ranked hubs near the front of L(y): the first
+  // two steps take 64 labels each (one step in flight), later steps 128
   unsigned long long c0 = 0, c1 = 0;
   if (lane < len) c0 = L[lane];
   if (kWave + lane < len) c1 = L[kWave + lane];
-  for (uint32_t j = 0; j < len; j += kStep) {
+  if (__ballot(probe_label<V>(a, c0, lane < len, dy, keys, vals, cap, K1))) {
+    reads = min(len, kWave);
+    return true;
+  }
+  if (len <= kWave) {
+    reads = len;
+    return false;
+  }
+  {
+    unsigned long long n0 = 0, n1 = 0;
+    if (kStep + lane < len) n0 = L[kStep + lane];
+    if (kStep + kWave + lane < len) n1 = L[kStep + kWave + lane];
+    if (__ballot(probe_label<V>(a, c1, kWave + lane < len, dy, keys, vals, cap, K1))) {
+      reads = min(len, kStep);
+      return true;
+    }
+    c0 = n0;
+    c1 = n1;
+  }
+  for (uint32_t j = kStep; j < len; j += kStep) {
     const uint32_t jn = j + kStep;
     unsigned long long n0 = 0, n1 = 0;
     if (jn + lane < len) n0 = L[jn + lane];

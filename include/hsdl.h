@@ -161,6 +161,8 @@ typedef struct hsdl_stats {
   uint64_t prune_label_reads;       /* labels of L(y) streamed by the prune */
   uint64_t prune_tile_reads;        /* labels of L(s) staged into tiles     */
   uint64_t clean_seq_vertices;      /* vertices the clean walked in order   */
+  uint64_t clean_label_reads;       /* labels of the heads' lists L(u) read
+                                       by the clean's first pass            */
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 
@@ -260,6 +262,10 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
  * compared with, and the known-traffic kernel PMC byte counters are
  * calibrated against. */
 int hsdl_device_copy_bench(uint64_t bytes, uint32_t iters, double *gbps);
+/* Returning 32-bit atomicMin on uniformly random words of a `bytes` array:
+ * *gops = atomics per second / 1e9 (the dense dedupe slots' operation; no
+ * counterpart in the reference, used for the expansion roofline). */
+int hsdl_device_atomic_bench(uint64_t bytes, uint64_t ops, double *gops);
 
 const char *hsdl_strerror(int code);
 const char *hsdl_last_error(void);

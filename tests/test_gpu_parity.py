@@ -92,6 +92,23 @@ def test_slot_tags_wrap_with_wide_weights(mode, hub_batch):
     same_labels(idx, want.arrays())
 
 
+def test_config2_full_size_queries_match_bellman_ford():
+    """Config 2 at its full size (R-MAT scale 18, edge factor 16, K=5, the
+    bench workload): the batched build's cleaned index answers sampled
+    (s, t, k) exactly as a hop-bounded Bellman-Ford from s on the CPU --
+    12 sources x 5 hop bounds x 512 targets, unreachable pairs included."""
+    g = H.Graph.rmat(18, 16, 0x5EED0002, 1, 100)
+    c = csr_of(g)
+    idx = H.hop_constrained_two_hop_labels_generation(g, 5, clean=True)
+    rng = np.random.default_rng(0x5EED0299)
+    for s in rng.integers(0, c.n, 12):
+        t = rng.integers(0, c.n, 512).astype(np.uint32)
+        for k in range(1, 6):
+            want = O.bf(c, int(s), k)[t]
+            got = idx.query(np.full(len(t), s, np.uint32), t, np.full(len(t), k, np.int8))
+            assert np.array_equal(got, want), (int(s), k)
+
+
 def test_deadline_reports_timeout(c1):
     g, _ = c1
     with pytest.raises(H.HSDLError) as e:
