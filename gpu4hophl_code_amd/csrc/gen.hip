@@ -314,11 +314,16 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t edges_seen = 0;
   ListRes res;
-  for (;;) {
-    // waves claim 64-item chunks of the frontier
-    uint32_t chunk = 0;
-    if (lane == 0) chunk = atomicAdd(&a.ctrl->expand_ctr, 1u);
-    chunk = __shfl(chunk, 0, kWave);
+  const uint32_t wave_id = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  for (bool first = true;; first = false) {
+    // 64-item chunks of the frontier: a wave's first chunk is its own id,
+    // later ones come from a counter (waves with nothing to do never touch it)
+    uint32_t chunk = wave_id;
+    if (!first) {
+      if (lane == 0) chunk = nwaves + atomicAdd(&a.ctrl->expand_ctr, 1u);
+      chunk = __shfl(chunk, 0, kWave);
+    }
     const uint64_t base = (uint64_t)chunk * kWave;
     if (base >= fcnt) break;
     const uint64_t i = base + lane;
@@ -382,9 +387,11 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t wv = threadIdx.x / kWave;
   const uint32_t cnt = min(a.ctrl->heavy_cnt, a.heavy_cap);
+  if (blockIdx.x >= cnt) return;  // uniform: most launches find no pieces
   ListRes res;
-  for (;;) {
-    if (threadIdx.x == 0) sh_h = atomicAdd(&a.ctrl->heavy_ctr, 1u);
+  for (bool first = true;; first = false) {
+    // a workgroup's first piece is its own id, later ones come from a counter
+    if (threadIdx.x == 0) sh_h = first ? blockIdx.x : gridDim.x + atomicAdd(&a.ctrl->heavy_ctr, 1u);
     __syncthreads();
     const uint32_t h = sh_h;
     __syncthreads();
@@ -634,9 +641,10 @@ k_prune(GenArgs a) {
   unsigned long long wave_reads = 0, tile_reads = 0;
   const uint64_t gwords = a.gtile_cap / gridDim.x;
   uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
-  for (;;) {
+  for (bool first = true;; first = false) {
     if (threadIdx.x == 0) {
-      sh_w = atomicAdd(&a.ctrl->work_ctr, 1u);
+      // a workgroup's first item is its own id, later ones come from a counter
+      sh_w = first ? blockIdx.x : gridDim.x + atomicAdd(&a.ctrl->work_ctr, 1u);
       sh_acc_n = 0;
       sh_next = 0;
       sh_touch_n = 0;
