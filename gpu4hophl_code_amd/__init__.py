@@ -382,7 +382,7 @@ class Server:
     STAGING = {"mapped": 0, "copy": 1}
 
     def __init__(self, index, max_batch=65536, max_wait_us=200, inflight=4,
-                 queue_capacity=1 << 24, hbm_budget=0, staging="mapped"):
+                 queue_capacity=1 << 22, hbm_budget=0, staging="mapped"):
         o = hsdl_server_opts()
         lib().hsdl_server_opts_init(ctypes.byref(o))
         o.max_batch = max_batch

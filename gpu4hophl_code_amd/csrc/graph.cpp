@@ -444,7 +444,7 @@ void hsdl_server_opts_init(hsdl_server_opts *o) {
   o->max_batch = 65536;
   o->max_wait_us = 200;
   o->inflight = 4;
-  o->queue_capacity = 1ull << 24;
+  o->queue_capacity = 1ull << 22;
 }
 
 }  // extern "C"

@@ -17,7 +17,6 @@
 #include <climits>
 #include <condition_variable>
 #include <cstring>
-#include <deque>
 #include <mutex>
 #include <string>
 #include <cstdio>
@@ -72,9 +71,14 @@ struct hsdl_server {
   std::vector<int8_t> q_k;
   std::vector<uint64_t> q_arr;
   uint64_t q_head = 0, q_tail = 0;  // absolute positions; id == position
-  // finished results
+  // finished results: a ring of queue_capacity entries, preallocated (no
+  // allocation on the serving path).  The server thread is the only writer
+  // and publishes r_tail; readers (poll, replay) take rmu and advance r_head.
+  // Submission keeps submitted - polled <= queue_capacity, so the ring
+  // cannot overrun.
   std::mutex rmu;
-  std::deque<hsdl::Result> results;
+  std::vector<hsdl::Result> ring;
+  std::atomic<uint64_t> r_head{0}, r_tail{0};
   std::atomic<uint64_t> completed{0}, batches{0}, batched_items{0};
   std::atomic<bool> stop{false};
   std::thread worker;
@@ -123,7 +127,6 @@ static void worker_loop(hsdl_server *srv) {
   (void)hipSetDevice(srv->device);
   const uint64_t cap = srv->o.queue_capacity;
   const uint64_t wait_ns = (uint64_t)srv->o.max_wait_us * 1000ull;
-  std::vector<Result> done;
   uint64_t last_iter = now_ns();
   while (true) {
     bool progressed = false;
@@ -145,13 +148,11 @@ static void worker_loop(hsdl_server *srv) {
       srv->note(TraceEvent{'B', (uint32_t)(&sl - srv->slots.data()),
                            {sl.count, sl.count ? sl.arrival[0] : 0, sl.t_close, sl.t_h2d,
                             sl.t_kernel, sl.t_d2h, sl.t_launched, t}});
-      done.clear();
+      const uint64_t rt = srv->r_tail.load(std::memory_order_relaxed);
       for (size_t i = 0; i < sl.count; ++i)
-        done.push_back(Result{sl.ids[i], sl.h_out[i], t > sl.arrival[i] ? t - sl.arrival[i] : 0});
-      {
-        std::lock_guard<std::mutex> g(srv->rmu);
-        srv->results.insert(srv->results.end(), done.begin(), done.end());
-      }
+        srv->ring[(rt + i) % cap] =
+            Result{sl.ids[i], sl.h_out[i], t > sl.arrival[i] ? t - sl.arrival[i] : 0};
+      srv->r_tail.store(rt + sl.count, std::memory_order_release);
       srv->completed += sl.count;
       sl.busy = false;
       sl.count = 0;
@@ -248,6 +249,7 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
   srv->q_t.resize(o.queue_capacity);
   srv->q_k.resize(o.queue_capacity);
   srv->q_arr.resize(o.queue_capacity);
+  srv->ring.resize(o.queue_capacity);
   srv->slots.resize(o.inflight);
   for (Slot &sl : srv->slots) {
     hipError_t e = hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking);
@@ -303,7 +305,9 @@ int hsdl_server_submit(hsdl_server *srv, size_t n, const uint32_t *s, const uint
   const uint64_t cap = srv->o.queue_capacity;
   const uint64_t now = now_ns();
   std::lock_guard<std::mutex> g(srv->mu);
-  if (srv->q_tail - srv->q_head + n > cap) return fail(HSDL_EBUSY, "request queue full");
+  // pending, in flight and unpolled requests share the capacity
+  if (srv->q_tail - srv->r_head.load(std::memory_order_acquire) + n > cap)
+    return fail(HSDL_EBUSY, "request queue full");
   if (first_id) *first_id = srv->q_tail;
   for (size_t i = 0; i < n; ++i) {
     const uint64_t p = (srv->q_tail + i) % cap;
@@ -320,14 +324,17 @@ int hsdl_server_poll(hsdl_server *srv, size_t cap, uint64_t *ids, int32_t *answe
                      uint64_t *latency_ns, size_t *n_out) {
   if (!srv || !n_out) return fail(HSDL_EINVAL, "null argument");
   std::lock_guard<std::mutex> g(srv->rmu);
-  size_t n = std::min(cap, srv->results.size());
+  const uint64_t rcap = srv->o.queue_capacity;
+  const uint64_t h = srv->r_head.load(std::memory_order_relaxed);
+  const uint64_t t = srv->r_tail.load(std::memory_order_acquire);
+  const size_t n = (size_t)std::min<uint64_t>(cap, t - h);
   for (size_t i = 0; i < n; ++i) {
-    const Result &r = srv->results[i];
+    const Result &r = srv->ring[(h + i) % rcap];
     if (ids) ids[i] = r.id;
     if (answers) answers[i] = r.ans;
     if (latency_ns) latency_ns[i] = r.lat;
   }
-  srv->results.erase(srv->results.begin(), srv->results.begin() + n);
+  srv->r_head.store(h + n, std::memory_order_release);
   *n_out = n;
   return srv->error ? fail(srv->error, "a batch launch failed") : HSDL_OK;
 }
@@ -398,13 +405,14 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
     base_id = srv->q_tail;
   }
   {
+    // earlier unpolled results are dropped
     std::lock_guard<std::mutex> g(srv->rmu);
-    srv->results.clear();
+    srv->r_head.store(srv->r_tail.load(std::memory_order_acquire), std::memory_order_release);
   }
   const uint64_t start = now_ns();
+  const uint64_t rcap = srv->o.queue_capacity;
   std::vector<uint64_t> arr;
   size_t sent = 0, got = 0;
-  std::vector<Result> batch;
   while (got < n) {
     const uint64_t now = now_ns();
     size_t upto = sent;
@@ -418,21 +426,25 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
       if (rc == HSDL_OK) sent = upto;
       else if (rc != HSDL_EBUSY) return rc;
     }
+    size_t fresh = 0;
     {
       std::lock_guard<std::mutex> g(srv->rmu);
-      batch.assign(srv->results.begin(), srv->results.end());
-      srv->results.clear();
-    }
-    for (const Result &r : batch) {
-      const uint64_t i = r.id - base_id;
-      if (i < n) {
-        if (answers) answers[i] = r.ans;
-        if (latency_ns) latency_ns[i] = r.lat;
-        ++got;
+      const uint64_t h = srv->r_head.load(std::memory_order_relaxed);
+      const uint64_t t = srv->r_tail.load(std::memory_order_acquire);
+      for (uint64_t p = h; p < t; ++p) {
+        const Result &r = srv->ring[p % rcap];
+        const uint64_t i = r.id - base_id;
+        if (i < n) {
+          if (answers) answers[i] = r.ans;
+          if (latency_ns) latency_ns[i] = r.lat;
+          ++got;
+        }
       }
+      fresh = (size_t)(t - h);
+      srv->r_head.store(t, std::memory_order_release);
     }
     if (srv->error) return fail(srv->error, "a batch launch failed");
-    if (batch.empty() && upto == sent) std::this_thread::yield();
+    if (fresh == 0 && upto == sent) std::this_thread::yield();
   }
   return HSDL_OK;
 }

@@ -215,7 +215,7 @@ typedef struct hsdl_server_opts {
   uint32_t max_batch;      /* queries per launch (default 65536)            */
   uint32_t max_wait_us;    /* batching window (default 200)                 */
   uint32_t inflight;       /* concurrent batches on the device (default 4)  */
-  uint64_t queue_capacity; /* pending requests (default 1<<24)              */
+  uint64_t queue_capacity; /* submitted, not yet polled (default 1<<22)     */
   uint64_t hbm_budget;     /* device bytes for batch buffers; 0 = derive    */
   uint32_t staging;        /* HSDL_STAGE_MAPPED (default) or HSDL_STAGE_COPY */
   uint32_t reserved[7];
