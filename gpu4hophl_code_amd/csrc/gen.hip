@@ -659,12 +659,24 @@ template <class V>
 __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
                                const V *vals, uint32_t cap, uint32_t &reads) {
   reads = 0;
-  for (uint32_t x = 1; x < a.h; ++x) {
-    const uint64_t sid = seg_of(a, y, x);
-    const uint32_t len = a.seg_len[sid];
+  const uint32_t ns = a.h - 1;  // round 1: no label can be a witness
+  // the segments' lengths and offsets are loaded together, one lane each
+  const uint32_t lane = (uint32_t)lane_id();
+  uint32_t my_len = 0;
+  unsigned long long my_st = 0;
+  if (lane < ns) {
+    const uint64_t sid = seg_of(a, y, lane + 1);
+    my_len = a.seg_len[sid];
+    my_st = a.seg_start[sid];
+  }
+  for (uint32_t x = 0; x < ns; ++x) {
+    const uint32_t len = __builtin_amdgcn_readlane(my_len, (int)x);
     if (len == 0) continue;
+    const unsigned long long st =
+        ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(my_st >> 32), (int)x) << 32) |
+        __builtin_amdgcn_readlane((uint32_t)my_st, (int)x);
     uint32_t r = 0;
-    const bool found = witness_in<V>(a, a.pool + a.seg_start[sid], len, dy, keys, vals, cap, r);
+    const bool found = witness_in<V>(a, a.pool + st, len, dy, keys, vals, cap, r);
     reads += r;
     if (found) return true;
   }
