@@ -131,6 +131,7 @@ struct GenArgs {
   uint32_t slot_shift;        // tag << slot_shift | dist << 1 | stale
   unsigned long long *cflat;  // (b << 32 | y), ccap entries
   uint32_t *cgroup;           // y grouped by hub, ccap entries
+  uint32_t *witem;            // hub slot of each prune work item, ccap entries
   uint32_t ccap;
   uint32_t fcap;
   uint32_t *hub_ccnt;
@@ -441,14 +442,18 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     __syncthreads();
   }
   uint32_t run = part[t] - sum, runc = partc[t] - csum;
+  const bool dead = a.ctrl->overflow != 0;
   for (uint32_t b = lo; b < hi; ++b) {
+    const uint32_t nch = (a.hub_ccnt[b] + a.tile - 1) / a.tile;
     a.chunk_off[b] = run;
     a.hub_off[b] = runc;
-    run += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
+    // work item -> hub slot (items never outnumber candidates, <= ccap)
+    if (!dead)
+      for (uint32_t j = 0; j < nch; ++j) a.witem[run + j] = b;
+    run += nch;
     runc += a.hub_ccnt[b];
   }
   if (t == blockDim.x - 1) {
-    const bool dead = a.ctrl->overflow != 0;
     a.chunk_off[a.B] = part[t];
     a.hub_off[a.B] = partc[t];
     a.ctrl->total_chunks = dead ? 0u : part[t];
@@ -708,13 +713,7 @@ k_prune(GenArgs a) {
     __syncthreads();
     const uint32_t w = sh_w;
     if (w >= a.ctrl->total_chunks) break;
-    // hub slot b with chunk_off[b] <= w < chunk_off[b+1]
-    uint32_t lo = 0, hi = a.B;  // invariant chunk_off[lo] <= w < chunk_off[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (a.chunk_off[mid] <= w) lo = mid; else hi = mid;
-    }
-    const uint32_t b = lo;
+    const uint32_t b = a.witem[w];  // hub slot: chunk_off[b] <= w < chunk_off[b+1]
     const uint32_t s = a.s0 + b;
     const uint32_t c_beg = (w - a.chunk_off[b]) * a.tile;
     const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
@@ -947,7 +946,8 @@ static int upload_graph(hsdl_graph *g, int dev) {
 struct Gen {
   GenArgs a{};
   DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
-  DevBuf cand_d, cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
+  DevBuf cand_d, cflat, cgroup, witem, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl,
+      heavy;
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
@@ -963,7 +963,8 @@ struct Gen {
   uint64_t n_prune = 0;
   double h_grow = 0, h_enqueue = 0, h_wait = 0;  // host-side phase timers (verbose)
   uint64_t held() const {
-    return fixed + pool.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes + gtile.bytes;
+    return fixed + pool.bytes + cflat.bytes + cgroup.bytes + witem.bytes + F0.bytes + F1.bytes +
+           gtile.bytes;
   }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1019,12 +1020,15 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
   if (ccap != g.a.ccap) {
     g.cflat.release();
     g.cgroup.release();
-    if (g.held() + (uint64_t)ccap * 12 > g.budget)
+    g.witem.release();
+    if (g.held() + (uint64_t)ccap * 16 > g.budget)
       return fail(HSDL_ENOMEM, "candidate lists exceed the HBM budget");
-    if ((rc = g.cflat.alloc((uint64_t)ccap * 8)) || (rc = g.cgroup.alloc((uint64_t)ccap * 4)))
+    if ((rc = g.cflat.alloc((uint64_t)ccap * 8)) || (rc = g.cgroup.alloc((uint64_t)ccap * 4)) ||
+        (rc = g.witem.alloc((uint64_t)ccap * 4)))
       return rc;
     g.a.cflat = g.cflat.as<unsigned long long>();
     g.a.cgroup = g.cgroup.as<uint32_t>();
+    g.a.witem = g.witem.as<uint32_t>();
     g.a.ccap = ccap;
   }
   if (fcap != g.a.fcap) {
@@ -1144,7 +1148,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint32_t heavy_cap = 1u << 20;  // heavy pieces per round (24 MB)
   g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20) +
             (uint64_t)heavy_cap * sizeof(HeavyItem);
-  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 36)
+  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 40)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
   if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(nseg * 8)) ||
@@ -1323,6 +1327,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   g.cand_d.release();
   g.cflat.release();
   g.cgroup.release();
+  g.witem.release();
   g.F0.release();
   g.F1.release();
   g.gtile.release();
