@@ -118,12 +118,10 @@ struct GenArgs {
   const uint2 *adj;
   unsigned long long *pool;
   unsigned long long pool_cap;
-  // label segments: vertex v keeps its hop-x labels (x = 1..K) in segment
-  // v*K + x-1 of the pool; the self label (v,0,0) is implicit until export
   unsigned long long *seg_start;
   uint32_t *seg_len;
   uint32_t *seg_cap;
-  uint32_t *add_cnt;  // per vertex: labels accepted this round (hop h)
+  uint32_t *add_cnt;
   uint32_t *add_cur;
   uint32_t *touched;
   uint32_t *cand_d;           // dense slots [Bmax][n], tagged (slot_fresh / slot_stale)
@@ -131,7 +129,6 @@ struct GenArgs {
   uint32_t slot_shift;        // tag << slot_shift | dist << 1 | stale
   unsigned long long *cflat;  // (b << 32 | y), ccap entries
   uint32_t *cgroup;           // y grouped by hub, ccap entries
-  uint32_t *witem;            // hub slot of each prune work item, ccap entries
   uint32_t ccap;
   uint32_t fcap;
   uint32_t *hub_ccnt;
@@ -148,11 +145,6 @@ struct GenArgs {
 };
 
 __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
-
-// segment of vertex v's hop-x labels (1 <= x <= K)
-__device__ inline uint64_t seg_of(const GenArgs &a, uint32_t v, uint32_t x) {
-  return (uint64_t)v * a.K + (x - 1);
-}
 
 // Dense dedupe slots keep, per (hub slot, vertex), the smallest candidate
 // distance of every round of the current batch attempt:
@@ -184,14 +176,12 @@ __device__ inline bool slot_opens(const GenArgs &a, uint32_t old) {
 __global__ void k_init_labels(GenArgs a, uint32_t c0) {
   for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < a.n;
        v += (uint64_t)gridDim.x * blockDim.x) {
-    for (uint32_t x = 1; x <= a.K; ++x) {
-      const uint64_t sid = seg_of(a, (uint32_t)v, x);
-      a.seg_start[sid] = sid * c0;
-      a.seg_len[sid] = 0;
-      a.seg_cap[sid] = c0;
-    }
+    a.seg_start[v] = v * c0;
+    a.seg_len[v] = 1;
+    a.seg_cap[v] = c0;
     a.add_cnt[v] = 0;
     a.add_cur[v] = 0;
+    a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
   }
 }
 
@@ -442,18 +432,14 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     __syncthreads();
   }
   uint32_t run = part[t] - sum, runc = partc[t] - csum;
-  const bool dead = a.ctrl->overflow != 0;
   for (uint32_t b = lo; b < hi; ++b) {
-    const uint32_t nch = (a.hub_ccnt[b] + a.tile - 1) / a.tile;
     a.chunk_off[b] = run;
     a.hub_off[b] = runc;
-    // work item -> hub slot (items never outnumber candidates, <= ccap)
-    if (!dead)
-      for (uint32_t j = 0; j < nch; ++j) a.witem[run + j] = b;
-    run += nch;
+    run += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
     runc += a.hub_ccnt[b];
   }
   if (t == blockDim.x - 1) {
+    const bool dead = a.ctrl->overflow != 0;
     a.chunk_off[a.B] = part[t];
     a.hub_off[a.B] = partc[t];
     a.ctrl->total_chunks = dead ? 0u : part[t];
@@ -559,37 +545,20 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals
   for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) keys[i] = kInf32;
   for (uint32_t i = threadIdx.x; i < cap * K1; i += blockDim.x) vals[i] = (V)TileVal<V>::kEmpty;
   __syncthreads();
-  // the labels of L(s) a round-h witness can use: the self label and hops
-  // 1..h-1 (a hop-h label of s pairs only with a hop-0 label of L(y), i.e.
-  // hub y > s, which L(s) cannot hold)
-  for (uint32_t x = 0; x < a.h; ++x) {
-    unsigned long long st = 0;
-    uint32_t len = 1;
-    if (x > 0) {
-      const uint64_t sid = seg_of(a, s, x);
-      st = a.seg_start[sid];
-      len = a.seg_len[sid];
+  const unsigned long long st = a.seg_start[s];
+  const uint32_t len = a.seg_len[s];
+  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
+    const unsigned long long l = a.pool[st + i];
+    const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
+    uint32_t slot = hash_slot(hub, cap);
+    for (;;) {
+      const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
+      if (prev == kInf32 || prev == hub) break;
+      slot = (slot + 1 == cap) ? 0 : slot + 1;
     }
-    for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-      const unsigned long long l = x ? a.pool[st + i] : pack_label(s, 0, 0);
-      const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
-      uint32_t slot = hash_slot(hub, cap);
-      for (;;) {
-        const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
-        if (prev == kInf32 || prev == hub) break;
-        slot = (slot + 1 == cap) ? 0 : slot + 1;
-      }
-      for (uint32_t y = hb; y < K1; ++y) atomic_min_v(&vals[slot * K1 + y], db);
-    }
+    for (uint32_t x = hb; x < K1; ++x) atomic_min_v(&vals[slot * K1 + x], db);
   }
   __syncthreads();
-}
-
-// labels of L(s) a round-h tile holds (self label + hops 1..h-1)
-__device__ inline uint32_t tile_len(const GenArgs &a, uint32_t s) {
-  uint32_t len = 1;
-  for (uint32_t x = 1; x < a.h; ++x) len += a.seg_len[seg_of(a, s, x)];
-  return len;
 }
 
 
@@ -609,10 +578,12 @@ __device__ inline bool probe_label(const GenArgs &a, unsigned long long l, bool 
 }
 
 template <class V>
-__device__ bool witness_in(const GenArgs &a, const unsigned long long *L, uint32_t len, uint32_t dy,
-                           const uint32_t *keys, const V *vals, uint32_t cap, uint32_t &reads) {
+__device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
+                               const V *vals, uint32_t cap, uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
+  const unsigned long long *L = a.pool + a.seg_start[y];
+  const uint32_t len = a.seg_len[y];
   constexpr uint32_t kStep = 2 * kWave;
   // witnesses are mostly top-ranked hubs near the front of L(y): the first
   // two steps take 64 labels each (one step in flight), later steps 128
@@ -656,38 +627,6 @@ __device__ bool witness_in(const GenArgs &a, const unsigned long long *L, uint32
   return false;
 }
 
-// Round h: only labels of L(y) with hop 1..h-1 can be witnesses (a hop-h
-// label of y from hub a pairs only with hop 0 in L(s), i.e. a = s, whose hop-h
-// labels do not exist until this round appends them; the self label of y
-// pairs only with hub y > s).  The segments are streamed hop by hop.
-template <class V>
-__device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
-                               const V *vals, uint32_t cap, uint32_t &reads) {
-  reads = 0;
-  const uint32_t ns = a.h - 1;  // round 1: no label can be a witness
-  // the segments' lengths and offsets are loaded together, one lane each
-  const uint32_t lane = (uint32_t)lane_id();
-  uint32_t my_len = 0;
-  unsigned long long my_st = 0;
-  if (lane < ns) {
-    const uint64_t sid = seg_of(a, y, lane + 1);
-    my_len = a.seg_len[sid];
-    my_st = a.seg_start[sid];
-  }
-  for (uint32_t x = 0; x < ns; ++x) {
-    const uint32_t len = __builtin_amdgcn_readlane(my_len, (int)x);
-    if (len == 0) continue;
-    const unsigned long long st =
-        ((unsigned long long)__builtin_amdgcn_readlane((uint32_t)(my_st >> 32), (int)x) << 32) |
-        __builtin_amdgcn_readlane((uint32_t)my_st, (int)x);
-    uint32_t r = 0;
-    const bool found = witness_in<V>(a, a.pool + st, len, dy, keys, vals, cap, r);
-    reads += r;
-    if (found) return true;
-  }
-  return false;
-}
-
 template <class V>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune(GenArgs a) {
@@ -713,13 +652,18 @@ k_prune(GenArgs a) {
     __syncthreads();
     const uint32_t w = sh_w;
     if (w >= a.ctrl->total_chunks) break;
-    const uint32_t b = a.witem[w];  // hub slot: chunk_off[b] <= w < chunk_off[b+1]
+    // hub slot b with chunk_off[b] <= w < chunk_off[b+1]
+    uint32_t lo = 0, hi = a.B;  // invariant chunk_off[lo] <= w < chunk_off[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.chunk_off[mid] <= w) lo = mid; else hi = mid;
+    }
+    const uint32_t b = lo;
     const uint32_t s = a.s0 + b;
     const uint32_t c_beg = (w - a.chunk_off[b]) * a.tile;
     const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
     const uint32_t *cands = a.cgroup + a.hub_off[b];
-    const uint32_t tlen = tile_len(a, s);
-    const uint32_t cap = tile_cap_for(tlen);
+    const uint32_t cap = tile_cap_for(a.seg_len[s]);
     uint32_t *keys;
     V *vals;
     const bool in_lds = cap <= a.lds_slots;
@@ -729,7 +673,7 @@ k_prune(GenArgs a) {
       if (tile_b != b) {
         build_tile<V>(a, s, keys, vals, cap);
         tile_b = b;
-        tile_reads += tlen;
+        tile_reads += a.seg_len[s];
       }
     } else {
       if (tile_words<V>(cap, K1) > gwords) {
@@ -746,7 +690,7 @@ k_prune(GenArgs a) {
       // this workgroup's arena slice before plain loads read it
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tile_b = kInf32;
-      tile_reads += tlen;
+      tile_reads += a.seg_len[s];
     }
     // waves take the item's candidates one at a time (L(y) lengths vary widely)
     for (;;) {
@@ -804,8 +748,7 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
   const uint32_t cnt = a.ctrl->touched_cnt;
   for (uint32_t t = wave; t < cnt; t += nwaves) {
     const uint32_t y = a.touched[t];
-    const uint64_t sid = seg_of(a, y, a.h);
-    const uint32_t len = a.seg_len[sid], cap = a.seg_cap[sid];
+    const uint32_t len = a.seg_len[y], cap = a.seg_cap[y];
     const uint32_t need = len + a.add_cnt[y];
     if (need <= cap) continue;
     uint32_t ncap = max(need, cap * 2u);
@@ -817,11 +760,11 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
       if (lane == 0) atomicOr(&a.ctrl->overflow, kOvfPool);
       continue;
     }
-    const unsigned long long os = a.seg_start[sid];
+    const unsigned long long os = a.seg_start[y];
     for (uint32_t i = lane; i < len; i += kWave) a.pool[ns + i] = a.pool[os + i];
     if (lane == 0) {
-      a.seg_start[sid] = ns;
-      a.seg_cap[sid] = ncap;
+      a.seg_start[y] = ns;
+      a.seg_cap[y] = ncap;
     }
   }
 }
@@ -831,10 +774,9 @@ __global__ void __launch_bounds__(256) k_append(GenArgs a) {
   const uint32_t cnt = min(a.ctrl->fcnt[a.cur ^ 1], a.fcap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
     const Item it = Fn[i];
-    const uint64_t sid = seg_of(a, it.y, a.h);
-    const uint32_t p = a.seg_len[sid] + atomicAdd(&a.add_cur[it.y], 1u);
-    if (p < a.seg_cap[sid])
-      a.pool[a.seg_start[sid] + p] = pack_label(a.s0 + it.b, a.h, it.d);
+    const uint32_t p = a.seg_len[it.y] + atomicAdd(&a.add_cur[it.y], 1u);
+    if (p < a.seg_cap[it.y])
+      a.pool[a.seg_start[it.y] + p] = pack_label(a.s0 + it.b, a.h, it.d);
     else
       atomicOr(&a.ctrl->overflow, kOvfPool);
   }
@@ -845,45 +787,37 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
   // a lost frontier item leaves a counted label unwritten: publish nothing
   // (the host rolls the batch back)
   const bool keep = !(a.ctrl->overflow & kOvfFront);
+  uint32_t mx = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
     const uint32_t y = a.touched[t];
     if (keep) {
       // capped: when the pool ran out some appends were dropped
-      const uint64_t sid = seg_of(a, y, a.h);
-      a.seg_len[sid] = min(a.seg_len[sid] + a.add_cnt[y], a.seg_cap[sid]);
+      const uint32_t nl = min(a.seg_len[y] + a.add_cnt[y], a.seg_cap[y]);
+      a.seg_len[y] = nl;
+      mx = max(mx, nl);
     }
     a.add_cnt[y] = 0;
     a.add_cur[y] = 0;
   }
+  if (mx) atomicMax(&a.ctrl->maxlen, mx);
 }
 
 // Removes the labels of hubs [s0, s0+B) (the batch being rolled back): they
-// sit at the tail of every segment, after all labels of earlier batches.  The
-// redo runs under a new slot tag, so the batch's dense slots need no refill.
+// sit at the tail of every segment, after all labels of earlier batches; the
+// self label at position 0 stays.  The redo runs under a new slot tag, so the
+// batch's dense slots need no refill.
 __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
-    for (uint32_t x = 1; x <= a.K; ++x) {
-      const uint64_t sid = seg_of(a, v, x);
-      uint32_t len = a.seg_len[sid];
-      const unsigned long long st = a.seg_start[sid];
-      while (len > 0) {
-        const uint32_t hub = label_hub(a.pool[st + len - 1]);
-        if (hub < a.s0 || hub - a.s0 >= a.B) break;
-        --len;
-      }
-      a.seg_len[sid] = len;
+    uint32_t len = a.seg_len[v];
+    const unsigned long long st = a.seg_start[v];
+    while (len > 1) {
+      const uint32_t hub = label_hub(a.pool[st + len - 1]);
+      if (hub < a.s0 || hub - a.s0 >= a.B) break;
+      --len;
     }
+    a.seg_len[v] = len;
     a.add_cnt[v] = 0;
     a.add_cur[v] = 0;
-  }
-}
-
-// Labels per vertex at export: the self label plus every hop segment.
-__global__ void __launch_bounds__(256) k_vertex_totals(GenArgs a, uint32_t *tot) {
-  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
-    uint32_t t = 1;
-    for (uint32_t x = 1; x <= a.K; ++x) t += a.seg_len[seg_of(a, v, x)];
-    tot[v] = t;
   }
 }
 
@@ -894,16 +828,9 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
   for (uint32_t v = wave; v < a.n; v += nwaves) {
-    unsigned long long dst = out_off[v];
-    if (lane == 0) out[dst] = pack_label(v, 0, 0);
-    ++dst;
-    for (uint32_t x = 1; x <= a.K; ++x) {
-      const uint64_t sid = seg_of(a, v, x);
-      const unsigned long long src = a.seg_start[sid];
-      const uint32_t len = a.seg_len[sid];
-      for (uint32_t i = lane; i < len; i += kWave) out[dst + i] = a.pool[src + i];
-      dst += len;
-    }
+    const unsigned long long src = a.seg_start[v], dst = out_off[v];
+    const uint32_t len = a.seg_len[v];
+    for (uint32_t i = lane; i < len; i += kWave) out[dst + i] = a.pool[src + i];
   }
 }
 
@@ -946,8 +873,7 @@ static int upload_graph(hsdl_graph *g, int dev) {
 struct Gen {
   GenArgs a{};
   DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
-  DevBuf cand_d, cflat, cgroup, witem, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl,
-      heavy;
+  DevBuf cand_d, cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
@@ -963,8 +889,7 @@ struct Gen {
   uint64_t n_prune = 0;
   double h_grow = 0, h_enqueue = 0, h_wait = 0;  // host-side phase timers (verbose)
   uint64_t held() const {
-    return fixed + pool.bytes + cflat.bytes + cgroup.bytes + witem.bytes + F0.bytes + F1.bytes +
-           gtile.bytes;
+    return fixed + pool.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes + gtile.bytes;
   }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1020,15 +945,12 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
   if (ccap != g.a.ccap) {
     g.cflat.release();
     g.cgroup.release();
-    g.witem.release();
-    if (g.held() + (uint64_t)ccap * 16 > g.budget)
+    if (g.held() + (uint64_t)ccap * 12 > g.budget)
       return fail(HSDL_ENOMEM, "candidate lists exceed the HBM budget");
-    if ((rc = g.cflat.alloc((uint64_t)ccap * 8)) || (rc = g.cgroup.alloc((uint64_t)ccap * 4)) ||
-        (rc = g.witem.alloc((uint64_t)ccap * 4)))
+    if ((rc = g.cflat.alloc((uint64_t)ccap * 8)) || (rc = g.cgroup.alloc((uint64_t)ccap * 4)))
       return rc;
     g.a.cflat = g.cflat.as<unsigned long long>();
     g.a.cgroup = g.cgroup.as<uint32_t>();
-    g.a.witem = g.witem.as<uint32_t>();
     g.a.ccap = ccap;
   }
   if (fcap != g.a.fcap) {
@@ -1128,16 +1050,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   size_t free_b = 0, total_b = 0;
   HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
   g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
-  const uint32_t c0 = 4;  // initial label slots per (vertex, hop) segment
-  const uint64_t nseg = (uint64_t)n * K;
+  const uint32_t c0 = 16;  // initial label slots per vertex
   const uint64_t pool0 = opts->pool_init_labels
-                             ? std::max<uint64_t>(opts->pool_init_labels, nseg * c0)
-                             : std::max<uint64_t>(nseg * c0 * 4, 64ull << 20);
+                             ? std::max<uint64_t>(opts->pool_init_labels, (uint64_t)n * c0)
+                             : std::max<uint64_t>((uint64_t)n * c0 * 4, 64ull << 20);
   const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
                                                  : (uint64_t)g.grid * 16384;
   const uint32_t list0 =
       (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30);
-  const uint64_t vert_bytes = nseg * 16 + (uint64_t)n * 4 * 3;  // segments + per-vertex
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5);
   // dense dedupe slots: at most a quarter of the budget, <= 65536 hubs
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
   uint32_t Bmax =
@@ -1148,11 +1069,11 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint32_t heavy_cap = 1u << 20;  // heavy pieces per round (24 MB)
   g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20) +
             (uint64_t)heavy_cap * sizeof(HeavyItem);
-  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 40)
+  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 36)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
-  if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(nseg * 8)) ||
-      (rc = g.seg_len.alloc(nseg * 4)) || (rc = g.seg_cap.alloc(nseg * 4)) ||
+  if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
+      (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
       (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
       (rc = g.touched.alloc(n * 4ull)) || (rc = g.cand_d.alloc((uint64_t)Bmax * n * 4)) ||
       (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
@@ -1210,7 +1131,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
 
   {
     Ctrl c0v{};
-    c0v.pool_top = (unsigned long long)nseg * c0;
+    c0v.pool_top = (unsigned long long)n * c0;
     HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &c0v, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
   }
   hipLaunchKernelGGL(k_init_labels, dim3(1024), dim3(256), 0, g.st, a, c0);
@@ -1327,7 +1248,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   g.cand_d.release();
   g.cflat.release();
   g.cgroup.release();
-  g.witem.release();
   g.F0.release();
   g.F1.release();
   g.gtile.release();
@@ -1342,9 +1262,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     return code;
   };
   if ((rc = idx->off.alloc((n + 1ull) * 8))) return cleanup(rc);
-  // per-vertex totals (self label + hop segments) into add_cnt, now idle
-  hipLaunchKernelGGL(k_vertex_totals, dim3(g.grid), dim3(256), 0, g.st, a, a.add_cnt);
-  if ((rc = scan_counts_u32(a.add_cnt, idx->off.as<uint64_t>(), n, g.st))) return cleanup(rc);
+  if ((rc = scan_counts_u32(a.seg_len, idx->off.as<uint64_t>(), n, g.st))) return cleanup(rc);
   uint64_t total = 0;
   if (hipMemcpy(&total, idx->off.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost) != hipSuccess)
     return cleanup(fail(HSDL_EDEVICE, "reading label total failed"));
