@@ -970,6 +970,9 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
 
 // Enqueues one batch (K rounds) of hubs [s0, s0+B).
 static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
+  // k_batch_begin writes the B hubs into the first frontier unchecked
+  if (B == 0 || B > g.a.fcap || s0 + (uint64_t)B > g.a.n)
+    return fail(HSDL_EINVAL, "hub batch does not fit the frontier list or the graph");
   GenArgs a = g.a;
   a.s0 = s0;
   a.B = B;
@@ -1056,8 +1059,12 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
                              : std::max<uint64_t>((uint64_t)n * c0 * 4, 64ull << 20);
   const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
                                                  : (uint64_t)g.grid * 16384;
-  const uint32_t list0 =
-      (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30);
+  // first candidate / frontier list size; HSDL_LIST_INIT starts them small
+  // (the lists grow by rolling the batch back, like the pool)
+  uint32_t list0 = env_u32(
+      "HSDL_LIST_INIT",
+      (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30), 64,
+      1u << 30);
   const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5);
   // dense dedupe slots: at most a quarter of the budget, <= 65536 hubs
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
@@ -1066,7 +1073,10 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   if (plain) Bmax = 1;
   if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
   if (Bmax == 0) return fail(HSDL_ENOMEM, "HBM budget below one hub's dedupe slots");
-  const uint32_t heavy_cap = 1u << 20;  // heavy pieces per round (24 MB)
+  list0 = std::max(list0, Bmax);  // a batch's first frontier (its hubs) must fit
+  // heavy pieces per round (24 MB); when a round has more, the waves keep
+  // the items that did not fit (HSDL_HEAVY_CAP lowers the cap)
+  const uint32_t heavy_cap = env_u32("HSDL_HEAVY_CAP", 1u << 20, 1, 1u << 20);
   g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20) +
             (uint64_t)heavy_cap * sizeof(HeavyItem);
   if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 36)

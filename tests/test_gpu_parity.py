@@ -67,6 +67,31 @@ def test_c1_global_arena_tiles(c1):
     same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
 
 
+@pytest.fixture(scope="module")
+def heavy13():
+    """R-MAT scale 13, EF 16: 14 vertices with more than 1024 edges (up to
+    2171, three heavy pieces) and the oracle's K=4 label set."""
+    g = H.Graph.rmat(13, 16, 77, 1, 100)
+    return g, O.kround(csr_of(g), 4).arrays()
+
+
+@pytest.mark.parametrize("list_init,heavy_cap,hub_batch",
+                         [("64", None, 0), ("4096", "3", 512), (None, "1", 0), ("1000", "2", 97)])
+def test_list_and_heavy_overflow_paths(monkeypatch, heavy13, list_init, heavy_cap, hub_batch):
+    """Candidate and frontier lists that start far too small force their
+    rollback-regrow-redo path (several times per batch); a heavy-piece cap of
+    1-3 leaves most high-degree items to the waves that found them, and an
+    item whose pieces straddle the cap writes empty pieces below it.  The
+    labels must not change."""
+    if list_init:
+        monkeypatch.setenv("HSDL_LIST_INIT", list_init)
+    if heavy_cap:
+        monkeypatch.setenv("HSDL_HEAVY_CAP", heavy_cap)
+    g, want = heavy13
+    idx = H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=hub_batch)
+    same_labels(idx, want)
+
+
 def test_wide_weights_use_32bit_tiles():
     """K * max weight >= 65535 selects 32-bit tile values."""
     g = H.Graph.rmat(10, 8, 4242, 1, 40000)
