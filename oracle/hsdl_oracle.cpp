@@ -48,6 +48,7 @@ constexpr int32_t INF32 = INT32_MAX;
 
 struct Label {
   int32_t hub, hop, dist;
+  int32_t parent;  // K-round generation only (0 elsewhere): see kround()
 };
 
 struct Graph {
@@ -173,22 +174,25 @@ Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) 
   L.assign(n, {});
   std::vector<std::vector<std::pair<int32_t, int32_t>>> F(n), Fn(n);  // (x, dx)
   for (int32_t v = 0; v < n; ++v) {
-    L[v].push_back({v, 0, 0});
+    L[v].push_back({v, 0, 0, v});
     F[v].push_back({v, 0});
   }
   threads = nthreads_or_default(threads);
   struct Scratch {
-    std::vector<int32_t> tempD;
+    std::vector<int32_t> tempD, tempP;
     HubTable T;
     std::vector<int32_t> D;
     int64_t c_items = 0, c_edges = 0, c_cand = 0, c_cand_reads = 0,
             c_hub_reads = 0, c_acc = 0;
-    explicit Scratch(int32_t n) : tempD(n, INF32), T(n) {}
+    explicit Scratch(int32_t n) : tempD(n, INF32), tempP(n, INF32), T(n) {}
   };
   std::vector<Scratch *> scr;
   for (int t = 0; t < threads; ++t) scr.push_back(new Scratch(n));
-  // per-hub accepted labels of the current round: (y, dist)
-  std::vector<std::vector<std::pair<int32_t, int32_t>>> acc(n);
+  // per-hub accepted labels of the current round: (y, dist, parent)
+  struct Acc {
+    int32_t y, d, p;
+  };
+  std::vector<std::vector<Acc>> acc(n);
 
   for (int h = 1; h <= K; ++h) {
     std::atomic<bool> any{false};
@@ -214,6 +218,13 @@ Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) 
           if (nd < S.tempD[y] && (mutation != 2 || S.tempD[y] == INF32)) {
             if (S.tempD[y] == INF32) S.D.push_back(y);
             S.tempD[y] = nd;
+            S.tempP[y] = x;
+          } else if (nd == S.tempD[y]) {
+            // parent: the smallest frontier vertex at the minimum distance
+            // (the reference keeps the first one its scan meets,
+            // Hybrid_Generation_Clean/src/label_gen.cu:531-541, which depends
+            // on the scan order); mutation 7 keeps the largest instead
+            if (mutation == 7 ? x > S.tempP[y] : x < S.tempP[y]) S.tempP[y] = x;
           }
         }
       }
@@ -225,8 +236,9 @@ Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) 
         S.c_cand_reads += (int64_t)L[y].size();
         int64_t q = pair_min(L[y], S.T, h, mutation);
         bool accept = (mutation == 3) ? (dy <= q) : (dy < q);
-        if (accept) acc[s].push_back({y, dy});
+        if (accept) acc[s].push_back({y, dy, S.tempP[y]});
         S.tempD[y] = INF32;
+        S.tempP[y] = INF32;
       }
       S.D.clear();
       S.T.clear();
@@ -236,8 +248,8 @@ Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) 
     for (int32_t s = 0; s < n; ++s) {
       Fn[s].clear();
       for (const auto &a : acc[s]) {
-        L[a.first].push_back({s, h, a.second});
-        Fn[s].push_back(a);
+        L[a.y].push_back({s, h, a.d, a.p});
+        Fn[s].push_back({a.y, a.d});
       }
       acc_round += (int64_t)acc[s].size();
     }
@@ -337,7 +349,7 @@ Labels *dijkstra(const Graph &g, int K, int mutation) {
   std::vector<int64_t> changed;
   const int K1 = K + 1;
   for (int32_t vk = 0; vk < n; ++vk) {
-    L[vk].push_back({vk, 0, 0});  // reference :74-85
+    L[vk].push_back({vk, 0, 0, 0});  // reference :74-85
     for (const Label &x : L[vk]) T.add(x.hub, x.dist, x.hop);
     int64_t st0 = (int64_t)vk * K1 + 0;
     Q.push(st0, 0);
@@ -351,7 +363,7 @@ Labels *dijkstra(const Graph &g, int K, int mutation) {
       int64_t q = pair_min(L[u], T, uh, mutation);
       bool add = (mutation == 3) ? (P <= q) : (P < q);
       if (add || q == 0) {
-        if (add) L[u].push_back({vk, uh, P});
+        if (add) L[u].push_back({vk, uh, P, 0});
         if (uh + 1 > K) continue;
         for (int64_t e = g.off[u]; e < g.off[u + 1]; ++e) {
           const int32_t y = g.col[e];
@@ -472,6 +484,68 @@ int32_t query_one(const LabelSet &L, int32_t s, int32_t t, int32_t k, int mutati
 }
 
 // ---------------------------------------------------------------------------
+// Hop-constrained shortest path, hop_constrained_extract_shortest_path:
+// CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319.
+// One step merges L(s) and L(t) like the distance query and takes the first
+// pair, in the merge's order (hub, then position in L(s), then position in
+// L(t)), whose dist sum is smaller than every earlier one among pairs with
+// hop sum <= k.  It emits (s, parent of the L(s) label) unless that parent is
+// s, then (t, parent of the L(t) label) unless that parent is t, moves s and
+// t to those parents and lowers k by one per emitted edge; the reference
+// recurses, this loops.  It stops when s == t or no pair qualifies (a step
+// that moves neither endpoint also stops it: the reference would recurse
+// forever, which its own parents never cause).  Mutation 6 takes the last
+// pair at the minimum instead of the first.
+// ---------------------------------------------------------------------------
+void path_one(const LabelSet &L, int32_t s, int32_t t, int32_t k,
+              std::vector<std::pair<int32_t, int32_t>> &out, int mutation) {
+  for (;;) {
+    if (s == t) return;
+    int64_t best = INF32;
+    int32_t p1 = s, p2 = t;
+    const auto &A = L[s];
+    const auto &B = L[t];
+    size_t i = 0, j = 0;
+    while (i < A.size() && j < B.size()) {
+      if (A[i].hub == B[j].hub) {
+        size_t ie = i, je = j;
+        while (ie < A.size() && A[ie].hub == A[i].hub) ++ie;
+        while (je < B.size() && B[je].hub == B[j].hub) ++je;
+        for (size_t a = i; a < ie; ++a) {
+          for (size_t b = j; b < je; ++b) {
+            if (A[a].hop + B[b].hop > k) break;
+            int64_t d = (int64_t)A[a].dist + B[b].dist;
+            if (d < best || (mutation == 6 && d == best)) {
+              best = d;
+              p1 = A[a].parent;
+              p2 = B[b].parent;
+            }
+          }
+        }
+        i = ie;
+        j = je;
+      } else if (A[i].hub > B[j].hub) {
+        ++j;
+      } else {
+        ++i;
+      }
+    }
+    if (best >= INF32) return;
+    if (p1 == s && p2 == t) return;
+    if (s != p1) {
+      out.push_back({s, p1});
+      s = p1;
+      --k;
+    }
+    if (t != p2) {
+      out.push_back({t, p2});
+      t = p2;
+      --k;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Hop-bounded shortest distances from one source:
 // CPU_HopHL/include/graph_v_of_v/graph_v_of_v_hop_constrained_shortest_distance.h:7-38
 // restated as K rounds of Bellman-Ford over a per-round copy (distance[v] is
@@ -517,7 +591,7 @@ LabelSet canonical_bruteforce(const Graph &g, int K) {
     for (int32_t u = 0; u <= v; ++u) {
       for (int h = 0; h <= K; ++h) {
         if (h == 0) {
-          if (u == v) L[v].push_back({v, 0, 0});
+          if (u == v) L[v].push_back({v, 0, 0, 0});
           continue;
         }
         if (u == v) continue;
@@ -534,7 +608,7 @@ LabelSet canonical_bruteforce(const Graph &g, int K) {
             if ((int64_t)a + b <= dh) dominated = true;
           }
         }
-        if (!dominated) L[v].push_back({u, h, dh});
+        if (!dominated) L[v].push_back({u, h, dh, 0});
       }
     }
   }
@@ -682,15 +756,22 @@ orc_labels *orc_clean(const orc_labels *in, int lu_mode, int threads, int mutati
   return wrap(clean(in->L, lu_mode, threads, mutation));
 }
 
-// Build a label set from flat CSR arrays (any per-vertex order; sorted here).
-orc_labels *orc_labels_from_arrays(int32_t n, const int64_t *off, const uint32_t *hub,
-                                   const uint8_t *hop, const int32_t *dist) {
+// Build a label set from flat CSR arrays (any per-vertex order; sorted here);
+// `parent` may be null (parents 0).
+orc_labels *orc_labels_from_arrays_p(int32_t n, const int64_t *off, const uint32_t *hub,
+                                     const uint8_t *hop, const int32_t *dist,
+                                     const int32_t *parent) {
   LabelSet L(n);
   for (int32_t v = 0; v < n; ++v)
     for (int64_t i = off[v]; i < off[v + 1]; ++i)
-      L[v].push_back({(int32_t)hub[i], (int32_t)hop[i], dist[i]});
+      L[v].push_back({(int32_t)hub[i], (int32_t)hop[i], dist[i], parent ? parent[i] : 0});
   sort_all(L);
   return wrap(std::move(L));
+}
+
+orc_labels *orc_labels_from_arrays(int32_t n, const int64_t *off, const uint32_t *hub,
+                                   const uint8_t *hop, const int32_t *dist) {
+  return orc_labels_from_arrays_p(n, off, hub, hop, dist, nullptr);
 }
 
 int64_t orc_labels_total(const orc_labels *l) {
@@ -728,7 +809,31 @@ void orc_labels_export(const orc_labels *l, int64_t *off, uint32_t *hub, uint8_t
   }
 }
 
+// parent vertex of every label, in the export's order
+void orc_labels_parents(const orc_labels *l, int32_t *par) {
+  int64_t p = 0;
+  for (const auto &v : l->L)
+    for (const Label &x : v) par[p++] = x.parent;
+}
+
 void orc_labels_free(orc_labels *l) { delete l; }
+
+// Paths for nq (s, t, k): counts[i] edges of query i (at most cap are
+// stored, the count is the full length) at edges[2 * cap * i ...] as
+// (vertex, parent) pairs in the order the reference emits them.
+void orc_paths(const orc_labels *l, int64_t nq, const uint32_t *s, const uint32_t *t,
+               const int32_t *k, int32_t cap, int32_t *edges, int32_t *counts, int threads,
+               int mutation) {
+  parallel_for(nq, threads, [&](int, int64_t i) {
+    std::vector<std::pair<int32_t, int32_t>> out;
+    path_one(l->L, (int32_t)s[i], (int32_t)t[i], k[i], out, mutation);
+    counts[i] = (int32_t)out.size();
+    for (int32_t e = 0; e < (int32_t)out.size() && e < cap; ++e) {
+      edges[2 * (cap * i + e)] = out[e].first;
+      edges[2 * (cap * i + e) + 1] = out[e].second;
+    }
+  });
+}
 
 void orc_query(const orc_labels *l, int64_t nq, const uint32_t *s, const uint32_t *t,
                const int32_t *k, int32_t *out, int threads, int mutation) {

@@ -49,6 +49,13 @@ def lib():
     L.orc_labels_counters.argtypes = [vp, _i64p]
     L.orc_labels_export.restype = None
     L.orc_labels_export.argtypes = [vp, _i64p, _u32p, _u8p, _i32p]
+    L.orc_labels_from_arrays_p.restype = vp
+    L.orc_labels_from_arrays_p.argtypes = [ctypes.c_int32, _i64p, _u32p, _u8p, _i32p, _i32p]
+    L.orc_labels_parents.restype = None
+    L.orc_labels_parents.argtypes = [vp, _i32p]
+    L.orc_paths.restype = None
+    L.orc_paths.argtypes = [vp, ctypes.c_int64, _u32p, _u32p, _i32p, ctypes.c_int32, _i32p,
+                            _i32p, ctypes.c_int, ctypes.c_int]
     L.orc_labels_free.restype = None
     L.orc_labels_free.argtypes = [vp]
     L.orc_query.restype = None
@@ -129,6 +136,12 @@ class OracleLabels:
         lib().orc_labels_export(self.h, off, hub, hop, dist)
         return off, hub, hop, dist
 
+    def parents(self):
+        """Parent vertex of every label, in the order of arrays()."""
+        par = np.zeros(self.total(), dtype=np.int32)
+        lib().orc_labels_parents(self.h, par)
+        return par
+
     def as_lists(self):
         off, hub, hop, dist = self.arrays()
         return [list(zip(hub[off[v]:off[v + 1]].tolist(), hop[off[v]:off[v + 1]].tolist(),
@@ -160,6 +173,25 @@ def labels_from_arrays(n, off, hub, hop, dist):
     return OracleLabels(lib().orc_labels_from_arrays(
         n, np.ascontiguousarray(off, np.int64), np.ascontiguousarray(hub, np.uint32),
         np.ascontiguousarray(hop, np.uint8), np.ascontiguousarray(dist, np.int32)))
+
+
+def labels_from_arrays_p(n, off, hub, hop, dist, parent):
+    return OracleLabels(lib().orc_labels_from_arrays_p(
+        n, np.ascontiguousarray(off, np.int64), np.ascontiguousarray(hub, np.uint32),
+        np.ascontiguousarray(hop, np.uint8), np.ascontiguousarray(dist, np.int32),
+        np.ascontiguousarray(parent, np.int32)))
+
+
+def paths(labels, s, t, k, cap, threads=0, mutation=0):
+    """Hop-constrained shortest paths: (edges int32[nq, cap, 2], counts int32[nq]);
+    edges are (vertex, parent) pairs in the order the reference emits them."""
+    s = np.ascontiguousarray(s, np.uint32)
+    t = np.ascontiguousarray(t, np.uint32)
+    k = np.ascontiguousarray(k, np.int32)
+    edges = np.zeros((len(s), cap, 2), dtype=np.int32)
+    counts = np.zeros(len(s), dtype=np.int32)
+    lib().orc_paths(labels.h, len(s), s, t, k, cap, edges.reshape(-1), counts, threads, mutation)
+    return edges, counts
 
 
 def query(labels, s, t, k, threads=0, mutation=0):

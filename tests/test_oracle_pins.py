@@ -220,3 +220,162 @@ def test_planted_mistakes_are_caught(mutation):
                 caught = True
                 break
     assert caught, MUTATIONS[mutation]
+
+
+# ---------------------------------------------------------------- parents
+def labels_with_parents(L):
+    """Per-vertex lists of (hub, hop, dist, parent) from an oracle label set."""
+    off, hub, hop, dist = L.arrays()
+    par = L.parents()
+    return [list(zip(hub[off[v]:off[v + 1]].tolist(), hop[off[v]:off[v + 1]].tolist(),
+                     dist[off[v]:off[v + 1]].tolist(), par[off[v]:off[v + 1]].tolist()))
+            for v in range(L.n)]
+
+
+def parent_rule(g, lists):
+    """The parent each K-round label must carry, derived from the finished
+    label set alone: the self label's is the vertex; for (hub, h, d) at y with
+    h >= 1 it is the smallest neighbour x holding (hub, h-1, d - w(x, y)) --
+    the frontier vertices of round h that reach y at distance d."""
+    have = [set((hub, hop, d) for hub, hop, d, _ in lst) for lst in lists]
+    want = []
+    for y, lst in enumerate(lists):
+        row = []
+        for hub, hop, d, _ in lst:
+            if hop == 0:
+                row.append(y)
+                continue
+            xs = [int(g.col[e]) for e in range(g.off[y], g.off[y + 1])
+                  if (hub, hop - 1, d - int(g.w[e])) in have[int(g.col[e])]]
+            row.append(min(xs) if xs else None)
+        want.append(row)
+    return want
+
+
+def test_kround_parents_follow_the_min_id_rule():
+    """The oracle tracks parents while it expands (the smallest frontier
+    vertex at the minimum); the rule above recovers them from the labels.
+    Weight ranges of 1-3 force equal-distance ties.  The clean keeps each
+    kept label's parent."""
+    ties = 0
+    for g, K in tiny_graphs(21, 200, nmax=28):
+        L = O.kround(g, K)
+        lists = labels_with_parents(L)
+        want = parent_rule(g, lists)
+        for y, lst in enumerate(lists):
+            got = [p for _, _, _, p in lst]
+            assert got == want[y], (y, lst, want[y])
+            for hub, hop, d, _ in lst:
+                if hop:
+                    n_at_min = sum(1 for e in range(g.off[y], g.off[y + 1])
+                                   if (hub, hop - 1, d - int(g.w[e])) in
+                                   set((a, b, c) for a, b, c, _ in lists[int(g.col[e])]))
+                    ties += n_at_min > 1
+        by_key = {(v, hub, hop): p for v, lst in enumerate(lists) for hub, hop, _, p in lst}
+        for v, lst in enumerate(labels_with_parents(O.clean(L))):
+            for hub, hop, _, p in lst:
+                assert by_key[(v, hub, hop)] == p
+    assert ties > 0
+
+
+# ------------------------------------------------------------------ paths
+def path_by_argmin(lists, s, t, k):
+    """hop_constrained_extract_shortest_path
+    (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319) with
+    each step's pair chosen as the lexicographic minimum of (dist sum, hub,
+    position in L(s), position in L(t)) over all pairs with hop sum <= k --
+    the pair the reference's merge meets first among those at the minimum."""
+    out = []
+    while s != t:
+        best = None
+        for i, (ha, pa, da, fa) in enumerate(lists[s]):
+            for j, (hb, pb, db, fb) in enumerate(lists[t]):
+                if ha == hb and pa + pb <= k:
+                    key = (da + db, ha, i, j)
+                    if best is None or key < best[0]:
+                        best = (key, fa, fb)
+        if best is None:
+            break
+        _, p1, p2 = best
+        if p1 == s and p2 == t:
+            break
+        if s != p1:
+            out.append((s, p1))
+            s, k = p1, k - 1
+        if t != p2:
+            out.append((t, p2))
+            t, k = p2, k - 1
+    return out
+
+
+def walk_weight(g, s, t, edges):
+    """Total weight of the path's edges if they form a walk from s to t (the
+    s side's edges extend s, the t side's extend t, and the two ends meet),
+    else None."""
+    cs, ct, total = s, t, 0
+    for a, b in edges:
+        nb = g.col[g.off[a]:g.off[a + 1]]
+        j = np.searchsorted(nb, b)
+        if j >= len(nb) or nb[j] != b:
+            return None
+        total += int(g.w[g.off[a] + j])
+        if a == cs:
+            cs = b
+        elif a == ct:
+            ct = b
+        else:
+            return None
+    return total if cs == ct else None
+
+
+def test_paths_are_shortest_walks_and_follow_the_merge_order():
+    """On every (s, t, k) of tiny graphs (cleaned K-round labels): the path
+    is a walk from s to t whose weight is the hop-bounded distance and whose
+    length is at most k (empty when s == t or t is out of reach), and it is
+    the one the declarative argmin above picks."""
+    checked = 0
+    for g, K in tiny_graphs(23, 80, nmax=16):
+        L = O.clean(O.kround(g, K))
+        lists = labels_with_parents(L)
+        D = minplus_hop_distances(g, K)
+        n = g.n
+        s, t, k = zip(*[(a, b, kk) for a in range(n) for b in range(n) for kk in range(0, K + 1)])
+        edges, counts = O.paths(L, s, t, k, 2 * K + 2)
+        for i in range(len(s)):
+            got = [tuple(e) for e in edges[i, :counts[i]].tolist()]
+            assert got == path_by_argmin(lists, s[i], t[i], k[i]), (s[i], t[i], k[i])
+            d = D[k[i]][s[i], t[i]]
+            if s[i] == t[i] or np.isinf(d):
+                assert got == []
+                continue
+            assert len(got) <= k[i]
+            assert walk_weight(g, s[i], t[i], got) == int(d), (s[i], t[i], k[i], got)
+            checked += 1
+    assert checked > 1000
+
+
+def test_planted_parent_and_path_mistakes_are_caught():
+    """Largest parent id instead of the smallest (mutation 7) breaks the
+    parent rule; the last pair at the minimum instead of the first
+    (mutation 6) breaks the merge-order check."""
+    caught7 = caught6 = False
+    for g, K in tiny_graphs(31, 60, nmax=20):
+        L7 = O.kround(g, K, mutation=7)
+        lists = labels_with_parents(L7)
+        want = parent_rule(g, lists)
+        if any([p for _, _, _, p in lst] != want[y] for y, lst in enumerate(lists)):
+            caught7 = True
+        L = O.clean(O.kround(g, K))
+        lists = labels_with_parents(L)
+        n = g.n
+        s, t = zip(*[(a, b) for a in range(n) for b in range(n)])
+        k = [K] * len(s)
+        edges, counts = O.paths(L, s, t, k, 2 * K + 2, mutation=6)
+        for i in range(len(s)):
+            got = [tuple(e) for e in edges[i, :counts[i]].tolist()]
+            if got != path_by_argmin(lists, s[i], t[i], k[i]):
+                caught6 = True
+                break
+        if caught6 and caught7:
+            break
+    assert caught7 and caught6
