@@ -53,7 +53,7 @@ class hsdl_opts(ctypes.Structure):
                 ("deadline_s", ctypes.c_double), ("device", ctypes.c_int32),
                 ("pool_init_labels", ctypes.c_uint64), ("arena_init_words", ctypes.c_uint64),
                 ("kernel_timing", ctypes.c_uint32), ("lds_tile_slots", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("parents", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
 
 
 class hsdl_stats(ctypes.Structure):
@@ -66,7 +66,8 @@ class hsdl_stats(ctypes.Structure):
                 ("scanned_edges", ctypes.c_uint64), ("t_expand_s", ctypes.c_double),
                 ("t_prune_s", ctypes.c_double), ("prune_launches", ctypes.c_uint64),
                 ("prune_label_reads", ctypes.c_uint64), ("prune_tile_reads", ctypes.c_uint64),
-                ("clean_seq_vertices", ctypes.c_uint64), ("clean_label_reads", ctypes.c_uint64)]
+                ("clean_seq_vertices", ctypes.c_uint64), ("clean_label_reads", ctypes.c_uint64),
+                ("t_parents_s", ctypes.c_double)]
 
 
 class hsdl_server_opts(ctypes.Structure):
@@ -100,6 +101,9 @@ _SIGNATURES = [
     ("hsdl_index_stats", ctypes.c_int, [_vp, ctypes.POINTER(hsdl_stats)]),
     ("hsdl_query", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
     ("hsdl_query_device", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp]),
+    ("hsdl_extract_paths", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, ctypes.c_uint32, _vp, _u32p]),
+    ("hsdl_export_parents", ctypes.c_int, [_vp, _u32p]),
+    ("hsdl_index_set_parents", ctypes.c_int, [_vp, _u32p]),
     ("hsdl_label_count", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint64)]),
     ("hsdl_export_labels", ctypes.c_int, [_vp, _u64p, _vp, _vp, _vp]),
     ("hsdl_index_from_labels", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint8, _u64p, _u32p, _u8p, _i32p, ctypes.c_int32, ctypes.POINTER(_vp)]),
@@ -300,6 +304,34 @@ class Index:
                                         hop.ctypes.data_as(_vp), dist.ctypes.data_as(_vp)))
         return off, hub, hop, dist
 
+    def parents(self):
+        """Parent vertex of every label, in the order of labels() (needs an
+        index built with parents=True)."""
+        out = np.zeros(self.label_count(), np.uint32)
+        _check(lib().hsdl_export_parents(self.h, out))
+        return out
+
+    def set_parents(self, parent):
+        """Attaches parents (one per label, labels() order) to an index."""
+        _check(lib().hsdl_index_set_parents(self.h, np.ascontiguousarray(parent, np.uint32)))
+
+    def paths(self, s, t, k, cap=None):
+        """Hop-constrained shortest paths: (edges uint32[nq, cap, 2], counts
+        uint32[nq]); edges are (vertex, parent) pairs in the reference's
+        order.  cap defaults to the largest k (a path has at most k edges)."""
+        s = np.ascontiguousarray(s, np.uint32)
+        t = np.ascontiguousarray(t, np.uint32)
+        k = np.ascontiguousarray(k, np.int8)
+        if not (len(s) == len(t) == len(k)):
+            raise ValueError("s, t, k must have equal length")
+        if cap is None:
+            cap = max(1, int(k.max(initial=0)))
+        edges = np.zeros((len(s), cap, 2), np.uint32)
+        counts = np.zeros(len(s), np.uint32)
+        _check(lib().hsdl_extract_paths(self.h, len(s), s, t, k, cap, edges.ctypes.data_as(_vp),
+                                        counts))
+        return edges, counts
+
     def offsets(self):
         """off[n+1] only (label list of v = [off[v], off[v+1]))."""
         n, _ = self.info()
@@ -328,9 +360,11 @@ class Index:
 
 
 def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, device=-1,
-          pool_init_labels=0, arena_init_words=0, kernel_timing=False, lds_tile_slots=0):
+          pool_init_labels=0, arena_init_words=0, kernel_timing=False, lds_tile_slots=0,
+          parents=False):
     o = hsdl_opts()
     lib().hsdl_opts_init(ctypes.byref(o))
+    o.parents = 1 if parents else 0
     o.kernel_timing = 1 if kernel_timing else 0
     o.lds_tile_slots = lds_tile_slots
     o.pool_init_labels = pool_init_labels
@@ -348,11 +382,12 @@ def hop_constrained_two_hop_labels_generation(graph, K, mode=MODE_BATCHED, hub_b
                                               hbm_budget=0, deadline_s=0.0, device=-1,
                                               clean=False, pool_init_labels=0,
                                               arena_init_words=0, kernel_timing=False,
-                                              lds_tile_slots=0):
+                                              lds_tile_slots=0, parents=False):
     """Generates the K-hop label index of `graph` on the GPU (uncleaned unless
-    clean=True, which also applies hop_constrained_clean_L)."""
+    clean=True, which also applies hop_constrained_clean_L); parents=True
+    also gives every label its parent vertex (for path extraction)."""
     o = _opts(mode, hub_batch, tile, hbm_budget, deadline_s, device, pool_init_labels,
-              arena_init_words, kernel_timing, lds_tile_slots)
+              arena_init_words, kernel_timing, lds_tile_slots, parents)
     h = _vp()
     _check(lib().hsdl_generate(graph.h, K, ctypes.byref(o), ctypes.byref(h)))
     idx = Index(h)
@@ -374,6 +409,15 @@ def hop_constrained_extract_distance(index, s, t, k):
 
 
 gpu_query = hop_constrained_extract_distance
+
+
+def hop_constrained_extract_shortest_path(index, s, t, k):
+    """Edges [(vertex, parent), ...] of the hop-constrained shortest path from
+    s to t, in the order hop_constrained_extract_shortest_path
+    (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319)
+    returns them; empty when s == t or t is out of reach."""
+    edges, counts = index.paths([s], [t], [k], cap=max(1, int(k)))
+    return [tuple(int(x) for x in e) for e in edges[0, :counts[0]]]
 
 
 class Server:

@@ -371,11 +371,13 @@ __global__ void __launch_bounds__(256) k_clean_seq(uint32_t K, const unsigned lo
   }
 }
 
+// Kept labels (and their parents, when par is not null) into the new CSR.
 __global__ void __launch_bounds__(256) k_compact(uint32_t n, const unsigned long long *off,
                                                  const unsigned long long *lab,
                                                  const uint8_t *keep,
                                                  const unsigned long long *noff,
-                                                 unsigned long long *nlab) {
+                                                 unsigned long long *nlab,
+                                                 const uint32_t *par, uint32_t *npar) {
   const int lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
@@ -387,7 +389,10 @@ __global__ void __launch_bounds__(256) k_compact(uint32_t n, const unsigned long
       const bool valid = i + lane < e;
       const bool k = valid && keep[i + lane];
       const unsigned long long mk = __ballot(k);
-      if (k) nlab[dst + __popcll(mk & lt)] = lab[i + lane];
+      if (k) {
+        nlab[dst + __popcll(mk & lt)] = lab[i + lane];
+        if (par) npar[dst + __popcll(mk & lt)] = par[i + lane];
+      }
       dst += __popcll(mk);
     }
   }
@@ -467,13 +472,18 @@ int clean_index(hsdl_index *idx) {
   uint64_t total = 0;
   HSDL_HIP(hipMemcpy(&total, noff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost));
   if ((rc = nlab.alloc(std::max<uint64_t>(total, 1) * 8))) return rc;
+  DevBuf npar;
+  if (idx->has_parents && (rc = npar.alloc(std::max<uint64_t>(total, 1) * 4))) return rc;
   hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, st, n, idx->off.as<unsigned long long>(),
                      idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
-                     noff.as<unsigned long long>(), nlab.as<unsigned long long>());
+                     noff.as<unsigned long long>(), nlab.as<unsigned long long>(),
+                     idx->has_parents ? idx->par.as<uint32_t>() : nullptr,
+                     idx->has_parents ? npar.as<uint32_t>() : nullptr);
   HSDL_HIP(hipGetLastError());
   HSDL_HIP(hipStreamSynchronize(st));
   idx->off = std::move(noff);
   idx->lab = std::move(nlab);
+  if (idx->has_parents) idx->par = std::move(npar);
   idx->total = total;
   idx->cleaned = true;
   idx->stats.labels_cleaned = total;

@@ -179,6 +179,8 @@ struct hsdl_index {
   uint64_t total = 0;
   hsdl::DevBuf off;  // u64[n+1]
   hsdl::DevBuf lab;  // u64[total]
+  hsdl::DevBuf par;  // u32[total]: parent vertex of each label (has_parents)
+  bool has_parents = false;
   bool cleaned = false;
   hsdl_stats stats{};
 };
@@ -193,6 +195,9 @@ int segmented_sort_labels(uint64_t *d_lab, uint64_t total, const uint64_t *d_off
 // Exclusive scan of u32 counts into u64 offsets (n+1 entries, last = total).
 int scan_counts_u32(const uint32_t *d_cnt, uint64_t *d_off, uint32_t n, hipStream_t st);
 int scan_counts_u64(const uint64_t *d_cnt, uint64_t *d_off, uint32_t n, hipStream_t st);
+
+// Parent vertices of an uncleaned K-round index (parents.hip).
+int derive_parents(hsdl_index *idx, const hsdl_graph *g, hipStream_t st);
 
 // Query launch used by hsdl_query_device and the server.
 int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,

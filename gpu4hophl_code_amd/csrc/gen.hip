@@ -1285,7 +1285,11 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint64_t t_sort0 = now_ns();
   if ((rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st)))
     return cleanup(rc);
+  const uint64_t t_sorted = now_ns();
+  // parents (optional): derived from the sorted uncleaned labels and the graph
+  if (opts->parents && (rc = derive_parents(idx, hg, g.st))) return cleanup(rc);
   const uint64_t t_end = now_ns();
+  idx->stats.t_parents_s = (t_end - t_sorted) * 1e-9;
   if (g.verbose)
     fprintf(stderr, "[hsdl] generate: %.3f s batches (%llu, %llu redo; host enqueue %.3f s, wait %.3f s, "
             "grow %.3f s), %.3f s export (%.3f s sort), %llu labels\n",
@@ -1294,7 +1298,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
             (unsigned long long)total);
   idx->stats.labels_uncleaned = total;
   idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
-  idx->stats.t_sort_s = (t_end - t_sort0) * 1e-9;
+  idx->stats.t_sort_s = (t_sorted - t_sort0) * 1e-9;
   idx->stats.hub_batch = tuner.B;
   idx->stats.tile = a.tile;
   idx->stats.rounds = K;

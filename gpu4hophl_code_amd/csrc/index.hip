@@ -184,6 +184,11 @@ int hsdl_save_labels(const hsdl_index *idx, const char *path) {
   std::vector<int32_t> dist(idx->total);
   int rc = hsdl_export_labels(idx, off.data(), hub.data(), hop.data(), dist.data());
   if (rc) return rc;
+  std::vector<uint32_t> par;
+  if (idx->has_parents) {
+    par.resize(idx->total);
+    if ((rc = hsdl_export_parents(idx, par.data()))) return rc;
+  }
   FILE *f = fopen(path, "wb");
   if (!f) return fail(HSDL_EIO, std::string("cannot open ") + path);
   size_t n = idx->n;
@@ -195,7 +200,7 @@ int hsdl_save_labels(const hsdl_index *idx, const char *path) {
     rec.assign(4 * c, 0);
     for (size_t i = 0; i < c; ++i) {
       rec[4 * i + 0] = (int32_t)hub[off[v] + i];
-      rec[4 * i + 1] = 0;
+      rec[4 * i + 1] = par.empty() ? 0 : (int32_t)par[off[v] + i];
       rec[4 * i + 2] = hop[off[v] + i];
       rec[4 * i + 3] = dist[off[v] + i];
     }
@@ -212,28 +217,50 @@ int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **o
   size_t n = 0;
   bool ok = fread(&n, sizeof(size_t), 1, f) == 1 && n <= 0xFFFFFFFFull;
   std::vector<uint64_t> off(1, 0);
-  std::vector<uint32_t> hub;
+  std::vector<uint32_t> hub, par;
   std::vector<uint8_t> hop;
   std::vector<int32_t> dist;
   std::vector<int32_t> rec;
+  // a vertex's records sorted by (hub, hop, dist) with their parents, the
+  // order the index keeps
+  std::vector<std::pair<uint64_t, int32_t>> row;
+  bool any_parent = false;
   for (size_t v = 0; ok && v < n; ++v) {
     size_t c = 0;
     ok = fread(&c, sizeof(size_t), 1, f) == 1 && c < (1ull << 40);
     if (!ok) break;
     rec.resize(4 * c);
     if (c) ok = fread(rec.data(), sizeof(int32_t), 4 * c, f) == 4 * c;
+    row.clear();
     for (size_t i = 0; ok && i < c; ++i) {
-      if (rec[4 * i] < 0 || rec[4 * i + 2] < 0 || rec[4 * i + 2] > 255) { ok = false; break; }
-      hub.push_back((uint32_t)rec[4 * i]);
-      hop.push_back((uint8_t)rec[4 * i + 2]);
-      dist.push_back(rec[4 * i + 3]);
+      if (rec[4 * i] < 0 || rec[4 * i + 2] < 0 || rec[4 * i + 2] > 255 || rec[4 * i + 3] < 0 ||
+          (uint32_t)rec[4 * i + 3] > kMaxDist) {
+        ok = false;
+        break;
+      }
+      row.push_back({pack_label((uint32_t)rec[4 * i], (uint32_t)rec[4 * i + 2],
+                                (uint32_t)rec[4 * i + 3]), rec[4 * i + 1]});
+      any_parent = any_parent || rec[4 * i + 1] != 0;
+    }
+    std::sort(row.begin(), row.end());
+    for (const auto &r : row) {
+      hub.push_back(label_hub(r.first));
+      hop.push_back((uint8_t)label_hop(r.first));
+      dist.push_back((int32_t)label_dist(r.first));
+      par.push_back((uint32_t)r.second);
     }
     off.push_back(hub.size());
   }
   fclose(f);
   if (!ok) return fail(HSDL_EIO, std::string("malformed label file ") + path);
-  return hsdl_index_from_labels((uint32_t)n, K, off.data(), hub.data(), hop.data(), dist.data(),
-                                device, out);
+  int rc = hsdl_index_from_labels((uint32_t)n, K, off.data(), hub.data(), hop.data(), dist.data(),
+                                  device, out);
+  if (rc || !any_parent) return rc;
+  if ((rc = hsdl_index_set_parents(*out, par.data()))) {
+    hsdl_free_index(*out);
+    *out = nullptr;
+  }
+  return rc;
 }
 
 void hsdl_free_index(hsdl_index *idx) {

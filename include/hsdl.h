@@ -124,7 +124,10 @@ typedef struct hsdl_opts {
   uint32_t lds_tile_slots;    /* cap on the hub tile slots kept in LDS, 0 =
                                  as many as 60 KB hold; larger tiles use the
                                  global arena                               */
-  uint32_t reserved[2];
+  uint32_t parents;           /* 1: give every label its parent vertex (the
+                                 smallest frontier vertex at the label's
+                                 distance), needed by hsdl_extract_paths   */
+  uint32_t reserved[1];
 } hsdl_opts;
 
 /* Fills opts with the defaults above. */
@@ -163,6 +166,8 @@ typedef struct hsdl_stats {
   uint64_t clean_seq_vertices;      /* vertices the clean walked in order   */
   uint64_t clean_label_reads;       /* labels of the heads' lists L(u) read
                                        by the clean's first pass            */
+  double t_parents_s;               /* parent derivation (opts.parents),
+                                       included in t_generate_s            */
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 
@@ -182,6 +187,16 @@ int hsdl_query_device(const hsdl_index *idx, size_t n, const uint32_t *d_s,
                       const uint32_t *d_t, const int8_t *d_k, int32_t *d_out,
                       void *hip_stream);
 
+/* Batched hop-constrained shortest paths from host arrays; blocks until
+ * done.  Replaces hop_constrained_extract_shortest_path
+ * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319) and
+ * returns the same edges: query i's path is counts[i] (vertex, parent) pairs
+ * at edges[2*cap*i ...] (the first cap are stored), in the reference's
+ * order -- per step the source side's edge, then the terminal side's.  Empty
+ * when s == t or t is out of reach within k hops.  Needs parents. */
+int hsdl_extract_paths(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, uint32_t cap, uint32_t *edges, uint32_t *counts);
+
 /* ---------------------------------------------------- labels in and out */
 
 int hsdl_label_count(const hsdl_index *idx, uint64_t *total);
@@ -196,9 +211,21 @@ int hsdl_index_from_labels(uint32_t n, uint8_t K, const uint64_t *off, const uin
                            hsdl_index **out);
 int hsdl_index_info(const hsdl_index *idx, uint32_t *n, uint8_t *K);
 
+/* Parent vertex of every label, in hsdl_export_labels order: the vertex
+ * before the label's vertex on the hub's path (the reference's
+ * parent_vertex, hop_constrained_two_hop_labels.h:6-21), fixed here as the
+ * smallest such vertex at the label's distance.  HSDL_EINVAL when the index
+ * has none (built without opts.parents). */
+int hsdl_export_parents(const hsdl_index *idx, uint32_t *parent);
+/* Attaches parents (host array, one per label in hsdl_export_labels order)
+ * to an index made from imported labels. */
+int hsdl_index_set_parents(hsdl_index *idx, const uint32_t *parent);
+
 /* Label checkpoint in the reference's dump layout (Clean_pc/main.cu:11-57:
  * size_t n, then per vertex size_t count + count x {int hub, parent, hop,
- * dist}); parent is written as 0. */
+ * dist}).  parent is written as 0 when the index has no parents; a file
+ * whose parents are all 0 loads without parents (a real parent set gives
+ * vertex v >= 1 the parent v on its self label). */
 int hsdl_save_labels(const hsdl_index *idx, const char *path);
 int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **out);
 

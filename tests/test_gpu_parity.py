@@ -304,3 +304,118 @@ def test_server_results_same_alone_and_in_mix(c1):
     st = srv.stats()
     assert st["completed"] == n and st["batches"] >= 1
     srv.close()
+
+
+# ------------------------------------------------------- parents and paths
+def oracle_with_parents(idx):
+    """Oracle label set holding the GPU index's labels and parents."""
+    off, hub, hop, dist = idx.labels()
+    return O.labels_from_arrays_p(len(off) - 1, off.astype(np.int64), hub, hop, dist,
+                                  idx.parents().astype(np.int32))
+
+
+@pytest.mark.parametrize("graph,K,mode,hub_batch", [
+    (("rmat", 12, 8, 0x5EED0001, 100), 4, "batched", 0),
+    (("rmat", 11, 8, 515, 3), 5, "batched", 64),   # weights 1..3: many ties
+    (("rmat", 10, 8, 516, 2), 4, "plain", 0),
+    (("grid", 20, 20, 517, 4), 8, "batched", 0),
+])
+def test_parents_match_oracle(graph, K, mode, hub_batch):
+    """Parents derived on the GPU from the finished labels equal the oracle's
+    (tracked while it expands: the smallest frontier vertex at the minimum),
+    before and after the clean."""
+    if graph[0] == "rmat":
+        g = H.Graph.rmat(graph[1], graph[2], graph[3], 1, graph[4])
+    else:
+        g = H.Graph.grid(graph[1], graph[2], graph[3], 1, graph[4])
+    c = csr_of(g)
+    want = O.kround(c, K)
+    m = H.MODE_PLAIN if mode == "plain" else H.MODE_BATCHED
+    idx = H.hop_constrained_two_hop_labels_generation(g, K, mode=m, hub_batch=hub_batch,
+                                                      parents=True)
+    same_labels(idx, want.arrays())
+    assert np.array_equal(idx.parents().astype(np.int32), want.parents())
+    idx.clean()
+    cw = O.clean(want)
+    same_labels(idx, cw.arrays())
+    assert np.array_equal(idx.parents().astype(np.int32), cw.parents())
+
+
+def _path_queries(n, K, count, seed):
+    rng = np.random.default_rng(seed)
+    s = rng.integers(0, n, count).astype(np.uint32)
+    t = rng.integers(0, n, count).astype(np.uint32)
+    k = rng.integers(-1, K + 3, count).astype(np.int8)
+    s[:50] = t[:50]  # s == t
+    return s, t, k
+
+
+def test_paths_match_oracle():
+    """hsdl_extract_paths returns the oracle's edges, in the same order, for
+    every query: s == t, k <= 0, k above K, unreachable and ordinary pairs."""
+    g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True, parents=True)
+    want_l = O.clean(O.kround(csr_of(g), 4))
+    s, t, k = _path_queries(4096, 4, 20000, 0x5EED0301)
+    cap = int(k.max())
+    edges, counts = idx.paths(s, t, k, cap=cap)
+    we, wc = O.paths(want_l, s, t, k.astype(np.int32), cap)
+    assert np.array_equal(counts.astype(np.int32), wc)
+    for i in range(len(s)):
+        assert np.array_equal(edges[i, :counts[i]].astype(np.int32), we[i, :wc[i]]), i
+    # the paths of a labels-only import with the parents attached are the same
+    off, hub, hop, dist = idx.labels()
+    imp = H.Index.from_labels(len(off) - 1, 4, off, hub, hop, dist)
+    with pytest.raises(H.HSDLError):
+        imp.paths(s[:4], t[:4], k[:4])
+    imp.set_parents(idx.parents())
+    e2, c2 = imp.paths(s, t, k, cap=cap)
+    assert np.array_equal(c2, counts) and np.array_equal(e2, edges)
+
+
+def test_paths_are_shortest_walks_at_config2_size():
+    """Config 2's graph (R-MAT scale 18, EF 16, K=5): every sampled path is a
+    walk from s to t of at most k edges whose weight is the query distance
+    (empty exactly when the distance is unreachable or s == t)."""
+    g = H.Graph.rmat(18, 16, 0x5EED0002, 1, 100)
+    idx = H.hop_constrained_two_hop_labels_generation(g, 5, clean=True, parents=True)
+    off, col, w = g.csr()
+    s, t, k = _path_queries(len(off) - 1, 5, 4000, 0x5EED0302)
+    d = idx.query(s, t, k)
+    edges, counts = idx.paths(s, t, k, cap=7)
+    for i in range(len(s)):
+        n_e = int(counts[i])
+        if s[i] == t[i] or d[i] == INF:
+            assert n_e == 0
+            continue
+        assert 1 <= n_e <= k[i]
+        cs, ct, total = int(s[i]), int(t[i]), 0
+        for a, b in edges[i, :n_e].tolist():
+            row = col[off[a]:off[a + 1]]
+            j = np.searchsorted(row, b)
+            assert j < len(row) and row[j] == b, (i, a, b)
+            total += int(w[off[a] + j])
+            if a == cs:
+                cs = b
+            elif a == ct:
+                ct = b
+            else:
+                raise AssertionError((i, a, b))
+        assert cs == ct and total == d[i], (i, total, d[i])
+
+
+def test_parents_survive_save_and_load(tmp_path):
+    g = H.Graph.rmat(11, 8, 518, 1, 5)
+    idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True, parents=True)
+    p = tmp_path / "labels.bin"
+    idx.save(str(p))
+    back = H.Index.load(str(p), 4)
+    assert np.array_equal(back.parents(), idx.parents())
+    s, t, k = _path_queries(2048, 4, 3000, 0x5EED0303)
+    assert all(np.array_equal(a, b) for a, b in zip(back.paths(s, t, k, cap=6),
+                                                     idx.paths(s, t, k, cap=6)))
+    # without parents the dump's parent fields are 0 and load without parents
+    plain = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
+    plain.save(str(p))
+    with pytest.raises(H.HSDLError):
+        H.Index.load(str(p), 4).parents()
