@@ -178,6 +178,8 @@ __global__ void __launch_bounds__(256) k_paths(size_t nq, const unsigned long lo
         --k;
       }
     }
+    // slots past the path's end read 0
+    for (uint32_t e = 2 * min(cnt, cap) + (uint32_t)lane; e < 2 * cap; e += kWave) out[e] = 0;
     if (lane == 0) counts[q] = cnt;
   }
 }

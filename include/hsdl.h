@@ -191,9 +191,10 @@ int hsdl_query_device(const hsdl_index *idx, size_t n, const uint32_t *d_s,
  * done.  Replaces hop_constrained_extract_shortest_path
  * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319) and
  * returns the same edges: query i's path is counts[i] (vertex, parent) pairs
- * at edges[2*cap*i ...] (the first cap are stored), in the reference's
- * order -- per step the source side's edge, then the terminal side's.  Empty
- * when s == t or t is out of reach within k hops.  Needs parents. */
+ * at edges[2*cap*i ...] (the first cap are stored, the rest of the query's
+ * 2*cap words are 0), in the reference's order -- per step the source
+ * side's edge, then the terminal side's.  Empty when s == t or t is out of
+ * reach within k hops.  A path has at most max(k, 0) edges.  Needs parents. */
 int hsdl_extract_paths(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
                        const int8_t *k, uint32_t cap, uint32_t *edges, uint32_t *counts);
 
