@@ -380,6 +380,7 @@ def test_paths_are_shortest_walks_at_config2_size():
     g = H.Graph.rmat(18, 16, 0x5EED0002, 1, 100)
     idx = H.hop_constrained_two_hop_labels_generation(g, 5, clean=True, parents=True)
     off, col, w = g.csr()
+    off = off.astype(np.int64)
     s, t, k = _path_queries(len(off) - 1, 5, 4000, 0x5EED0302)
     d = idx.query(s, t, k)
     edges, counts = idx.paths(s, t, k, cap=7)
