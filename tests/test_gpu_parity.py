@@ -375,15 +375,21 @@ def test_paths_match_oracle():
 
 def test_paths_are_shortest_walks_at_config2_size():
     """Config 2's graph (R-MAT scale 18, EF 16, K=5): every sampled path is a
-    walk from s to t of at most k edges whose weight is the query distance
-    (empty exactly when the distance is unreachable or s == t)."""
+    walk from s to t of at most k edges (empty exactly when the distance is
+    unreachable or s == t).  For k <= K its weight is the query distance; for
+    k > K the labels (hops <= K) may miss walks with more than K hops on one
+    side of a hub, so the query distance is an upper bound and the path,
+    which re-queries with the remaining budget after each edge, may be
+    shorter, never below the Bellman-Ford distance."""
     g = H.Graph.rmat(18, 16, 0x5EED0002, 1, 100)
+    c = csr_of(g)
     idx = H.hop_constrained_two_hop_labels_generation(g, 5, clean=True, parents=True)
     off, col, w = g.csr()
     off = off.astype(np.int64)
     s, t, k = _path_queries(len(off) - 1, 5, 4000, 0x5EED0302)
     d = idx.query(s, t, k)
     edges, counts = idx.paths(s, t, k, cap=7)
+    bf_checks = 0
     for i in range(len(s)):
         n_e = int(counts[i])
         if s[i] == t[i] or d[i] == INF:
@@ -402,7 +408,14 @@ def test_paths_are_shortest_walks_at_config2_size():
                 ct = b
             else:
                 raise AssertionError((i, a, b))
-        assert cs == ct and total == d[i], (i, total, d[i])
+        assert cs == ct, (i, cs, ct)
+        if k[i] <= 5:
+            assert total == d[i], (i, total, d[i])
+        else:
+            assert total <= d[i], (i, total, d[i])
+            if bf_checks < 20:  # Bellman-Ford at this size takes ~0.1 s a source
+                assert O.bf(c, int(s[i]), int(k[i]))[t[i]] <= total, (i, total)
+                bf_checks += 1
 
 
 def test_parents_survive_save_and_load(tmp_path):
