@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--plain", default="on", choices=["on", "off"])
     ap.add_argument("--serve", default="on", choices=["on", "off"])
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
+    ap.add_argument("--paths", default="on", choices=["on", "off"],
+                    help="also build with parents and time hop-constrained path extraction")
     ap.add_argument("--rates", default="10e6,30e6,60e6")
     ap.add_argument("--queries", type=int, default=1 << 22)
     ap.add_argument("--plain-deadline", type=float, default=0.0,
@@ -312,6 +314,28 @@ def main():
                           "frac": round(qbytes / qsec / 1e9 / HBM_PEAK_GBS, 4)}}
     log(f"queries: {query}")
 
+    # hop-constrained shortest paths: a second build that also derives every
+    # label's parent, then 2^18 path queries through the host-array API
+    # (uploads and downloads included)
+    paths = None
+    if args.paths == "on":
+        pidx = H.hop_constrained_two_hop_labels_generation(g, K, clean=True, parents=True)
+        pst = pidx.stats()
+        rng = np.random.default_rng(0x5EED0400)
+        npq = 1 << 18
+        ps = rng.integers(0, n, npq).astype(np.uint32)
+        pt = rng.integers(0, n, npq).astype(np.uint32)
+        pk = rng.integers(1, K + 1, npq).astype(np.int8)
+        pidx.paths(ps[:1024], pt[:1024], pk[:1024], cap=K)  # warm-up
+        t0 = time.perf_counter()
+        _, pcounts = pidx.paths(ps, pt, pk, cap=K)
+        psec = time.perf_counter() - t0
+        paths = {"parents_s": pst["t_parents_s"], "queries": npq, "paths_per_s": npq / psec,
+                 "ms": psec * 1e3, "mean_edges": float(pcounts.mean()),
+                 "note": "host arrays in and out (PCIe copies included)"}
+        pidx.free()
+        log(f"paths: {paths}")
+
     serve = None
     if args.serve == "on":
         rates = [float(x) for x in args.rates.split(",")]
@@ -362,7 +386,7 @@ def main():
         "batched_vs_plain": (plain["build_s"] / (ms / 1e3)) if plain and plain["completed"] else None,
         "batched_vs_plain_at_least": (plain["lower_bound_s"] / (ms / 1e3))
         if plain and not plain["completed"] else None,
-        "query": query, "serving": serve, "roofline": roofline,
+        "query": query, "paths": paths, "serving": serve, "roofline": roofline,
         "roofline_other_kernels": roofline_other, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -11,11 +11,15 @@
 //   hop_constrained_extract_distance(L, s, t, k)
 //       (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:144-222)
 //   gpu_query(...)                       (Query_mdnd/src/gpu_query.cu:84)
+//   hop_constrained_extract_shortest_path(L, s, t, k)
+//       (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319)
 // Including this header instead (namespace hsdl_compat) keeps those calls;
 // the work runs in libhsdl.so on the GPU.  Differences a caller can see:
-//   - parent_vertex of exported labels is 0 (the reference's uncleaned GPU
-//     export writes 0 as well, Hybrid_Generation_Clean/include/HBPLL/
-//     hop_constrained_two_hop_labels_generation.h:1483);
+//   - parent_vertex is the smallest vertex before the label's vertex on a
+//     shortest hub path (the reference keeps whichever its scan meets
+//     first); with case_info.compute_parents = false it is 0, like the
+//     reference's uncleaned GPU export (Hybrid_Generation_Clean/include/
+//     HBPLL/hop_constrained_two_hop_labels_generation.h:1483);
 //   - the degree renaming breaks degree ties by old id (the reference's
 //     std::sort leaves them unordered);
 //   - errors throw hsdl_compat::error instead of printing and exiting.
@@ -23,6 +27,7 @@
 #ifndef HSDL_COMPAT_HPP
 #define HSDL_COMPAT_HPP
 
+#include <algorithm>
 #include <climits>
 #include <cstdint>
 #include <stdexcept>
@@ -161,6 +166,9 @@ class hop_constrained_case_info {
   bool use_rank_prune = true;
   bool use_2023WWW_generation = false;
   bool use_canonical_repair = true;
+  // not in the reference: false skips the parent vertices (path extraction
+  // then needs them from elsewhere)
+  bool compute_parents = true;
   double time_initialization = 0, time_generate_labels = 0, time_sortL = 0,
          time_canonical_repair = 0, time_total = 0;
   long long max_bit_size = (long long)1e12;
@@ -188,7 +196,8 @@ class hop_constrained_case_info {
   }
 };
 
-// Copies an index's labels into vector-of-vectors form (parent_vertex = 0).
+// Copies an index's labels into vector-of-vectors form (parent_vertex = 0
+// when the index has no parents).
 inline void export_labels(const hsdl_index *idx,
                           std::vector<std::vector<hop_constrained_two_hop_label>> &L) {
   uint32_t n = 0;
@@ -202,26 +211,40 @@ inline void export_labels(const hsdl_index *idx,
   std::vector<int32_t> dist(total);
   check(hsdl_export_labels(idx, off.data(), hub.data(), hop.data(), dist.data()),
         "hsdl_export_labels");
+  std::vector<uint32_t> par(total, 0);
+  if (total && hsdl_export_parents(idx, par.data()) != HSDL_OK) par.assign(total, 0);
   L.assign(n, {});
   for (uint32_t v = 0; v < n; ++v) {
     L[v].reserve(off[v + 1] - off[v]);
     for (uint64_t i = off[v]; i < off[v + 1]; ++i)
-      L[v].emplace_back((int)hub[i], 0, (int)hop[i], (int)dist[i]);
+      L[v].emplace_back((int)hub[i], (int)par[i], (int)hop[i], (int)dist[i]);
   }
 }
 
-// Index from vector-of-vectors labels (any per-vertex order).
+// Index from vector-of-vectors labels (any per-vertex order); parents are
+// attached when some label has a nonzero parent_vertex.
 inline hsdl_index *import_labels(const std::vector<std::vector<hop_constrained_two_hop_label>> &L,
                                  int K) {
   std::vector<uint64_t> off(1, 0);
-  std::vector<uint32_t> hub;
+  std::vector<uint32_t> hub, par;
   std::vector<uint8_t> hop;
   std::vector<int32_t> dist;
+  bool any_parent = false;
   for (auto &lst : L) {
-    for (auto &x : lst) {
-      hub.push_back((uint32_t)x.hub_vertex);
-      hop.push_back((uint8_t)x.hop);
-      dist.push_back(x.distance);
+    // the index keeps each vertex's labels in (hub, hop, dist) order
+    std::vector<const hop_constrained_two_hop_label *> row;
+    for (auto &x : lst) row.push_back(&x);
+    std::sort(row.begin(), row.end(), [](auto *a, auto *b) {
+      if (a->hub_vertex != b->hub_vertex) return a->hub_vertex < b->hub_vertex;
+      if (a->hop != b->hop) return a->hop < b->hop;
+      return a->distance < b->distance;
+    });
+    for (auto *x : row) {
+      hub.push_back((uint32_t)x->hub_vertex);
+      hop.push_back((uint8_t)x->hop);
+      dist.push_back(x->distance);
+      par.push_back((uint32_t)x->parent_vertex);
+      any_parent = any_parent || x->parent_vertex != 0;
     }
     off.push_back(hub.size());
   }
@@ -229,6 +252,13 @@ inline hsdl_index *import_labels(const std::vector<std::vector<hop_constrained_t
   check(hsdl_index_from_labels((uint32_t)L.size(), (uint8_t)K, off.data(), hub.data(), hop.data(),
                                dist.data(), -1, &idx),
         "hsdl_index_from_labels");
+  if (any_parent) {
+    const int rc = hsdl_index_set_parents(idx, par.data());
+    if (rc != HSDL_OK) {
+      hsdl_free_index(idx);
+      check(rc, "hsdl_index_set_parents");
+    }
+  }
   return idx;
 }
 
@@ -244,13 +274,14 @@ void hop_constrained_two_hop_labels_generation(graph_v_of_v<weight_type> &input_
   hsdl_opts o;
   hsdl_opts_init(&o);
   if (case_info.max_run_time_seconds < 1e11) o.deadline_s = case_info.max_run_time_seconds;
+  o.parents = case_info.compute_parents ? 1u : 0u;
   hsdl_index *idx = nullptr;
   int rc = hsdl_generate(g, (uint8_t)case_info.upper_k, &o, &idx);
   hsdl_free_graph(g);
   check(rc, "hsdl_generate");
   hsdl_stats st;
   hsdl_index_stats(idx, &st);
-  case_info.time_generate_labels = st.t_generate_s - st.t_sort_s;
+  case_info.time_generate_labels = st.t_generate_s - st.t_sort_s - st.t_parents_s;
   case_info.time_sortL = st.t_sort_s;
   case_info.label_size_before_canonical_repair = (double)st.labels_uncleaned;
   if (case_info.use_canonical_repair) {
@@ -348,6 +379,38 @@ inline void gpu_query(hop_constrained_case_info &case_info, int n, const query_i
   }
   check(hsdl_query(case_info.index, (size_t)n, s.data(), t.data(), k.data(), (int32_t *)ans),
         "hsdl_query");
+}
+
+// Hop-constrained shortest path on the resident index of case_info (built
+// with compute_parents): the (vertex, parent) edges in the reference's order.
+inline std::vector<std::pair<int, int>> hop_constrained_extract_shortest_path(
+    hop_constrained_case_info &case_info, int source, int terminal, int hop_cst) {
+  if (!case_info.index) throw error(HSDL_ESTATE, "case_info has no resident index");
+  const uint32_t s = (uint32_t)source, t = (uint32_t)terminal;
+  const int8_t k = (int8_t)(hop_cst > 127 ? 127 : (hop_cst < -1 ? -1 : hop_cst));
+  const uint32_t cap = k > 0 ? (uint32_t)k : 1u;
+  std::vector<uint32_t> edges(2 * cap);
+  uint32_t count = 0;
+  check(hsdl_extract_paths(case_info.index, 1, &s, &t, &k, cap, edges.data(), &count),
+        "hsdl_extract_paths");
+  std::vector<std::pair<int, int>> out;
+  for (uint32_t e = 0; e < count && e < cap; ++e)
+    out.push_back({(int)edges[2 * e], (int)edges[2 * e + 1]});
+  return out;
+}
+
+// The reference's signature on host labels (their parent_vertex fields are
+// used): builds a temporary device index per call.
+inline std::vector<std::pair<int, int>> hop_constrained_extract_shortest_path(
+    std::vector<std::vector<hop_constrained_two_hop_label>> &L, int source, int terminal,
+    int hop_cst) {
+  if (source == terminal) return {};
+  int K = 1;
+  for (auto &lst : L)
+    for (auto &x : lst) K = x.hop > K ? x.hop : K;
+  hop_constrained_case_info tmp;
+  tmp.index = import_labels(L, K);
+  return hop_constrained_extract_shortest_path(tmp, source, terminal, hop_cst);
 }
 
 }  // namespace hsdl_compat

@@ -78,6 +78,40 @@ int main(int argc, char **argv) {
     if (!qs.empty() &&
         hop_constrained_extract_distance(mm.L, qs[0].s, qs[0].t, qs[0].h) != want[0])
       ++bad;
+    // shortest paths: a walk from s to t of at most k edges whose weight is
+    // the distance (the edges extend the s side or the t side until they meet)
+    int paths = 0;
+    for (size_t i = 0; i < qs.size(); ++i) {
+      const auto p = hop_constrained_extract_shortest_path(mm, qs[i].s, qs[i].t, qs[i].h);
+      if (want[i] == INT_MAX || qs[i].s == qs[i].t) {
+        if (!p.empty()) ++bad;
+        continue;
+      }
+      int cs = qs[i].s, ct = qs[i].t;
+      long long w = 0;
+      bool ok = (int)p.size() <= qs[i].h;
+      for (auto &e : p) {
+        bool found = false;
+        for (auto &nb : g[e.first])
+          if (nb.first == e.second) {
+            w += nb.second;
+            found = true;
+          }
+        if (!found) ok = false;
+        if (e.first == cs) cs = e.second;
+        else if (e.first == ct) ct = e.second;
+        else ok = false;
+      }
+      if (!ok || cs != ct || w != want[i]) {
+        printf("path mismatch s=%d t=%d k=%d\n", qs[i].s, qs[i].t, qs[i].h);
+        ++bad;
+      }
+      ++paths;
+      // the host-label overload (parents from mm.L) gives the same edges
+      if (i < 5 && hop_constrained_extract_shortest_path(mm.L, qs[i].s, qs[i].t, qs[i].h) != p)
+        ++bad;
+    }
+    printf("%d paths checked\n", paths);
     // canonical repair of already-canonical labels changes nothing
     const double before = mm.label_size_after_canonical_repair;
     hop_constrained_clean_L(mm, 1);
