@@ -93,7 +93,6 @@ struct Ctrl {
   uint32_t expand_ctr;  // next 64-item chunk of the frontier (k_expand)
   uint32_t heavy_cnt;   // pieces pushed to the heavy list this round
   uint32_t heavy_ctr;   // next heavy piece (k_expand_heavy)
-  unsigned long long work_items;  // prune work items scheduled (all rounds)
 };
 
 // A piece of a high-degree frontier item: up to kHeavyPiece consecutive
@@ -447,7 +446,6 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     a.ctrl->work_ctr = 0;
     a.ctrl->cand_total += partc[t];
     a.ctrl->item_total += min(a.ctrl->fcnt[a.cur], a.fcap);
-    if (!dead) a.ctrl->work_items += part[t];
   }
 }
 
@@ -518,14 +516,9 @@ __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
 }
 
 // ----------------------------------------------------------------- prune
-// Two shapes.  Large: sixteen wavefronts share one tile; with a ~60 KB tile
-// two workgroups fit a CU, which keeps 32 waves (the maximum) streaming
-// label lists per CU.  Small: four wavefronts and a ~15 KB tile, eight
-// workgroups per CU -- for work items of a few candidates each (high-diameter,
-// low-degree graphs), where a large workgroup would leave most waves idle
-// while its one item builds its tile.  The host picks the shape per batch.
+// Sixteen wavefronts share one tile: with a ~60 KB tile two workgroups fit a
+// CU, which keeps 32 waves (the maximum) streaming label lists per CU.
 constexpr uint32_t kPruneThreads = 1024;
-constexpr uint32_t kPruneThreadsSmall = 256;
 
 // The tile of L(s) is the hash tile of device.hpp (hub -> per-hop prefix
 // minimum distance).
@@ -634,8 +627,8 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
   return false;
 }
 
-template <class V, uint32_t kThreads>
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8)))
+template <class V>
+__global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune(GenArgs a) {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next, sh_touch_n, sh_touch_base;
@@ -884,11 +877,8 @@ struct Gen {
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
-  int prune_grid = 0;        // resident k_prune workgroups, large shape (two per CU)
-  int prune_grid_small = 0;  // small shape (eight per CU)
-  size_t lds_bytes = 0, lds_bytes_small = 0;
-  uint32_t lds_slots = 0, lds_slots_small = 0;
-  bool prune_small = false;  // shape of the next batch's prune launches
+  int prune_grid = 0;  // resident k_prune workgroups (two per CU)
+  size_t lds_bytes = 0;
   bool small_vals = false;  // 16-bit tile values
   hipStream_t st = nullptr;
   bool verbose = false;
@@ -999,23 +989,10 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
-    if (g.prune_small) {
-      a.lds_slots = g.lds_slots_small;
-      if (g.small_vals)
-        hipLaunchKernelGGL((k_prune<uint16_t, kPruneThreadsSmall>), dim3(g.prune_grid_small),
-                           dim3(kPruneThreadsSmall), g.lds_bytes_small, g.st, a);
-      else
-        hipLaunchKernelGGL((k_prune<uint32_t, kPruneThreadsSmall>), dim3(g.prune_grid_small),
-                           dim3(kPruneThreadsSmall), g.lds_bytes_small, g.st, a);
-    } else {
-      a.lds_slots = g.lds_slots;
-      if (g.small_vals)
-        hipLaunchKernelGGL((k_prune<uint16_t, kPruneThreads>), dim3(g.prune_grid),
-                           dim3(kPruneThreads), g.lds_bytes, g.st, a);
-      else
-        hipLaunchKernelGGL((k_prune<uint32_t, kPruneThreads>), dim3(g.prune_grid),
-                           dim3(kPruneThreads), g.lds_bytes, g.st, a);
-    }
+    if (g.small_vals)
+      hipLaunchKernelGGL(k_prune<uint16_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
+    else
+      hipLaunchKernelGGL(k_prune<uint32_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
@@ -1063,7 +1040,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   } sg{g.st};
   g.grid = grid_for_device(dev);
   g.prune_grid = g.grid / 2;
-  g.prune_grid_small = g.grid * 2;
 
   // device graph (kept with the graph handle)
   if ((rc = upload_graph(const_cast<hsdl_graph *>(hg), dev))) return rc;
@@ -1119,27 +1095,13 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
   const uint32_t K1 = K + 1;
   g.small_vals = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1) < 0xFFFFull;
-  // (+ ~4 KB of static LDS per workgroup in both shapes)
-  auto slots_for = [&](uint64_t words) {
-    uint32_t sl = 64;
-    while ((g.small_vals ? tile_words<uint16_t>(sl + 64, K1) : tile_words<uint32_t>(sl + 64, K1)) <=
-           words)
-      sl += 64;
-    if (opts->lds_tile_slots) sl = std::min(sl, opts->lds_tile_slots);
-    return sl;
-  };
-  const uint64_t lds_words = 60 * 1024 / 4, lds_words_small = 15 * 1024 / 4;
-  const uint32_t lds_slots = slots_for(lds_words);
-  g.lds_slots = lds_slots;
-  g.lds_slots_small = slots_for(lds_words_small);
+  const uint64_t lds_words = 60 * 1024 / 4;  // + ~3 KB of static LDS
+  uint32_t lds_slots = 64;
+  while ((g.small_vals ? tile_words<uint16_t>(lds_slots + 64, K1)
+                       : tile_words<uint32_t>(lds_slots + 64, K1)) <= lds_words)
+    lds_slots += 64;
+  if (opts->lds_tile_slots) lds_slots = std::min(lds_slots, opts->lds_tile_slots);
   g.lds_bytes = (size_t)lds_words * 4;
-  g.lds_bytes_small = (size_t)lds_words_small * 4;
-  // prune shape: HSDL_PRUNE_THREADS = 256 or 1024 pins it; otherwise each
-  // batch takes the small shape when the previous batch's work items held
-  // fewer than HSDL_PRUNE_SMALL_BELOW candidates on average
-  const uint32_t prune_pin = env_u32("HSDL_PRUNE_THREADS", 0, 0, 1024);
-  const double small_below = (double)env_u32("HSDL_PRUNE_SMALL_BELOW", 48, 0, 256);
-  g.prune_small = prune_pin == kPruneThreadsSmall;
 
   GenArgs &a = g.a;
   a.n = n;
@@ -1252,7 +1214,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       fix.cand_total = c_ok.cand_total;
       fix.item_total = c_ok.item_total;
       fix.edge_total = c_ok.edge_total;
-      fix.work_items = c_ok.work_items;
       HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &fix, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
       HSDL_HIP(hipStreamSynchronize(g.st));
       g.h_grow += (now_ns() - tg) * 1e-9;
@@ -1268,12 +1229,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       const uint64_t tg = now_ns();
       if ((rc = grow_pool(g, c.pool_top, 0))) return rc;
       g.h_grow += (now_ns() - tg) * 1e-9;
-    }
-    if (!prune_pin) {
-      // candidates per prune work item in the batch just finished
-      const double items = (double)(c.work_items - c_ok.work_items);
-      const double cands = (double)(c.cand_total - c_ok.cand_total);
-      if (items > 0) g.prune_small = cands < small_below * items;
     }
     c_ok = c;
     s0 += B;

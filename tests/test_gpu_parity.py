@@ -67,21 +67,6 @@ def test_c1_global_arena_tiles(c1):
     same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
 
 
-@pytest.mark.parametrize("threads", ["256", "1024"])
-def test_prune_shapes_give_the_same_labels(monkeypatch, c1, threads):
-    """Both prune workgroup shapes (4 waves with a 15 KB tile, 16 waves with
-    a 60 KB tile), pinned for the whole build: LDS tiles, arena tiles and
-    tiny batches, and a K=8 grid."""
-    monkeypatch.setenv("HSDL_PRUNE_THREADS", threads)
-    g, z = c1
-    want = (z["lab_off"], z["hub"], z["hop"], z["dist"])
-    for kw in ({}, {"lds_tile_slots": 64, "arena_init_words": 1}, {"hub_batch": 7}):
-        same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, **kw), want)
-    g2 = H.Graph.grid(24, 24, 3, 1, 1000)
-    same_labels(H.hop_constrained_two_hop_labels_generation(g2, 8),
-                O.kround(csr_of(g2), 8).arrays())
-
-
 @pytest.fixture(scope="module")
 def heavy13():
     """R-MAT scale 13, EF 16: 14 vertices with more than 1024 edges (up to
