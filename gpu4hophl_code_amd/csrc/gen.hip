@@ -652,11 +652,18 @@ k_prune(GenArgs a) {
     __syncthreads();
     const uint32_t w = sh_w;
     if (w >= a.ctrl->total_chunks) break;
-    // hub slot b with chunk_off[b] <= w < chunk_off[b+1]
+    // hub slot b with chunk_off[b] <= w < chunk_off[b+1], by a 64-way search:
+    // each round the wave samples 64 evenly spaced offsets of [lo, hi) and
+    // keeps the last one <= w (three dependent loads for 2^16 hubs instead of
+    // sixteen; items of a few candidates each spent most of their time here)
     uint32_t lo = 0, hi = a.B;  // invariant chunk_off[lo] <= w < chunk_off[hi]
     while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (a.chunk_off[mid] <= w) lo = mid; else hi = mid;
+      const uint32_t step = (hi - lo + kWave - 1) / kWave;
+      const uint32_t pos = lo + (uint32_t)lane * step;
+      const bool le = pos < hi && a.chunk_off[pos] <= w;
+      const unsigned long long m = __ballot(le);  // lane 0 (pos = lo) always holds
+      lo += (uint32_t)(63 - __builtin_clzll(m)) * step;
+      hi = min(hi, lo + step);
     }
     const uint32_t b = lo;
     const uint32_t s = a.s0 + b;
