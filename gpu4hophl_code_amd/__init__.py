@@ -92,6 +92,7 @@ _SIGNATURES = [
     ("hsdl_graph_rmat", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_vp)]),
     ("hsdl_graph_grid", ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_vp)]),
     ("hsdl_graph_upload", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("hsdl_graph_release_device", ctypes.c_int, [_vp]),
     ("hsdl_graph_info", ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
     ("hsdl_graph_csr", ctypes.c_int, [_vp, _vp, _vp, _vp]),
     ("hsdl_free_graph", None, [_vp]),
@@ -211,6 +212,10 @@ class Graph:
     def upload(self, device=-1):
         """Keeps a device copy of the CSR for later builds."""
         _check(lib().hsdl_graph_upload(self.h, device))
+
+    def release_device(self):
+        """Frees the device CSR copy and the dedupe slots kept between builds."""
+        _check(lib().hsdl_graph_release_device(self.h))
 
     def save_txt(self, path):
         _check(lib().hsdl_graph_save_txt(self.h, os.fsencode(path)))

@@ -16,6 +16,11 @@ struct hsdl_graph {
   // device copy made by hsdl_graph_upload (off u64[n+1], adj (col,w) u32x2)
   void *d_off = nullptr;
   void *d_adj = nullptr;
+  // the generation's dense dedupe slots, kept between builds on this graph
+  // (tens of GB: freeing and reallocating them per build stalled the
+  // allocations that followed)
+  void *d_slots = nullptr;
+  size_t d_slots_bytes = 0;
   int d_device = -1;
   void (*d_free)(hsdl_graph *) = nullptr;
   ~hsdl_graph() {

@@ -848,8 +848,10 @@ static void free_device_graph(hsdl_graph *g) {
   if (g->d_device >= 0 && cur != g->d_device) (void)hipSetDevice(g->d_device);
   if (g->d_off) (void)hipFree(g->d_off);
   if (g->d_adj) (void)hipFree(g->d_adj);
+  if (g->d_slots) (void)hipFree(g->d_slots);
   if (cur >= 0 && cur != g->d_device) (void)hipSetDevice(cur);
-  g->d_off = g->d_adj = nullptr;
+  g->d_off = g->d_adj = g->d_slots = nullptr;
+  g->d_slots_bytes = 0;
   g->d_device = -1;
 }
 
@@ -880,7 +882,7 @@ static int upload_graph(hsdl_graph *g, int dev) {
 struct Gen {
   GenArgs a{};
   DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
-  DevBuf cand_d, cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
+  DevBuf cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
@@ -1059,6 +1061,10 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
 
   size_t free_b = 0, total_b = 0;
   HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
+  // dense slots kept from an earlier build on this graph count as free: this
+  // build reuses or replaces them
+  hsdl_graph *mg = const_cast<hsdl_graph *>(hg);
+  free_b += mg->d_slots_bytes;
   g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
   const uint32_t c0 = 16;  // initial label slots per vertex
   const uint64_t pool0 = opts->pool_init_labels
@@ -1092,12 +1098,27 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
       (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
       (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
-      (rc = g.touched.alloc(n * 4ull)) || (rc = g.cand_d.alloc((uint64_t)Bmax * n * 4)) ||
+      (rc = g.touched.alloc(n * 4ull)) ||
       (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))) ||
       (rc = g.heavy.alloc((uint64_t)heavy_cap * sizeof(HeavyItem))))
     return rc;
+  {
+    const size_t need = (size_t)Bmax * n * 4;
+    if (mg->d_slots_bytes < need) {
+      if (mg->d_slots) (void)hipFree(mg->d_slots);
+      mg->d_slots = nullptr;
+      mg->d_slots_bytes = 0;
+      hipError_t e = hipMalloc(&mg->d_slots, need);
+      if (e != hipSuccess) {
+        mg->d_slots = nullptr;
+        (void)hipGetLastError();
+        return fail(HSDL_ENOMEM, "hipMalloc of the dense slots failed");
+      }
+      mg->d_slots_bytes = need;
+    }
+  }
 
   // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
   const uint32_t K1 = K + 1;
@@ -1126,7 +1147,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.add_cnt = g.add_cnt.as<uint32_t>();
   a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
-  a.cand_d = g.cand_d.as<uint32_t>();
+  a.cand_d = static_cast<uint32_t *>(mg->d_slots);
   {
     // slot layout: candidate distances need bits for K * max weight; the
     // rest of the word holds the attempt tag
@@ -1262,7 +1283,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   }
   const uint64_t t_gen = now_ns();
   // release batch buffers before the export needs its own
-  g.cand_d.release();
   g.cflat.release();
   g.cgroup.release();
   g.F0.release();
@@ -1323,6 +1343,12 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
 }
 
 }  // namespace hsdl
+
+extern "C" int hsdl_graph_release_device(hsdl_graph *g) {
+  if (!g) return hsdl::fail(HSDL_EINVAL, "null graph");
+  g->drop_device_copy();
+  return HSDL_OK;
+}
 
 extern "C" int hsdl_graph_upload(hsdl_graph *g, int32_t device) {
   if (!g) return hsdl::fail(HSDL_EINVAL, "null graph");

@@ -58,6 +58,18 @@ def test_c1_pool_and_arena_growth_roll_back_and_redo(c1):
     same_labels(idx, (z["lab_off"], z["hub"], z["hop"], z["dist"]))
 
 
+def test_dense_slots_kept_between_builds(c1):
+    """The dedupe slots stay with the graph between builds (reused by a
+    batched, a plain and a small-batch build, then released and made again):
+    the labels never change."""
+    g, z = c1
+    want = (z["lab_off"], z["hub"], z["hop"], z["dist"])
+    for kw in ({}, {"mode": H.MODE_PLAIN}, {"hub_batch": 5}, {}):
+        same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, **kw), want)
+    g.release_device()
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4), want)
+
+
 def test_c1_global_arena_tiles(c1):
     """Tiles capped at 64 LDS slots: most hubs' tiles go to the global arena
     (which also starts too small and grows)."""
