@@ -205,8 +205,6 @@ def main():
     ap.add_argument("--cpu-baseline", default="on", choices=["on", "off"])
     ap.add_argument("--paths", default="on", choices=["on", "off"],
                     help="also build with parents and time hop-constrained path extraction")
-    ap.add_argument("--www2023", default="on", choices=["on", "off"],
-                    help="also time the 2023WWW generation (no rank pruning) as a baseline")
     ap.add_argument("--rates", default="10e6,30e6,60e6")
     ap.add_argument("--queries", type=int, default=1 << 22)
     ap.add_argument("--plain-deadline", type=float, default=0.0,
@@ -338,23 +336,6 @@ def main():
         pidx.free()
         log(f"paths: {paths}")
 
-    # the 2023WWW generation (all neighbours, no rank pruning) on the same
-    # graph: same cleaned labels, more candidates
-    www = None
-    if args.www2023 == "on":
-        t0 = time.perf_counter()
-        widx = H.hop_constrained_two_hop_labels_generation(g, K, clean=True,
-                                                           algorithm=H.ALGO_2023WWW)
-        wsec = time.perf_counter() - t0
-        wst = widx.stats()
-        www = {"build_s": wsec, "generate_s": wst["t_generate_s"], "clean_s": wst["t_clean_s"],
-               "labels_uncleaned": wst["labels_uncleaned"],
-               "labels_cleaned": wst["labels_cleaned"], "candidates": wst["candidates"],
-               "same_cleaned_count": wst["labels_cleaned"] == labels,
-               "kround_speedup": wsec / (ms / 1e3)}
-        widx.free()
-        log(f"2023WWW: {www}")
-
     serve = None
     if args.serve == "on":
         rates = [float(x) for x in args.rates.split(",")]
@@ -405,7 +386,7 @@ def main():
         "batched_vs_plain": (plain["build_s"] / (ms / 1e3)) if plain and plain["completed"] else None,
         "batched_vs_plain_at_least": (plain["lower_bound_s"] / (ms / 1e3))
         if plain and not plain["completed"] else None,
-        "query": query, "paths": paths, "www2023": www, "serving": serve, "roofline": roofline,
+        "query": query, "paths": paths, "serving": serve, "roofline": roofline,
         "roofline_other_kernels": roofline_other, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -35,8 +35,6 @@ LIB_PATH = os.path.join(_HERE, "libhsdl.so")
 
 MODE_BATCHED = 0
 MODE_PLAIN = 1
-ALGO_KROUND = 0
-ALGO_2023WWW = 1
 DROP_SELF_LOOPS = 0x1
 DEDUP_MIN = 0x2
 INT32_MAX = 2147483647
@@ -55,7 +53,7 @@ class hsdl_opts(ctypes.Structure):
                 ("deadline_s", ctypes.c_double), ("device", ctypes.c_int32),
                 ("pool_init_labels", ctypes.c_uint64), ("arena_init_words", ctypes.c_uint64),
                 ("kernel_timing", ctypes.c_uint32), ("lds_tile_slots", ctypes.c_uint32),
-                ("parents", ctypes.c_uint32), ("algorithm", ctypes.c_uint32)]
+                ("parents", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
 
 
 class hsdl_stats(ctypes.Structure):
@@ -368,11 +366,10 @@ class Index:
 
 def _opts(mode=MODE_BATCHED, hub_batch=0, tile=0, hbm_budget=0, deadline_s=0.0, device=-1,
           pool_init_labels=0, arena_init_words=0, kernel_timing=False, lds_tile_slots=0,
-          parents=False, algorithm=ALGO_KROUND):
+          parents=False):
     o = hsdl_opts()
     lib().hsdl_opts_init(ctypes.byref(o))
     o.parents = 1 if parents else 0
-    o.algorithm = algorithm
     o.kernel_timing = 1 if kernel_timing else 0
     o.lds_tile_slots = lds_tile_slots
     o.pool_init_labels = pool_init_labels
@@ -390,14 +387,12 @@ def hop_constrained_two_hop_labels_generation(graph, K, mode=MODE_BATCHED, hub_b
                                               hbm_budget=0, deadline_s=0.0, device=-1,
                                               clean=False, pool_init_labels=0,
                                               arena_init_words=0, kernel_timing=False,
-                                              lds_tile_slots=0, parents=False,
-                                              algorithm=ALGO_KROUND):
+                                              lds_tile_slots=0, parents=False):
     """Generates the K-hop label index of `graph` on the GPU (uncleaned unless
     clean=True, which also applies hop_constrained_clean_L); parents=True
-    also gives every label its parent vertex (for path extraction);
-    algorithm=ALGO_2023WWW runs the 2023WWW search (no rank pruning)."""
+    also gives every label its parent vertex (for path extraction)."""
     o = _opts(mode, hub_batch, tile, hbm_budget, deadline_s, device, pool_init_labels,
-              arena_init_words, kernel_timing, lds_tile_slots, parents, algorithm)
+              arena_init_words, kernel_timing, lds_tile_slots, parents)
     h = _vp()
     _check(lib().hsdl_generate(graph.h, K, ctypes.byref(o), ctypes.byref(h)))
     idx = Index(h)

@@ -113,7 +113,6 @@ struct GenArgs {
   uint32_t h;
   uint32_t cur;        // frontier buffer read this round
   uint32_t tile;       // candidates per prune work item
-  uint32_t rank_prune; // 1: expand only neighbours y > s (K-round); 0: all (2023WWW)
   uint32_t lds_slots;  // LDS tile capacity (slots)
   const unsigned long long *off;
   const uint2 *adj;
@@ -194,12 +193,8 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t val) {
 
 __global__ void k_batch_begin(GenArgs a) {
   Item *F = frontier(a, a.cur);
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x)
     F[b] = Item{b, a.s0 + b, 0u};
-    // without rank pruning a walk can come back to the hub: its own slot
-    // starts at distance 0, so such candidates never undercut it
-    if (!a.rank_prune) a.cand_d[(uint64_t)b * a.n + a.s0 + b] = slot_stale(a, 0u);
-  }
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->fcnt[a.cur] = a.B;
 }
 
@@ -340,7 +335,7 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       ib = it.b;
       id = it.d;
       const unsigned long long e0 = a.off[it.y], e1 = a.off[it.y + 1];
-      beg = a.rank_prune ? first_above(a.adj, e0, e1, a.s0 + it.b) : e0;  // rank pruning y > s
+      beg = first_above(a.adj, e0, e1, a.s0 + it.b);  // rank pruning y > s
       deg = (uint32_t)(e1 - beg);
       if (deg > kHeavyDeg) {
         const uint32_t pieces = (deg + kHeavyPiece - 1) / kHeavyPiece;
@@ -555,9 +550,6 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals
   for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
     const unsigned long long l = a.pool[st + i];
     const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
-    // hubs above s can only come from hubs of the same batch (no rank
-    // pruning): the sequential order has not run them yet when s runs
-    if (hub > s) continue;
     uint32_t slot = hash_slot(hub, cap);
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
@@ -1144,8 +1136,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.K = K;
   // accepted candidates of a work item are staged in 256 LDS slots
   a.tile = opts->tile ? std::min<uint32_t>(opts->tile, 256u) : 256u;
-  if (opts->algorithm > HSDL_ALGO_2023WWW) return fail(HSDL_EINVAL, "unknown algorithm");
-  a.rank_prune = opts->algorithm == HSDL_ALGO_KROUND ? 1u : 0u;
   a.lds_slots = lds_slots;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);

@@ -132,17 +132,8 @@ typedef struct hsdl_opts {
   uint32_t parents;           /* 1: give every label its parent vertex (the
                                  smallest frontier vertex at the label's
                                  distance), needed by hsdl_extract_paths   */
-  uint32_t algorithm;         /* HSDL_ALGO_KROUND (default) or
-                                 HSDL_ALGO_2023WWW                          */
+  uint32_t reserved[1];
 } hsdl_opts;
-/* HSDL_ALGO_2023WWW: the 2023WWW generation
- * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:226-370),
- * a per-hub search over all neighbours with no rank pruning, where the
- * pruning query alone keeps lower vertices out.  With one hub per batch
- * (HSDL_MODE_PLAIN) it gives exactly the K-round labels; batched, hubs of one
- * batch do not see each other's labels, so it can add labels the clean then
- * removes.  A baseline: more candidates for the same labels. */
-enum { HSDL_ALGO_KROUND = 0, HSDL_ALGO_2023WWW = 1 };
 
 /* Fills opts with the defaults above. */
 void hsdl_opts_init(hsdl_opts *opts);

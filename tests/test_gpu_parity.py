@@ -70,33 +70,6 @@ def test_dense_slots_kept_between_builds(c1):
     same_labels(H.hop_constrained_two_hop_labels_generation(g, 4), want)
 
 
-def test_2023www_plain_gives_the_kround_labels_and_batched_cleans_to_them(c1):
-    """The 2023WWW search on the GPU (all neighbours, no rank pruning): one
-    hub per batch yields exactly the K-round labels (the golden set, which
-    the oracle's sequential 2023WWW restatement also yields); batched, hubs of
-    a batch do not see each other's labels, so it may add labels, never drop
-    one, and the clean returns the golden set."""
-    g, z = c1
-    want = (z["lab_off"], z["hub"], z["hop"], z["dist"])
-    plain = H.hop_constrained_two_hop_labels_generation(g, 4, mode=H.MODE_PLAIN,
-                                                        algorithm=H.ALGO_2023WWW)
-    same_labels(plain, want)
-    same_labels(plain, O.www2023(csr_of(g), 4).arrays())
-    for hb in (0, 64):
-        idx = H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=hb,
-                                                          algorithm=H.ALGO_2023WWW)
-        off, hub, hop, dist = idx.labels()
-        got = set()
-        for v in range(len(off) - 1):
-            for i in range(off[v], off[v + 1]):
-                got.add((v, int(hub[i]), int(hop[i]), int(dist[i])))
-        for v in range(len(z["lab_off"]) - 1):
-            for i in range(z["lab_off"][v], z["lab_off"][v + 1]):
-                assert (v, int(z["hub"][i]), int(z["hop"][i]), int(z["dist"][i])) in got
-        idx.clean()
-        same_labels(idx, want)
-
-
 def test_c1_global_arena_tiles(c1):
     """Tiles capped at 64 LDS slots: most hubs' tiles go to the global arena
     (which also starts too small and grows)."""
