@@ -271,6 +271,77 @@ Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) 
 }
 
 // ---------------------------------------------------------------------------
+// 2023WWW generation, _2023WWW_thread_function + its driver:
+// CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:226-370
+// (per hub), :513-604 (hubs 0..N-1).  Run single-threaded in hub order, the
+// reference's output with thread_num = 1.  Per hub v_k: a level-synchronous
+// search over hops h = 0..K with one distance-so-far per vertex (dist_hop,
+// :262-279, improved by the previous level's pushes at the start of a
+// level); vertex u of level h is labelled (v_k, h, dist(u)) iff dist(u) is
+// below the pair minimum of L(u) and the snapshot of L(v_k) taken before the
+// search (:244-256, hop sum <= h, :289-303); a labelled u pushes every
+// neighbour it improves (:318-327).  No rank pruning (:284-288 is commented
+// out in the reference).  The reference's level lists may hold a vertex
+// twice; the second visit makes the same decision and at most repeats a
+// label, so the restatement keeps each level as a set.
+// ---------------------------------------------------------------------------
+Labels *www2023(const Graph &g, int K, int mutation) {
+  const int32_t n = g.n;
+  Labels *res = new Labels();
+  LabelSet &L = res->L;
+  L.assign(n, {});
+  std::vector<int32_t> dist(n, INF32), stamp(n, -1), changed;
+  HubTable T(n);
+  std::vector<int32_t> cur, nxt;
+  std::vector<std::pair<int32_t, int32_t>> updates;  // (vertex, distance)
+  for (int32_t vk = 0; vk < n; ++vk) {
+    for (const Label &x : L[vk]) T.add(x.hub, x.dist, x.hop);
+    cur.assign(1, vk);
+    dist[vk] = 0;
+    changed.push_back(vk);
+    for (int h = 0; h <= K; ++h) {
+      for (const auto &u : updates) {
+        if (u.second < dist[u.first]) {
+          if (dist[u.first] == INF32) changed.push_back(u.first);
+          dist[u.first] = u.second;
+        }
+      }
+      updates.clear();
+      nxt.clear();
+      for (int32_t u : cur) {
+        const int32_t Pu = dist[u];
+        const int64_t q = pair_min(L[u], T, h, 0);
+        if (Pu < q) {
+          L[u].push_back({vk, h, Pu, 0});
+          for (int64_t e = g.off[u]; e < g.off[u + 1]; ++e) {
+            const int32_t x = g.col[e];
+            const int64_t nd = (int64_t)Pu + g.w[e];
+            // mutation 8 also pushes a neighbour the walk does not improve
+            if (nd < dist[x] || (mutation == 8 && nd == dist[x])) {
+              if (stamp[x] != h + 1) {
+                stamp[x] = h + 1;
+                nxt.push_back(x);
+              }
+              updates.push_back({x, (int32_t)nd});
+            }
+          }
+        }
+      }
+      std::swap(cur, nxt);
+      for (int32_t x : cur) stamp[x] = -1;
+    }
+    updates.clear();
+    for (int32_t v : changed) dist[v] = INF32;
+    changed.clear();
+    T.clear();
+  }
+  sort_all(L);
+  res->accepted = 0;
+  for (auto &v : L) res->accepted += (int64_t)v.size();
+  return res;
+}
+
+// ---------------------------------------------------------------------------
 // Dijkstra generation of CPU_HopHL (the parity target):
 // HSDL_thread_function, CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:45-224,
 // heap order (dist asc, hop asc) from CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:42-52,
@@ -738,6 +809,12 @@ orc_labels *orc_kround_grouped(int32_t n, const int64_t *off, const int32_t *col
                                const int32_t *w, int K, int group, int threads) {
   Graph g = make_graph(n, off, col, w);
   return kround(g, K, threads, 0, group);
+}
+
+orc_labels *orc_2023www(int32_t n, const int64_t *off, const int32_t *col,
+                        const int32_t *w, int K, int mutation) {
+  Graph g = make_graph(n, off, col, w);
+  return www2023(g, K, mutation);
 }
 
 orc_labels *orc_dijkstra(int32_t n, const int64_t *off, const int32_t *col,

@@ -33,6 +33,8 @@ def lib():
     L.orc_kround.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.orc_kround_grouped.restype = vp
     L.orc_kround_grouped.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_2023www.restype = vp
+    L.orc_2023www.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int]
     L.orc_dijkstra.restype = vp
     L.orc_dijkstra.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int]
     L.orc_canonical_bruteforce.restype = vp
@@ -155,6 +157,11 @@ def kround(g, K, threads=0, mutation=0):
 def kround_grouped(g, K, group, threads=0):
     """K-round labels with pruning restricted to hubs of the same id group."""
     return OracleLabels(lib().orc_kround_grouped(g.n, g.off, g.col, g.w, K, group, threads))
+
+
+def www2023(g, K, mutation=0):
+    """The 2023WWW per-hub level search, single-threaded in hub order."""
+    return OracleLabels(lib().orc_2023www(g.n, g.off, g.col, g.w, K, mutation))
 
 
 def dijkstra(g, K, mutation=0):

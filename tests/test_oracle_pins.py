@@ -379,3 +379,36 @@ def test_planted_parent_and_path_mistakes_are_caught():
         if caught6 and caught7:
             break
     assert caught7 and caught6
+
+
+# ---------------------------------------------------------------- 2023WWW
+def test_2023www_labels_are_first_hop_exact_distances_and_clean_to_canonical():
+    """The 2023WWW search (no rank pruning, one distance-so-far per vertex
+    across hops): every label (hub, h, d) of u has d equal to the <= h-edge
+    distance from hub to u and strictly below the <= h-1-edge one (h is the
+    first hop count reaching d); the clean of its labels is the brute-force
+    canonical labeling.  Without rank pruning the queries still keep every
+    hub at or above its vertex (hub <= u): for u < hub the pair is already
+    covered by hub u's own labels.  Run in hub order it yields exactly the
+    K-round set.  Mutation 8 (a neighbour pushed without improvement) breaks
+    the first property."""
+    for g, K in tiny_graphs(41, 200, nmax=22):
+        W = O.www2023(g, K)
+        D = minplus_hop_distances(g, K)
+        for u, lst in enumerate(W.as_lists()):
+            for hub, h, d in lst:
+                assert d == D[h][hub, u], (u, hub, h, d)
+                assert h == 0 or d < D[h - 1][hub, u], (u, hub, h, d)
+                assert hub <= u
+        want = O.canonical(g, K).as_lists()
+        assert O.clean(W, 0).as_lists() == want and O.clean(W, 1).as_lists() == want
+        assert W.as_lists() == O.kround(g, K).as_lists()
+    caught = False
+    for g, K in tiny_graphs(43, 80, nmax=18):
+        D = minplus_hop_distances(g, K)
+        for u, lst in enumerate(O.www2023(g, K, mutation=8).as_lists()):
+            if any(h > 0 and d >= D[h - 1][hub, u] for hub, h, d in lst):
+                caught = True
+        if caught:
+            break
+    assert caught
