@@ -627,13 +627,53 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
   return false;
 }
 
+// Accepted labels staged in LDS by a prune workgroup (one work item holds at
+// most `tile` <= kAccSlots candidates).
+constexpr uint32_t kAccSlots = 256;
+
+// Publishes the staged accepted labels (whole workgroup; the caller has
+// synchronised after staging): per-vertex add counts, the vertices touched
+// for the first time this round gathered in LDS, one reservation each on the
+// frontier and the touched list, then the writes.  Empties the staging.
+__device__ inline void publish_accepted(const GenArgs &a, Item *Fn, Item *sh_acc,
+                                        uint32_t *sh_touch, uint32_t &sh_acc_n,
+                                        uint32_t &sh_acc_base, uint32_t &sh_touch_n,
+                                        uint32_t &sh_touch_base) {
+  const uint32_t nacc = sh_acc_n;
+  for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x) {
+    const uint32_t y = sh_acc[i].y;
+    if (atomicAdd(&a.add_cnt[y], 1u) == 0) sh_touch[atomicAdd(&sh_touch_n, 1u)] = y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (nacc) {
+      sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], nacc);
+      if (sh_acc_base + nacc > a.fcap) atomicOr(&a.ctrl->overflow, kOvfFront);
+    }
+    if (sh_touch_n) sh_touch_base = atomicAdd(&a.ctrl->touched_cnt, sh_touch_n);
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x)
+    if (sh_acc_base + i < a.fcap) Fn[sh_acc_base + i] = sh_acc[i];
+  const uint32_t ntouch = sh_touch_n;
+  for (uint32_t i = threadIdx.x; i < ntouch; i += blockDim.x)
+    a.touched[sh_touch_base + i] = sh_touch[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    sh_acc_n = 0;
+    sh_touch_n = 0;
+  }
+  __syncthreads();
+}
+
 template <class V>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune(GenArgs a) {
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next, sh_touch_n, sh_touch_base;
-  __shared__ Item sh_acc[256];
-  __shared__ uint32_t sh_touch[256];
+  __shared__ uint32_t sh_item, sh_item_end;
+  __shared__ Item sh_acc[kAccSlots];
+  __shared__ uint32_t sh_touch[kAccSlots];
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
   Item *Fn = frontier(a, a.cur ^ 1);
@@ -641,17 +681,32 @@ k_prune(GenArgs a) {
   unsigned long long wave_reads = 0, tile_reads = 0;
   const uint64_t gwords = a.gtile_cap / gridDim.x;
   uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
-  for (bool first = true;; first = false) {
+  const uint32_t total = a.ctrl->total_chunks;
+  // Work items are claimed `claim` at a time: four when the round's items
+  // hold fewer than 32 candidates on average (late hubs, low-degree graphs),
+  // where one claim per item made the shared counter the bottleneck; one
+  // otherwise, so large items stay balanced.  A workgroup's first claim is
+  // its own id, later ones come from the counter.
+  const uint32_t claim = (uint64_t)a.hub_off[a.B] < 32ull * total ? 4u : 1u;
+  if (threadIdx.x == 0) {
+    sh_item = blockIdx.x * claim;
+    sh_item_end = sh_item + claim;
+    sh_acc_n = 0;
+    sh_touch_n = 0;
+  }
+  __syncthreads();
+  for (;;) {
     if (threadIdx.x == 0) {
-      // a workgroup's first item is its own id, later ones come from a counter
-      sh_w = first ? blockIdx.x : gridDim.x + atomicAdd(&a.ctrl->work_ctr, 1u);
-      sh_acc_n = 0;
+      if (sh_item == sh_item_end) {
+        sh_item = gridDim.x * claim + atomicAdd(&a.ctrl->work_ctr, claim);
+        sh_item_end = sh_item + claim;
+      }
+      sh_w = sh_item++;
       sh_next = 0;
-      sh_touch_n = 0;
     }
     __syncthreads();
     const uint32_t w = sh_w;
-    if (w >= a.ctrl->total_chunks) break;
+    if (w >= total) break;
     // hub slot b with chunk_off[b] <= w < chunk_off[b+1], by a 64-way search:
     // each round the wave samples 64 evenly spaced offsets of [lo, hi) and
     // keeps the last one <= w (three dependent loads for 2^16 hubs instead of
@@ -669,6 +724,10 @@ k_prune(GenArgs a) {
     const uint32_t s = a.s0 + b;
     const uint32_t c_beg = (w - a.chunk_off[b]) * a.tile;
     const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
+    // the staged labels of earlier items are published once this item's
+    // could overflow the staging slots (c_end - c_beg <= tile <= kAccSlots)
+    if (sh_acc_n + (c_end - c_beg) > kAccSlots)
+      publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
     const uint32_t *cands = a.cgroup + a.hub_off[b];
     const uint32_t cap = tile_cap_for(a.seg_len[s]);
     uint32_t *keys;
@@ -720,29 +779,8 @@ k_prune(GenArgs a) {
       }
     }
     __syncthreads();
-    // the item's accepted labels: per-vertex add counts; vertices touched
-    // for the first time this round are gathered in LDS and published with
-    // one reservation per item (no per-label atomic on the shared counter)
-    const uint32_t nacc = sh_acc_n;
-    for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x) {
-      const uint32_t y = sh_acc[i].y;
-      if (atomicAdd(&a.add_cnt[y], 1u) == 0) sh_touch[atomicAdd(&sh_touch_n, 1u)] = y;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      if (nacc) {
-        sh_acc_base = atomicAdd(&a.ctrl->fcnt[a.cur ^ 1], nacc);
-        if (sh_acc_base + nacc > a.fcap) atomicOr(&a.ctrl->overflow, kOvfFront);
-      }
-      if (sh_touch_n) sh_touch_base = atomicAdd(&a.ctrl->touched_cnt, sh_touch_n);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x)
-      if (sh_acc_base + i < a.fcap) Fn[sh_acc_base + i] = sh_acc[i];
-    for (uint32_t i = threadIdx.x; i < sh_touch_n; i += blockDim.x)
-      a.touched[sh_touch_base + i] = sh_touch[i];
-    __syncthreads();
   }
+  publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
   if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
   if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
 }
