@@ -682,12 +682,14 @@ k_prune(GenArgs a) {
   const uint64_t gwords = a.gtile_cap / gridDim.x;
   uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
   const uint32_t total = a.ctrl->total_chunks;
-  // Work items are claimed `claim` at a time: four when the round's items
-  // hold fewer than 32 candidates on average (late hubs, low-degree graphs),
-  // where one claim per item made the shared counter the bottleneck; one
-  // otherwise, so large items stay balanced.  A workgroup's first claim is
-  // its own id, later ones come from the counter.
-  const uint32_t claim = (uint64_t)a.hub_off[a.B] < 32ull * total ? 4u : 1u;
+  // Work items are claimed `claim` at a time, enough for about 64 candidates
+  // per claim (1..8 items): with one claim per item, rounds of small items
+  // (late hubs, low-degree graphs) were bound by the shared counter, while
+  // items of 64+ candidates stay one per claim to keep the load balanced.  A
+  // workgroup's first claim is its own id, later ones come from the counter.
+  const uint64_t cands = a.hub_off[a.B];
+  const uint32_t claim =
+      cands ? (uint32_t)std::min<uint64_t>(8, std::max<uint64_t>(1, (64ull * total) / cands)) : 1u;
   if (threadIdx.x == 0) {
     sh_item = blockIdx.x * claim;
     sh_item_end = sh_item + claim;
