@@ -68,7 +68,8 @@ __device__ inline uint32_t lb_hub(const unsigned long long *lab, uint32_t lo, ui
 
 // ---------------------------------------------------------------- pass 1
 // Slot of `hub` in a tile, or kInf32 when absent.
-__device__ inline uint32_t tile_find(const uint32_t *keys, uint32_t cap, uint32_t hub) {
+template <class KP>
+__device__ inline uint32_t tile_slot(KP keys, uint32_t cap, uint32_t hub) {
   uint32_t slot = hash_slot(hub, cap);
   for (;;) {
     const uint32_t k = keys[slot];
@@ -81,14 +82,17 @@ __device__ inline uint32_t tile_find(const uint32_t *keys, uint32_t cap, uint32_
 // True when the all-earlier predicate removes a label of a group whose head
 // lies in L(v)[p0, p0 + win).  The wave walks the window's heads in order
 // (uniform control flow: the group's labels are uniform scalar loads) and
-// streams each head's list L(u) 256 elements per step with coalesced loads;
-// only the filter and the tile probe are per lane.
-template <class V>
+// streams each head's list L(u) 128 elements per step with coalesced loads;
+// only the filter and the tile probe are per lane.  The two step buffers
+// alternate (a, b) and loads are clamped to the list (lanes past the end are
+// masked), so a step's probes run while the next step's loads are in flight.
+// KP / VP: the tile's address space (LDS or the global arena).
+template <class V, class KP, class VP>
 __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off,
                                const unsigned long long *__restrict__ lab,
                                const unsigned long long *__restrict__ Lv, uint32_t len,
-                               uint32_t p0, uint32_t win, const uint32_t *keys, const V *vals,
-                               uint32_t cap, uint64_t &streamed) {
+                               uint32_t p0, uint32_t win, KP keys, VP vals, uint32_t cap,
+                               uint64_t &streamed) {
   const uint32_t lane = (uint32_t)lane_id();
   bool hit = false;
   const uint32_t pend = min(p0 + win, len);
@@ -104,50 +108,108 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     }
     const unsigned long long ub = off[u];
     const uint32_t ulen = (uint32_t)(off[u + 1] - ub);
+    if (ulen == 0) continue;
     const unsigned long long *Lu = lab + ub;
     streamed += ulen;
-    constexpr int kSteps = 4;
-    for (uint32_t jb = 0; jb < ulen; jb += kSteps * kWave) {
-      unsigned long long el[kSteps];
-#pragma unroll
-      for (int r = 0; r < kSteps; ++r) {
-        const uint32_t j = jb + r * kWave + lane;
-        el[r] = j < ulen ? Lu[j] : ~0ull;  // hub ~0u: never < u, never == u
-      }
-#pragma unroll
-      for (int r = 0; r < kSteps; ++r) {
-        const unsigned long long l = el[r];
-        const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
-        if (w < u) {
-          // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
-          if (h1 + 1 <= maxh && d1 <= maxd) {
-            const uint32_t slot = tile_find(keys, cap, w);
-            if (slot != kInf32) {
-              for (uint32_t t = 0; t < glen; ++t) {
-                const unsigned long long g = Lv[p + t];
-                const uint32_t hg = label_hop(g);
-                if (hg > h1) {
-                  const uint32_t m = vals[slot * K1 + (hg - h1)];
-                  if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
-                }
+    const uint32_t last = ulen - 1;
+    // one element (w,h1,d1) of L(u) at position j against the group
+    auto check = [&](unsigned long long l, uint32_t j) {
+      if (j >= ulen) return;
+      const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
+      if (w < u) {
+        // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
+        if (h1 + 1 <= maxh && d1 <= maxd) {
+          const uint32_t slot = tile_slot(keys, cap, w);
+          if (slot != kInf32) {
+            for (uint32_t t = 0; t < glen; ++t) {
+              const unsigned long long g = Lv[p + t];
+              const uint32_t hg = label_hop(g);
+              if (hg > h1) {
+                const uint32_t m = vals[slot * K1 + (hg - h1)];
+                if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
               }
             }
           }
-        } else if (w == u) {
-          // through hub u: earlier labels of the group against later ones
-          for (uint32_t t1 = 0; t1 < glen; ++t1) {
-            const unsigned long long g1 = Lv[p + t1];
-            for (uint32_t t2 = t1 + 1; t2 < glen; ++t2) {
-              const unsigned long long g2 = Lv[p + t2];
-              if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
-                hit = true;
-            }
+        }
+      } else if (w == u) {
+        // through hub u: earlier labels of the group against later ones
+        for (uint32_t t1 = 0; t1 < glen; ++t1) {
+          const unsigned long long g1 = Lv[p + t1];
+          for (uint32_t t2 = t1 + 1; t2 < glen; ++t2) {
+            const unsigned long long g2 = Lv[p + t2];
+            if (h1 + label_hop(g1) <= label_hop(g2) && d1 + label_dist(g1) <= label_dist(g2))
+              hit = true;
           }
         }
       }
+    };
+    auto ld = [&](uint32_t j) { return Lu[min(j, last)]; };
+    unsigned long long a0 = ld(lane), a1 = ld(kWave + lane);
+    for (uint32_t jb = 0; jb < ulen; jb += 4 * kWave) {
+      const unsigned long long b0 = ld(jb + 2 * kWave + lane), b1 = ld(jb + 3 * kWave + lane);
+      check(a0, jb + lane);
+      check(a1, jb + kWave + lane);
+      if (jb + 2 * kWave >= ulen) break;
+      a0 = ld(jb + 4 * kWave + lane);
+      a1 = ld(jb + 5 * kWave + lane);
+      check(b0, jb + 2 * kWave + lane);
+      check(b1, jb + 3 * kWave + lane);
     }
   }
   return __ballot(hit) != 0;
+}
+
+// Builds the tile of L(v) (keys, vals in LDS or in the workgroup's arena
+// slice) and runs the windows of L(v) over it; true when some window removes
+// a label.  Whole workgroup.
+template <class V, class KP, class VP>
+__device__ __attribute__((always_inline)) inline bool clean_vertex(
+    uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off,
+    const unsigned long long *__restrict__ lab, const unsigned long long *__restrict__ Lv,
+    uint32_t len, KP keys, VP vals, uint32_t cap, bool in_arena, uint32_t win, uint32_t &sh_win,
+    uint32_t &sh_removed, uint64_t &streamed) {
+  const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id();
+  for (uint32_t i = tid; i < cap; i += kTileThreads) keys[i] = kInf32;
+  __syncthreads();
+  // one thread per group: claim the hub's slot, write its per-hop prefix minima
+  for (uint32_t i = tid; i < len; i += kTileThreads) {
+    const uint32_t w = label_hub(Lv[i]);
+    if (i > 0 && label_hub(Lv[i - 1]) == w) continue;
+    uint32_t slot = hash_slot(w, cap);
+    while (atomicCAS((uint32_t *)&keys[slot], kInf32, w) != kInf32)
+      slot = (slot + 1 == cap) ? 0 : slot + 1;
+    uint32_t m[kMaxK + 1];
+#pragma unroll
+    for (uint32_t x = 0; x <= kMaxK; ++x) m[x] = kInf32;
+    for (uint32_t q = i; q < len && label_hub(Lv[q]) == w; ++q) {
+      const uint32_t h = label_hop(Lv[q]), d = label_dist(Lv[q]);
+#pragma unroll
+      for (uint32_t x = 0; x <= kMaxK; ++x)
+        if (x == h) m[x] = min(m[x], d);
+    }
+    uint32_t run = kInf32;
+#pragma unroll
+    for (uint32_t x = 0; x <= kMaxK; ++x) {
+      run = min(run, m[x]);
+      if (x < K1) vals[slot * K1 + x] = (V)min(run, (uint32_t)TileVal<V>::kEmpty);
+    }
+  }
+  // an arena tile was written through L2: drop stale L1 lines before reading
+  if (in_arena) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (in_arena) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  for (;;) {
+    uint32_t p0 = 0;
+    if (lane == 0) p0 = atomicAdd(&sh_win, win);
+    p0 = __builtin_amdgcn_readfirstlane(__shfl(p0, 0, kWave));
+    if (p0 >= len) break;
+    if (window_removes<V>(v, K1, off, lab, Lv, len, p0, win, keys, vals, cap, streamed)) {
+      if (lane == 0) sh_removed = 1;
+      break;
+    }
+  }
+  __syncthreads();
+  return sh_removed != 0;
 }
 
 // ctr[0]: redo count, ctr[2]: next vertex, ctr[4..5]: labels of L(u) read (u64).  gtile: per-workgroup arena slices
@@ -188,48 +250,12 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
       if (lane == 0 && mine) atomicAdd(&sh_heads, mine);
       __syncthreads();
       const uint32_t cap = tile_cap_for(sh_heads);
-      const bool in_lds = cap <= lds_slots;
-      uint32_t *keys = in_lds ? lds : gkeys;
-      V *vals = reinterpret_cast<V *>(keys + cap);
-      for (uint32_t i = tid; i < cap; i += kTileThreads) keys[i] = kInf32;
-      __syncthreads();
-      // one thread per group: claim the hub's slot, write its per-hop prefix minima
-      for (uint32_t i = tid; i < len; i += kTileThreads) {
-        const uint32_t w = label_hub(Lv[i]);
-        if (i > 0 && label_hub(Lv[i - 1]) == w) continue;
-        uint32_t slot = hash_slot(w, cap);
-        while (atomicCAS(&keys[slot], kInf32, w) != kInf32) slot = (slot + 1 == cap) ? 0 : slot + 1;
-        uint32_t m[kMaxK + 1];
-#pragma unroll
-        for (uint32_t x = 0; x <= kMaxK; ++x) m[x] = kInf32;
-        for (uint32_t q = i; q < len && label_hub(Lv[q]) == w; ++q) {
-          const uint32_t h = label_hop(Lv[q]), d = label_dist(Lv[q]);
-#pragma unroll
-          for (uint32_t x = 0; x <= kMaxK; ++x)
-            if (x == h) m[x] = min(m[x], d);
-        }
-        uint32_t run = kInf32;
-#pragma unroll
-        for (uint32_t x = 0; x <= kMaxK; ++x) {
-          run = min(run, m[x]);
-          if (x < K1) vals[slot * K1 + x] = (V)min(run, (uint32_t)TileVal<V>::kEmpty);
-        }
-      }
-      // a global tile was written through L2: drop stale L1 lines before reading
-      if (!in_lds) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __syncthreads();
-      if (!in_lds) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      for (;;) {
-        uint32_t p0 = 0;
-        if (lane == 0) p0 = atomicAdd(&sh_win, win);
-        p0 = __builtin_amdgcn_readfirstlane(__shfl(p0, 0, kWave));
-        if (p0 >= len) break;
-        if (window_removes<V>(v, K1, off, lab, Lv, len, p0, win, keys, vals, cap, streamed)) {
-          if (lane == 0) sh_removed = 1;
-          break;
-        }
-      }
-      __syncthreads();
+      if (cap <= lds_slots)
+        clean_vertex<V>(v, K1, off, lab, Lv, len, (lds_ptr<uint32_t>)lds,
+                        (lds_ptr<V>)(lds + cap), cap, false, win, sh_win, sh_removed, streamed);
+      else
+        clean_vertex<V>(v, K1, off, lab, Lv, len, (glb_ptr<uint32_t>)gkeys,
+                        (glb_ptr<V>)(gkeys + cap), cap, true, win, sh_win, sh_removed, streamed);
     }
     if (sh_removed) {
       if (tid == 0) redo[atomicAdd(&ctr[0], 1u)] = v;

@@ -77,9 +77,18 @@ __device__ __host__ inline uint64_t tile_words(uint32_t cap, uint32_t K1) {
   return (uint64_t)cap + ((uint64_t)cap * K1 * sizeof(V) + 3) / 4;
 }
 
+// Pointers in a named address space.  A tile read through a generic pointer
+// compiles to flat loads, and the wait for a flat load also waits for every
+// global load in flight (the list loads a wave issues ahead); tiles are read
+// through one of these instead, so LDS tiles use ds_read and arena tiles
+// global loads.
+template <class T> using lds_ptr = __attribute__((address_space(3))) T *;
+template <class T> using glb_ptr = __attribute__((address_space(1))) T *;
+
 // Value for (hub, hop budget x), or the empty value when the hub is absent.
-template <class V>
-__device__ inline uint32_t tile_lookup(const uint32_t *keys, const V *vals, uint32_t cap,
+// KP / VP: lds_ptr or glb_ptr of const uint32_t / const V.
+template <class V, class KP, class VP>
+__device__ inline uint32_t tile_lookup(KP keys, VP vals, uint32_t cap,
                                        uint32_t K1, uint32_t hub, uint32_t x) {
   uint32_t slot = hash_slot(hub, cap);
   for (;;) {

@@ -524,9 +524,12 @@ constexpr uint32_t kPruneThreads = 1024;
 // minimum distance).
 
 // LDS atomicMin on 16-bit values: CAS on the containing word.
+// The word is reached by pointer arithmetic on p, not through an integer, so
+// an LDS pointer stays an LDS pointer (ds_cmpst, not a flat atomic).
 __device__ inline void atomic_min_v(uint16_t *p, uint32_t v) {
-  uint32_t *w = reinterpret_cast<uint32_t *>(reinterpret_cast<uintptr_t>(p) & ~(uintptr_t)3);
-  const uint32_t sh = (reinterpret_cast<uintptr_t>(p) & 2) ? 16 : 0;
+  const uint32_t odd = (uint32_t)(reinterpret_cast<uintptr_t>(p) >> 1) & 1u;
+  uint32_t *w = reinterpret_cast<uint32_t *>(p - odd);
+  const uint32_t sh = odd ? 16 : 0;
   uint32_t old = *w;
   for (;;) {
     const uint32_t cur = (old >> sh) & 0xFFFFu;
@@ -566,31 +569,40 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals
 // receives the number of labels of L(y) examined (algorithmic: up to the
 // 128-label step that holds the witness).  L(y) is streamed 128 labels per
 // step (two per lane) with the next step's loads already in flight, so a wave
-// waits about one memory round trip per 128 labels, not per 64.
-template <class V>
+// waits about one memory round trip per 128 labels, not per 64.  The tile is
+// read through address-space pointers (KP / VP, device.hpp), so a probe's
+// wait does not also wait for the next step's loads.
+template <class V, class KP, class VP>
 __device__ inline bool probe_label(const GenArgs &a, unsigned long long l, bool valid,
-                                   uint32_t dy, const uint32_t *keys, const V *vals,
-                                   uint32_t cap, uint32_t K1) {
+                                   uint32_t dy, KP keys, VP vals, uint32_t cap, uint32_t K1) {
   const uint32_t ha = label_hop(l);
   if (!valid || ha > a.h) return false;
   const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, label_hub(l), a.h - ha);
   return v != TileVal<V>::kEmpty && label_dist(l) + v <= dy;
 }
 
-template <class V>
-__device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const uint32_t *keys,
-                               const V *vals, uint32_t cap, uint32_t &reads) {
+template <class V, class KP, class VP>
+__device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, KP keys, VP vals,
+                               uint32_t cap, uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
   const unsigned long long *L = a.pool + a.seg_start[y];
   const uint32_t len = a.seg_len[y];
+  if (len == 0) {
+    reads = 0;
+    return false;
+  }
+  // Loads are clamped to the list (a lane past the end reloads the last
+  // label and is masked in the probe), so no load sits behind a branch, and
+  // the two step buffers alternate (a, b) instead of being copied: the wait
+  // before a step's probes then leaves the next step's loads in flight.
+  const uint32_t last = len - 1;
+  auto ld = [&](uint32_t j) { return L[min(j, last)]; };
   constexpr uint32_t kStep = 2 * kWave;
   // witnesses are mostly top-ranked hubs near the front of L(y): the first
   // two steps take 64 labels each (one step in flight), later steps 128
-  unsigned long long c0 = 0, c1 = 0;
-  if (lane < len) c0 = L[lane];
-  if (kWave + lane < len) c1 = L[kWave + lane];
-  if (__ballot(probe_label<V>(a, c0, lane < len, dy, keys, vals, cap, K1))) {
+  unsigned long long a0 = ld(lane), a1 = ld(kWave + lane);
+  if (__ballot(probe_label<V>(a, a0, lane < len, dy, keys, vals, cap, K1))) {
     reads = min(len, kWave);
     return true;
   }
@@ -598,30 +610,28 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, const 
     reads = len;
     return false;
   }
-  {
-    unsigned long long n0 = 0, n1 = 0;
-    if (kStep + lane < len) n0 = L[kStep + lane];
-    if (kStep + kWave + lane < len) n1 = L[kStep + kWave + lane];
-    if (__ballot(probe_label<V>(a, c1, kWave + lane < len, dy, keys, vals, cap, K1))) {
-      reads = min(len, kStep);
-      return true;
-    }
-    c0 = n0;
-    c1 = n1;
+  unsigned long long b0 = ld(kStep + lane), b1 = ld(kStep + kWave + lane);
+  if (__ballot(probe_label<V>(a, a1, kWave + lane < len, dy, keys, vals, cap, K1))) {
+    reads = min(len, kStep);
+    return true;
   }
-  for (uint32_t j = kStep; j < len; j += kStep) {
-    const uint32_t jn = j + kStep;
-    unsigned long long n0 = 0, n1 = 0;
-    if (jn + lane < len) n0 = L[jn + lane];
-    if (jn + kWave + lane < len) n1 = L[jn + kWave + lane];
-    const bool f = probe_label<V>(a, c0, j + lane < len, dy, keys, vals, cap, K1) ||
-                   probe_label<V>(a, c1, j + kWave + lane < len, dy, keys, vals, cap, K1);
-    if (__ballot(f)) {
-      reads = min(len, jn);
+  for (uint32_t j = kStep; j < len; j += 2 * kStep) {
+    // b holds labels [j, j + kStep); a is loaded with [j + kStep, j + 2 kStep)
+    a0 = ld(j + kStep + lane);
+    a1 = ld(j + kStep + kWave + lane);
+    if (__ballot(probe_label<V>(a, b0, j + lane < len, dy, keys, vals, cap, K1) ||
+                 probe_label<V>(a, b1, j + kWave + lane < len, dy, keys, vals, cap, K1))) {
+      reads = min(len, j + kStep);
       return true;
     }
-    c0 = n0;
-    c1 = n1;
+    if (j + kStep >= len) break;
+    b0 = ld(j + 2 * kStep + lane);
+    b1 = ld(j + 2 * kStep + kWave + lane);
+    if (__ballot(probe_label<V>(a, a0, j + kStep + lane < len, dy, keys, vals, cap, K1) ||
+                 probe_label<V>(a, a1, j + kStep + kWave + lane < len, dy, keys, vals, cap, K1))) {
+      reads = min(len, j + 2 * kStep);
+      return true;
+    }
   }
   reads = len;
   return false;
@@ -664,6 +674,36 @@ __device__ inline void publish_accepted(const GenArgs &a, Item *Fn, Item *sh_acc
     sh_touch_n = 0;
   }
   __syncthreads();
+}
+
+// The waves of a prune workgroup take the item's candidates [c_beg, c_end) of
+// hub slot b one at a time (L(y) lengths vary widely) and stage the accepted
+// ones in LDS.  Instantiated once for an LDS tile and once for an arena tile.
+template <class V, class KP, class VP>
+__device__ __attribute__((always_inline)) inline void prune_candidates(
+    const GenArgs &a, uint32_t b, uint32_t c_beg, uint32_t c_end, const uint32_t *cands, KP keys,
+    VP vals, uint32_t cap, uint32_t &sh_next, uint32_t &sh_acc_n, Item *sh_acc,
+    unsigned long long &wave_reads) {
+  const int lane = lane_id();
+  for (;;) {
+    uint32_t c = 0;
+    if (lane == 0) c = c_beg + atomicAdd(&sh_next, 1u);
+    c = __shfl(c, 0, kWave);
+    if (c >= c_end) break;
+    const uint32_t y = cands[c];
+    const uint64_t slot = (uint64_t)b * a.n + y;
+    const uint32_t dy = slot_dist(a, a.cand_d[slot]);
+    uint32_t reads = 0;
+    const bool pruned = witness_exists<V>(a, y, dy, keys, vals, cap, reads);
+    wave_reads += reads;
+    if (lane == 0) {
+      a.cand_d[slot] = slot_stale(a, dy);  // later rounds must undercut dy
+      if (!pruned) {
+        const uint32_t p = atomicAdd(&sh_acc_n, 1u);
+        sh_acc[p] = Item{b, y, dy};
+      }
+    }
+  }
 }
 
 template <class V>
@@ -732,17 +772,15 @@ k_prune(GenArgs a) {
       publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
     const uint32_t *cands = a.cgroup + a.hub_off[b];
     const uint32_t cap = tile_cap_for(a.seg_len[s]);
-    uint32_t *keys;
-    V *vals;
-    const bool in_lds = cap <= a.lds_slots;
-    if (in_lds) {
-      keys = lds;
-      vals = reinterpret_cast<V *>(lds + cap);
+    if (cap <= a.lds_slots) {
       if (tile_b != b) {
-        build_tile<V>(a, s, keys, vals, cap);
+        build_tile<V>(a, s, lds, reinterpret_cast<V *>(lds + cap), cap);
         tile_b = b;
         tile_reads += a.seg_len[s];
       }
+      prune_candidates<V>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds,
+                          (lds_ptr<const V>)(lds + cap), cap, sh_next, sh_acc_n, sh_acc,
+                          wave_reads);
     } else {
       if (tile_words<V>(cap, K1) > gwords) {
         // arena too small: the host rolls the batch back, grows the arena
@@ -751,34 +789,15 @@ k_prune(GenArgs a) {
         __syncthreads();
         continue;
       }
-      keys = gkeys;
-      vals = reinterpret_cast<V *>(gkeys + cap);
-      build_tile<V>(a, s, keys, vals, cap);
+      build_tile<V>(a, s, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
       // the tile was written through L2 (atomics); drop stale L1 lines of
       // this workgroup's arena slice before plain loads read it
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tile_b = kInf32;
       tile_reads += a.seg_len[s];
-    }
-    // waves take the item's candidates one at a time (L(y) lengths vary widely)
-    for (;;) {
-      uint32_t c = 0;
-      if (lane == 0) c = c_beg + atomicAdd(&sh_next, 1u);
-      c = __shfl(c, 0, kWave);
-      if (c >= c_end) break;
-      const uint32_t y = cands[c];
-      const uint64_t slot = (uint64_t)b * a.n + y;
-      const uint32_t dy = slot_dist(a, a.cand_d[slot]);
-      uint32_t reads = 0;
-      const bool pruned = witness_exists<V>(a, y, dy, keys, vals, cap, reads);
-      wave_reads += reads;
-      if (lane == 0) {
-        a.cand_d[slot] = slot_stale(a, dy);  // later rounds must undercut dy
-        if (!pruned) {
-          const uint32_t p = atomicAdd(&sh_acc_n, 1u);
-          sh_acc[p] = Item{b, y, dy};
-        }
-      }
+      prune_candidates<V>(a, b, c_beg, c_end, cands, (glb_ptr<const uint32_t>)gkeys,
+                          (glb_ptr<const V>)(gkeys + cap), cap, sh_next, sh_acc_n, sh_acc,
+                          wave_reads);
     }
     __syncthreads();
   }
