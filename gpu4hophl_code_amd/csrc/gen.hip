@@ -581,28 +581,39 @@ __device__ inline bool probe_label(const GenArgs &a, unsigned long long l, bool 
   return v != TileVal<V>::kEmpty && label_dist(l) + v <= dy;
 }
 
+// Labels [0, 64) and [64, 128) of a list of `len` labels for this lane,
+// clamped to the list (a lane past the end reloads the last label and is
+// masked in the probe); an empty list reads its first slot, which lies in the
+// pool.
+__device__ inline void first_step(const unsigned long long *L, uint32_t len,
+                                  unsigned long long &f0, unsigned long long &f1) {
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint32_t last = len ? len - 1 : 0;
+  f0 = L[min(lane, last)];
+  f1 = L[min(kWave + lane, last)];
+}
+
+// Witness search over L(y) = L[0, len): true when some common hub gives
+// da + db <= dy; f0 / f1 hold the list's first step, loaded by the caller
+// ahead of time (first_step).  No load sits behind a branch and the two step
+// buffers alternate (a, b) instead of being copied, so the wait before a
+// step's probes leaves the next step's loads in flight.
 template <class V, class KP, class VP>
-__device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, KP keys, VP vals,
-                               uint32_t cap, uint32_t &reads) {
+__device__ bool witness_search(const GenArgs &a, const unsigned long long *L, uint32_t len,
+                               unsigned long long f0, unsigned long long f1, uint32_t dy,
+                               KP keys, VP vals, uint32_t cap, uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
-  const unsigned long long *L = a.pool + a.seg_start[y];
-  const uint32_t len = a.seg_len[y];
   if (len == 0) {
     reads = 0;
     return false;
   }
-  // Loads are clamped to the list (a lane past the end reloads the last
-  // label and is masked in the probe), so no load sits behind a branch, and
-  // the two step buffers alternate (a, b) instead of being copied: the wait
-  // before a step's probes then leaves the next step's loads in flight.
   const uint32_t last = len - 1;
   auto ld = [&](uint32_t j) { return L[min(j, last)]; };
   constexpr uint32_t kStep = 2 * kWave;
   // witnesses are mostly top-ranked hubs near the front of L(y): the first
   // two steps take 64 labels each (one step in flight), later steps 128
-  unsigned long long a0 = ld(lane), a1 = ld(kWave + lane);
-  if (__ballot(probe_label<V>(a, a0, lane < len, dy, keys, vals, cap, K1))) {
+  if (__ballot(probe_label<V>(a, f0, lane < len, dy, keys, vals, cap, K1))) {
     reads = min(len, kWave);
     return true;
   }
@@ -611,14 +622,13 @@ __device__ bool witness_exists(const GenArgs &a, uint32_t y, uint32_t dy, KP key
     return false;
   }
   unsigned long long b0 = ld(kStep + lane), b1 = ld(kStep + kWave + lane);
-  if (__ballot(probe_label<V>(a, a1, kWave + lane < len, dy, keys, vals, cap, K1))) {
+  if (__ballot(probe_label<V>(a, f1, kWave + lane < len, dy, keys, vals, cap, K1))) {
     reads = min(len, kStep);
     return true;
   }
   for (uint32_t j = kStep; j < len; j += 2 * kStep) {
     // b holds labels [j, j + kStep); a is loaded with [j + kStep, j + 2 kStep)
-    a0 = ld(j + kStep + lane);
-    a1 = ld(j + kStep + kWave + lane);
+    const unsigned long long a0 = ld(j + kStep + lane), a1 = ld(j + kStep + kWave + lane);
     if (__ballot(probe_label<V>(a, b0, j + lane < len, dy, keys, vals, cap, K1) ||
                  probe_label<V>(a, b1, j + kWave + lane < len, dy, keys, vals, cap, K1))) {
       reads = min(len, j + kStep);
@@ -676,32 +686,73 @@ __device__ inline void publish_accepted(const GenArgs &a, Item *Fn, Item *sh_acc
   __syncthreads();
 }
 
+// Candidates a prune wave claims at once, at most, and the divisor that sizes
+// a claim from the item's unclaimed candidates (claims shrink toward the
+// item's end, where the slowest wave decides when the workgroup moves on).
+constexpr uint32_t kClaimMax = 4;
+constexpr uint32_t kClaimSpread = 2 * (kPruneThreads / kWave);
+
 // The waves of a prune workgroup take the item's candidates [c_beg, c_end) of
-// hub slot b one at a time (L(y) lengths vary widely) and stage the accepted
-// ones in LDS.  Instantiated once for an LDS tile and once for an arena tile.
+// hub slot b (L(y) lengths vary widely, so dynamically) and stage the
+// accepted ones in LDS.  A wave claims up to kClaimMax candidates at once and
+// loads their vertices, then their slot distances and list extents, in one
+// round trip per level (lane i: claimed candidate i); the first step of the
+// next claimed candidate's list is issued before the current one's probes.
+// Instantiated once for an LDS tile and once for an arena tile.
 template <class V, class KP, class VP>
 __device__ __attribute__((always_inline)) inline void prune_candidates(
     const GenArgs &a, uint32_t b, uint32_t c_beg, uint32_t c_end, const uint32_t *cands, KP keys,
     VP vals, uint32_t cap, uint32_t &sh_next, uint32_t &sh_acc_n, Item *sh_acc,
     unsigned long long &wave_reads) {
-  const int lane = lane_id();
+  const uint32_t lane = (uint32_t)lane_id();
   for (;;) {
-    uint32_t c = 0;
-    if (lane == 0) c = c_beg + atomicAdd(&sh_next, 1u);
-    c = __shfl(c, 0, kWave);
-    if (c >= c_end) break;
-    const uint32_t y = cands[c];
-    const uint64_t slot = (uint64_t)b * a.n + y;
-    const uint32_t dy = slot_dist(a, a.cand_d[slot]);
-    uint32_t reads = 0;
-    const bool pruned = witness_exists<V>(a, y, dy, keys, vals, cap, reads);
-    wave_reads += reads;
+    uint32_t c = 0, k = 0;
     if (lane == 0) {
-      a.cand_d[slot] = slot_stale(a, dy);  // later rounds must undercut dy
-      if (!pruned) {
-        const uint32_t p = atomicAdd(&sh_acc_n, 1u);
-        sh_acc[p] = Item{b, y, dy};
+      const uint32_t seen =
+          __hip_atomic_load(&sh_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint32_t rem = c_end - min(c_end, c_beg + seen);
+      k = min(kClaimMax, max(1u, rem / kClaimSpread));
+      c = c_beg + atomicAdd(&sh_next, k);
+    }
+    c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, kWave));
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
+    if (c >= c_end) break;
+    const uint32_t cn = min(k, c_end - c);
+    uint32_t y = 0, dy = 0, len = 0;
+    unsigned long long st = 0;
+    if (lane < cn) {
+      y = cands[c + lane];
+      dy = slot_dist(a, a.cand_d[(uint64_t)b * a.n + y]);
+      st = a.seg_start[y];
+      len = a.seg_len[y];
+    }
+    auto start_of = [&](uint32_t i) {
+      const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)st, i);
+      const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(st >> 32), i);
+      return a.pool + (((unsigned long long)hi << 32) | lo);
+    };
+    unsigned long long f0, f1;
+    first_step(start_of(0), __builtin_amdgcn_readlane(len, 0), f0, f1);
+    for (uint32_t i = 0; i < cn; ++i) {
+      // next candidate's first step (the last one reloads its own, so the
+      // loads are unconditional)
+      const uint32_t nx = min(i + 1, cn - 1);
+      unsigned long long g0, g1;
+      first_step(start_of(nx), __builtin_amdgcn_readlane(len, nx), g0, g1);
+      const uint32_t yi = __builtin_amdgcn_readlane(y, i), di = __builtin_amdgcn_readlane(dy, i);
+      uint32_t reads = 0;
+      const bool pruned = witness_search<V>(a, start_of(i), __builtin_amdgcn_readlane(len, i), f0,
+                                            f1, di, keys, vals, cap, reads);
+      wave_reads += reads;
+      if (lane == 0) {
+        a.cand_d[(uint64_t)b * a.n + yi] = slot_stale(a, di);  // later rounds must undercut di
+        if (!pruned) {
+          const uint32_t p = atomicAdd(&sh_acc_n, 1u);
+          sh_acc[p] = Item{b, yi, di};
+        }
       }
+      f0 = g0;
+      f1 = g1;
     }
   }
 }
