@@ -250,10 +250,11 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
       if (lane == 0 && mine) atomicAdd(&sh_heads, mine);
       __syncthreads();
       const uint32_t cap = tile_cap_for(sh_heads);
-      if (cap <= lds_slots)
+      if (cap <= lds_slots) {
+        const uint32_t lcap = lds_tile_cap(sh_heads, lds_slots);
         clean_vertex<V>(v, K1, off, lab, Lv, len, (lds_ptr<uint32_t>)lds,
-                        (lds_ptr<V>)(lds + cap), cap, false, win, sh_win, sh_removed, streamed);
-      else
+                        (lds_ptr<V>)(lds + lcap), lcap, false, win, sh_win, sh_removed, streamed);
+      } else
         clean_vertex<V>(v, K1, off, lab, Lv, len, (glb_ptr<uint32_t>)gkeys,
                         (glb_ptr<V>)(gkeys + cap), cap, true, win, sh_win, sh_removed, streamed);
     }

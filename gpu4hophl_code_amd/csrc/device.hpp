@@ -71,6 +71,15 @@ __host__ __device__ inline uint32_t tile_cap_for(uint32_t len) {
   return len + (len >> 1) + 1;  // load factor <= 2/3
 }
 
+// Capacity of a tile that fits in `lds_slots` LDS slots: the load factor is
+// lowered to 1/4 where the slots allow (a lookup of an absent hub then ends
+// after about 1.4 probes on average instead of 5, and a wave waits for its
+// longest lane).
+__host__ __device__ inline uint32_t lds_tile_cap(uint32_t len, uint32_t lds_slots) {
+  const unsigned long long roomy = 4ull * len + 1;
+  return (uint32_t)(roomy < lds_slots ? roomy : lds_slots);
+}
+
 // u32 words a tile of `cap` slots takes (keys + values, value rows padded)
 template <class V>
 __device__ __host__ inline uint64_t tile_words(uint32_t cap, uint32_t K1) {

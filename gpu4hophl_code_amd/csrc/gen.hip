@@ -824,13 +824,14 @@ k_prune(GenArgs a) {
     const uint32_t *cands = a.cgroup + a.hub_off[b];
     const uint32_t cap = tile_cap_for(a.seg_len[s]);
     if (cap <= a.lds_slots) {
+      const uint32_t lcap = lds_tile_cap(a.seg_len[s], a.lds_slots);
       if (tile_b != b) {
-        build_tile<V>(a, s, lds, reinterpret_cast<V *>(lds + cap), cap);
+        build_tile<V>(a, s, lds, reinterpret_cast<V *>(lds + lcap), lcap);
         tile_b = b;
         tile_reads += a.seg_len[s];
       }
       prune_candidates<V>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds,
-                          (lds_ptr<const V>)(lds + cap), cap, sh_next, sh_acc_n, sh_acc,
+                          (lds_ptr<const V>)(lds + lcap), lcap, sh_next, sh_acc_n, sh_acc,
                           wave_reads);
     } else {
       if (tile_words<V>(cap, K1) > gwords) {
