@@ -72,11 +72,11 @@ __host__ __device__ inline uint32_t tile_cap_for(uint32_t len) {
 }
 
 // Capacity of a tile that fits in `lds_slots` LDS slots: the load factor is
-// lowered to 1/4 where the slots allow (a lookup of an absent hub then ends
-// after about 1.4 probes on average instead of 5, and a wave waits for its
-// longest lane).
+// lowered to 1/8 where the slots allow (a lookup of an absent hub then ends
+// after about 1.1 probes on average instead of 5 at 2/3, and a wave waits for
+// its longest lane).
 __host__ __device__ inline uint32_t lds_tile_cap(uint32_t len, uint32_t lds_slots) {
-  const unsigned long long roomy = 4ull * len + 1;
+  const unsigned long long roomy = 8ull * len + 1;
   return (uint32_t)(roomy < lds_slots ? roomy : lds_slots);
 }
 
