@@ -268,8 +268,15 @@ __device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uin
     const uint2 ed = a.adj[e];
     y = ed.x;
     const uint32_t nv = slot_fresh(a, kd + ed.y);
-    const uint32_t old = atomicMin(&a.cand_d[(uint64_t)kb * a.n + y], nv);
-    first = nv < old && slot_opens(a, old);
+    uint32_t *slot = &a.cand_d[(uint64_t)kb * a.n + y];
+    // A slot only falls while the expansion runs (atomicMin; clears happen
+    // between batches), so a read at or below nv, even a stale one, means the
+    // atomic would neither change the slot nor open it: most relaxations end
+    // with this read, which costs less than a returning atomic.
+    if (nv < *slot) {
+      const uint32_t old = atomicMin(slot, nv);
+      first = nv < old && slot_opens(a, old);
+    }
   }
   const unsigned long long firsts = __ballot(first);
   if (firsts) {
