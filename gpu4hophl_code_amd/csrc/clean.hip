@@ -48,7 +48,7 @@ constexpr uint32_t kSeqWordsPerWave = kSeqBitsPerWave / 32;
 constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vertex
 constexpr uint32_t kTileWindow = 4;            // labels of L(v) per work unit (default)
 constexpr uint32_t kMaxWindow = 64;            // largest window
-constexpr uint32_t kTileLdsBytes = 60 * 1024;  // LDS tile budget (two workgroups per CU)
+constexpr uint32_t kTileLdsBytes = 64 * 1024 + 16;  // LDS tile budget (two workgroups per CU)
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -71,12 +71,12 @@ __device__ inline uint32_t lb_hub(const unsigned long long *lab, uint32_t lo, ui
 template <class KP>
 __device__ inline uint32_t tile_slot(KP keys, uint32_t cap, uint32_t hub) {
   uint32_t slot = hash_slot(hub, cap);
-  for (;;) {
-    const uint32_t k = keys[slot];
-    if (k == hub) return slot;
-    if (k == kInf32) return kInf32;
-    slot = (slot + 1 == cap) ? 0 : slot + 1;
+  uint32_t k = keys[slot];
+  while (k != hub && k != kInf32) {
+    slot = (slot + 1) & (cap - 1);
+    k = keys[slot];
   }
+  return k == hub ? slot : kInf32;
 }
 
 // True when the all-earlier predicate removes a label of a group whose head
@@ -125,7 +125,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
               const unsigned long long g = Lv[p + t];
               const uint32_t hg = label_hop(g);
               if (hg > h1) {
-                const uint32_t m = vals[slot * K1 + (hg - h1)];
+                const uint32_t m = vals[mul_u24(slot, K1) + (hg - h1)];
                 if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
               }
             }
@@ -176,8 +176,7 @@ __device__ __attribute__((always_inline)) inline bool clean_vertex(
     const uint32_t w = label_hub(Lv[i]);
     if (i > 0 && label_hub(Lv[i - 1]) == w) continue;
     uint32_t slot = hash_slot(w, cap);
-    while (atomicCAS((uint32_t *)&keys[slot], kInf32, w) != kInf32)
-      slot = (slot + 1 == cap) ? 0 : slot + 1;
+    while (atomicCAS((uint32_t *)&keys[slot], kInf32, w) != kInf32) slot = (slot + 1) & (cap - 1);
     uint32_t m[kMaxK + 1];
 #pragma unroll
     for (uint32_t x = 0; x <= kMaxK; ++x) m[x] = kInf32;

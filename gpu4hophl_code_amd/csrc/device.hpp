@@ -57,18 +57,39 @@ inline uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t h
 // open-addressing hash with keys[c] = hub and vals[c*(K+1)+x] = min dist over
 // the list's labels (hub, hb, db) with hb <= x.
 // V is u16 when every label distance fits (K * max weight < 65535), halving
-// the tile; the empty value is V's maximum.  Capacities need not be powers of
-// two: the home slot is the high word of (mixed key) * cap.
+// the tile; the empty value is V's maximum.  Capacities are powers of two.
 template <class V> struct TileVal;
 template <> struct TileVal<uint16_t> { static constexpr uint32_t kEmpty = 0xFFFFu; };
 template <> struct TileVal<uint32_t> { static constexpr uint32_t kEmpty = 0xFFFFFFFFu; };
 
-__device__ inline uint32_t hash_slot(uint32_t key, uint32_t cap) {
-  return (uint32_t)(((unsigned long long)(key * 0x9E3779B1u) * cap) >> 32);
+__host__ __device__ inline uint32_t pow2_at_least(uint64_t x) {
+  uint32_t p = 2;
+  while (p < x) p <<= 1;
+  return p;
+}
+__host__ __device__ inline uint32_t pow2_at_most(uint32_t x) {
+  uint32_t p = 2;
+  while ((uint64_t)p * 2 <= x) p <<= 1;
+  return p;
 }
 
+// Fibonacci hashing: the top log2(cap) bits of key * G mod 2^32, with G =
+// 2^24 / golden ratio (odd), so consecutive hub ids land about 8 / 1.618
+// slots apart per 2^11 slots and never cluster.  The product is a 24-bit
+// multiply of the key's low 24 bits (full rate, where a 32-bit multiply runs
+// at a quarter); keys that differ only above bit 23 share a home slot, which
+// costs probes, never correctness.
+// (HIP's __umul24 returns int: the product is taken as unsigned before the
+// shift, so the shift is logical and the slot stays below cap.)
+constexpr uint32_t kGold24 = 0x9E3779u;
+__device__ inline uint32_t mul_u24(uint32_t x, uint32_t y) { return (uint32_t)__umul24(x, y); }
+__device__ inline uint32_t hash_slot(uint32_t key, uint32_t cap) {
+  return mul_u24(key, kGold24) >> (__builtin_clz(cap) + 1);
+}
+
+// Tile capacity for `len` distinct hubs: a power of two, load factor <= 2/3.
 __host__ __device__ inline uint32_t tile_cap_for(uint32_t len) {
-  return len + (len >> 1) + 1;  // load factor <= 2/3
+  return pow2_at_least((uint64_t)len + (len >> 1) + 1);
 }
 
 // Capacity of a tile that fits in `lds_slots` LDS slots: the load factor is
@@ -76,8 +97,9 @@ __host__ __device__ inline uint32_t tile_cap_for(uint32_t len) {
 // after about 1.1 probes on average instead of 5 at 2/3, and a wave waits for
 // its longest lane).
 __host__ __device__ inline uint32_t lds_tile_cap(uint32_t len, uint32_t lds_slots) {
-  const unsigned long long roomy = 8ull * len + 1;
-  return (uint32_t)(roomy < lds_slots ? roomy : lds_slots);
+  const uint32_t roomy = pow2_at_least(8ull * len + 1);
+  const uint32_t most = pow2_at_most(lds_slots);
+  return roomy < most ? roomy : most;
 }
 
 // u32 words a tile of `cap` slots takes (keys + values, value rows padded)
@@ -96,16 +118,21 @@ template <class T> using glb_ptr = __attribute__((address_space(1))) T *;
 
 // Value for (hub, hop budget x), or the empty value when the hub is absent.
 // KP / VP: lds_ptr or glb_ptr of const uint32_t / const V.
+// The first probe is straight-line code; the loop runs only for lanes whose
+// home slot holds another hub.  The value read is unconditional (slot < cap)
+// and selected afterwards.
 template <class V, class KP, class VP>
 __device__ inline uint32_t tile_lookup(KP keys, VP vals, uint32_t cap,
                                        uint32_t K1, uint32_t hub, uint32_t x) {
+  const uint32_t mask = cap - 1;
   uint32_t slot = hash_slot(hub, cap);
-  for (;;) {
-    const uint32_t k = keys[slot];
-    if (k == hub) return vals[slot * K1 + x];
-    if (k == kInf32) return TileVal<V>::kEmpty;
-    slot = (slot + 1 == cap) ? 0 : slot + 1;
+  uint32_t k = keys[slot];
+  while (k != hub && k != kInf32) {
+    slot = (slot + 1) & mask;
+    k = keys[slot];
   }
+  const uint32_t v = vals[mul_u24(slot, K1) + x];
+  return k == hub ? v : TileVal<V>::kEmpty;
 }
 
 // Device buffer owned by RAII.

@@ -564,7 +564,7 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
       if (prev == kInf32 || prev == hub) break;
-      slot = (slot + 1 == cap) ? 0 : slot + 1;
+      slot = (slot + 1) & (cap - 1);
     }
     for (uint32_t x = hb; x < K1; ++x) atomic_min_v(&vals[slot * K1 + x], db);
   }
@@ -1241,7 +1241,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
   const uint32_t K1 = K + 1;
   g.small_vals = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1) < 0xFFFFull;
-  const uint64_t lds_words = 60 * 1024 / 4;  // + ~3 KB of static LDS
+  const uint64_t lds_words = 64 * 1024 / 4;  // + ~4.5 KB of static LDS; two workgroups per CU
   uint32_t lds_slots = 64;
   while ((g.small_vals ? tile_words<uint16_t>(lds_slots + 64, K1)
                        : tile_words<uint32_t>(lds_slots + 64, K1)) <= lds_words)
