@@ -588,27 +588,29 @@ __device__ inline bool probe_label(const GenArgs &a, unsigned long long l, bool 
   return v != TileVal<V>::kEmpty && label_dist(l) + v <= dy;
 }
 
-// Labels [0, 64) and [64, 128) of a list of `len` labels for this lane,
-// clamped to the list (a lane past the end reloads the last label and is
-// masked in the probe); an empty list reads its first slot, which lies in the
-// pool.
-__device__ inline void first_step(const unsigned long long *L, uint32_t len,
-                                  unsigned long long &f0, unsigned long long &f1) {
+// Label `lane` of a list of `len` labels, clamped to the list (a lane past
+// the end reloads the last label and is masked in the probe); an empty list
+// reads its first slot, which lies in the pool.
+__device__ inline unsigned long long first_step(const unsigned long long *L, uint32_t len) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t last = len ? len - 1 : 0;
-  f0 = L[min(lane, last)];
-  f1 = L[min(kWave + lane, last)];
+  return L[min(lane, last)];
 }
 
 // Witness search over L(y) = L[0, len): true when some common hub gives
-// da + db <= dy; f0 / f1 hold the list's first step, loaded by the caller
-// ahead of time (first_step).  No load sits behind a branch and the two step
-// buffers alternate (a, b) instead of being copied, so the wait before a
-// step's probes leaves the next step's loads in flight.
+// da + db <= dy; f0 holds the list's first 64 labels, loaded by the caller
+// ahead of time (first_step).  The prune moves HBM bytes near the rate the
+// memory sustains, so loads are speculative only where they pay: witnesses
+// mostly sit among the first, top-ranked hubs, so labels [64, 192) are
+// loaded only once the first 64 hold none, and only lists still unresolved
+// after 192 labels (mostly candidates that will be accepted and read to the
+// end) keep the next 128-label step in flight while one is probed.  No load
+// sits behind a branch and the step buffers alternate (a, b) instead of
+// being copied, so a wait leaves the step in flight alone.
 template <class V, class KP, class VP>
 __device__ bool witness_search(const GenArgs &a, const unsigned long long *L, uint32_t len,
-                               unsigned long long f0, unsigned long long f1, uint32_t dy,
-                               KP keys, VP vals, uint32_t cap, uint32_t &reads) {
+                               unsigned long long f0, uint32_t dy, KP keys, VP vals,
+                               uint32_t cap, uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
   if (len == 0) {
@@ -617,9 +619,12 @@ __device__ bool witness_search(const GenArgs &a, const unsigned long long *L, ui
   }
   const uint32_t last = len - 1;
   auto ld = [&](uint32_t j) { return L[min(j, last)]; };
+  auto probe2 = [&](unsigned long long l0, unsigned long long l1, uint32_t j) {
+    const bool h0 = probe_label<V>(a, l0, j + lane < len, dy, keys, vals, cap, K1);
+    const bool h1 = probe_label<V>(a, l1, j + kWave + lane < len, dy, keys, vals, cap, K1);
+    return __ballot(h0 || h1) != 0;
+  };
   constexpr uint32_t kStep = 2 * kWave;
-  // witnesses are mostly top-ranked hubs near the front of L(y): the first
-  // two steps take 64 labels each (one step in flight), later steps 128
   if (__ballot(probe_label<V>(a, f0, lane < len, dy, keys, vals, cap, K1))) {
     reads = min(len, kWave);
     return true;
@@ -628,24 +633,30 @@ __device__ bool witness_search(const GenArgs &a, const unsigned long long *L, ui
     reads = len;
     return false;
   }
-  unsigned long long b0 = ld(kStep + lane), b1 = ld(kStep + kWave + lane);
-  if (__ballot(probe_label<V>(a, f1, kWave + lane < len, dy, keys, vals, cap, K1))) {
-    reads = min(len, kStep);
-    return true;
+  {
+    const unsigned long long s0 = ld(kWave + lane), s1 = ld(kWave + kWave + lane);
+    if (probe2(s0, s1, kWave)) {
+      reads = min(len, kWave + kStep);
+      return true;
+    }
   }
-  for (uint32_t j = kStep; j < len; j += 2 * kStep) {
+  constexpr uint32_t kLong = kWave + kStep;  // 192
+  if (len <= kLong) {
+    reads = len;
+    return false;
+  }
+  unsigned long long b0 = ld(kLong + lane), b1 = ld(kLong + kWave + lane);
+  for (uint32_t j = kLong; j < len; j += 2 * kStep) {
     // b holds labels [j, j + kStep); a is loaded with [j + kStep, j + 2 kStep)
     const unsigned long long a0 = ld(j + kStep + lane), a1 = ld(j + kStep + kWave + lane);
-    if (__ballot(probe_label<V>(a, b0, j + lane < len, dy, keys, vals, cap, K1) ||
-                 probe_label<V>(a, b1, j + kWave + lane < len, dy, keys, vals, cap, K1))) {
+    if (probe2(b0, b1, j)) {
       reads = min(len, j + kStep);
       return true;
     }
     if (j + kStep >= len) break;
     b0 = ld(j + 2 * kStep + lane);
     b1 = ld(j + 2 * kStep + kWave + lane);
-    if (__ballot(probe_label<V>(a, a0, j + kStep + lane < len, dy, keys, vals, cap, K1) ||
-                 probe_label<V>(a, a1, j + kStep + kWave + lane < len, dy, keys, vals, cap, K1))) {
+    if (probe2(a0, a1, j + kStep)) {
       reads = min(len, j + 2 * kStep);
       return true;
     }
@@ -703,8 +714,8 @@ constexpr uint32_t kClaimSpread = 2 * (kPruneThreads / kWave);
 // hub slot b (L(y) lengths vary widely, so dynamically) and stage the
 // accepted ones in LDS.  A wave claims up to kClaimMax candidates at once and
 // loads their vertices, then their slot distances and list extents, in one
-// round trip per level (lane i: claimed candidate i); the first step of the
-// next claimed candidate's list is issued before the current one's probes.
+// round trip per level (lane i: claimed candidate i); the first 64 labels of
+// the next claimed candidate's list are loaded before the current one's probes.
 // Instantiated once for an LDS tile and once for an arena tile.
 template <class V, class KP, class VP>
 __device__ __attribute__((always_inline)) inline void prune_candidates(
@@ -738,18 +749,16 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
       const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(st >> 32), i);
       return a.pool + (((unsigned long long)hi << 32) | lo);
     };
-    unsigned long long f0, f1;
-    first_step(start_of(0), __builtin_amdgcn_readlane(len, 0), f0, f1);
+    unsigned long long f0 = first_step(start_of(0), __builtin_amdgcn_readlane(len, 0));
     for (uint32_t i = 0; i < cn; ++i) {
       // next candidate's first step (the last one reloads its own, so the
       // loads are unconditional)
       const uint32_t nx = min(i + 1, cn - 1);
-      unsigned long long g0, g1;
-      first_step(start_of(nx), __builtin_amdgcn_readlane(len, nx), g0, g1);
+      const unsigned long long g0 = first_step(start_of(nx), __builtin_amdgcn_readlane(len, nx));
       const uint32_t yi = __builtin_amdgcn_readlane(y, i), di = __builtin_amdgcn_readlane(dy, i);
       uint32_t reads = 0;
       const bool pruned = witness_search<V>(a, start_of(i), __builtin_amdgcn_readlane(len, i), f0,
-                                            f1, di, keys, vals, cap, reads);
+                                            di, keys, vals, cap, reads);
       wave_reads += reads;
       if (lane == 0) {
         a.cand_d[(uint64_t)b * a.n + yi] = slot_stale(a, di);  // later rounds must undercut di
@@ -759,7 +768,6 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
         }
       }
       f0 = g0;
-      f1 = g1;
     }
   }
 }
