@@ -248,15 +248,17 @@ def main():
 
     # rooflines of the two build kernels (HIP events on the build stream,
     # summed over their launches); the slower one is reported as `roofline`.
-    #   k_prune:  8 B per label streamed from L(y) or staged from L(s)
+    #   k_prune:  4 or 8 B per label streamed from L(y) (the stream's label
+    #             width, prune_label_bytes) + 8 B per label staged from L(s)
     #             + 8 B per candidate (its list entry and dense slot)
     #   k_expand: 28 B per frontier item (item + its two CSR offsets)
-    #             + 12 B per scanned edge ((col, w) + the 4 B slot atomic)
+    #             + 12 B per scanned edge ((col, w) + the 4 B dense slot,
+    #             read, and updated by an atomic when the edge lowers it)
     #             + 8 B per candidate appended
     launches = max(1, stats["prune_launches"])  # one expand and one prune per round
     kern = {
-        "k_prune": (8 * (stats["prune_label_reads"] + stats["prune_tile_reads"])
-                    + 8 * stats["candidates"], stats["t_prune_s"]),
+        "k_prune": ((stats.get("prune_label_bytes") or 8) * stats["prune_label_reads"]
+                    + 8 * stats["prune_tile_reads"] + 8 * stats["candidates"], stats["t_prune_s"]),
         "k_expand": (28 * stats["frontier_items"] + 12 * stats["scanned_edges"]
                      + 8 * stats["candidates"], stats["t_expand_s"]),
     }

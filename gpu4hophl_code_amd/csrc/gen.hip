@@ -118,6 +118,11 @@ struct GenArgs {
   const uint2 *adj;
   unsigned long long *pool;
   unsigned long long pool_cap;
+  // the pool's labels again as 32-bit words (hub | hop | dist fields, widths
+  // below), kept only when the fields fit; the prune streams these
+  uint32_t *pool32;
+  uint32_t p32_hop_bits;
+  uint32_t p32_dist_bits;
   unsigned long long *seg_start;
   uint32_t *seg_len;
   uint32_t *seg_cap;
@@ -145,6 +150,33 @@ struct GenArgs {
 };
 
 __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
+
+// 32-bit labels of the pool copy: hub << (hop_bits + dist_bits) | hop <<
+// dist_bits | dist.
+__device__ inline uint32_t pack32(const GenArgs &a, uint32_t hub, uint32_t hop, uint32_t dist) {
+  return (hub << (a.p32_hop_bits + a.p32_dist_bits)) | (hop << a.p32_dist_bits) | dist;
+}
+__device__ inline void unpack(const GenArgs &a, uint32_t l, uint32_t &hub, uint32_t &hop,
+                              uint32_t &dist) {
+  hub = l >> (a.p32_hop_bits + a.p32_dist_bits);
+  hop = (l >> a.p32_dist_bits) & ((1u << a.p32_hop_bits) - 1u);
+  dist = l & ((1u << a.p32_dist_bits) - 1u);
+}
+__device__ inline void unpack(const GenArgs &, unsigned long long l, uint32_t &hub,
+                              uint32_t &hop, uint32_t &dist) {
+  hub = label_hub(l);
+  hop = label_hop(l);
+  dist = label_dist(l);
+}
+// the pool a prune streams: LT = uint32_t (pool32) or unsigned long long
+template <class LT> __device__ inline const LT *stream_pool(const GenArgs &a);
+template <> __device__ inline const uint32_t *stream_pool<uint32_t>(const GenArgs &a) {
+  return a.pool32;
+}
+template <> __device__ inline const unsigned long long *stream_pool<unsigned long long>(
+    const GenArgs &a) {
+  return a.pool;
+}
 
 // Dense dedupe slots keep, per (hub slot, vertex), the smallest candidate
 // distance of every round of the current batch attempt:
@@ -182,6 +214,7 @@ __global__ void k_init_labels(GenArgs a, uint32_t c0) {
     a.add_cnt[v] = 0;
     a.add_cur[v] = 0;
     a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
+    if (a.pool32) a.pool32[v * c0] = pack32(a, (uint32_t)v, 0, 0);
   }
 }
 
@@ -579,19 +612,21 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals
 // waits about one memory round trip per 128 labels, not per 64.  The tile is
 // read through address-space pointers (KP / VP, device.hpp), so a probe's
 // wait does not also wait for the next step's loads.
-template <class V, class KP, class VP>
-__device__ inline bool probe_label(const GenArgs &a, unsigned long long l, bool valid,
-                                   uint32_t dy, KP keys, VP vals, uint32_t cap, uint32_t K1) {
-  const uint32_t ha = label_hop(l);
+template <class V, class KP, class VP, class LT>
+__device__ inline bool probe_label(const GenArgs &a, LT l, bool valid, uint32_t dy, KP keys,
+                                   VP vals, uint32_t cap, uint32_t K1) {
+  uint32_t hub, ha, da;
+  unpack(a, l, hub, ha, da);
   if (!valid || ha > a.h) return false;
-  const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, label_hub(l), a.h - ha);
-  return v != TileVal<V>::kEmpty && label_dist(l) + v <= dy;
+  const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, hub, a.h - ha);
+  return v != TileVal<V>::kEmpty && da + v <= dy;
 }
 
 // Label `lane` of a list of `len` labels, clamped to the list (a lane past
 // the end reloads the last label and is masked in the probe); an empty list
 // reads its first slot, which lies in the pool.
-__device__ inline unsigned long long first_step(const unsigned long long *L, uint32_t len) {
+template <class LT>
+__device__ inline LT first_step(const LT *L, uint32_t len) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t last = len ? len - 1 : 0;
   return L[min(lane, last)];
@@ -607,10 +642,9 @@ __device__ inline unsigned long long first_step(const unsigned long long *L, uin
 // end) keep the next 128-label step in flight while one is probed.  No load
 // sits behind a branch and the step buffers alternate (a, b) instead of
 // being copied, so a wait leaves the step in flight alone.
-template <class V, class KP, class VP>
-__device__ bool witness_search(const GenArgs &a, const unsigned long long *L, uint32_t len,
-                               unsigned long long f0, uint32_t dy, KP keys, VP vals,
-                               uint32_t cap, uint32_t &reads) {
+template <class V, class KP, class VP, class LT>
+__device__ bool witness_search(const GenArgs &a, const LT *L, uint32_t len, LT f0, uint32_t dy,
+                               KP keys, VP vals, uint32_t cap, uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
   if (len == 0) {
@@ -619,7 +653,7 @@ __device__ bool witness_search(const GenArgs &a, const unsigned long long *L, ui
   }
   const uint32_t last = len - 1;
   auto ld = [&](uint32_t j) { return L[min(j, last)]; };
-  auto probe2 = [&](unsigned long long l0, unsigned long long l1, uint32_t j) {
+  auto probe2 = [&](LT l0, LT l1, uint32_t j) {
     const bool h0 = probe_label<V>(a, l0, j + lane < len, dy, keys, vals, cap, K1);
     const bool h1 = probe_label<V>(a, l1, j + kWave + lane < len, dy, keys, vals, cap, K1);
     return __ballot(h0 || h1) != 0;
@@ -634,7 +668,7 @@ __device__ bool witness_search(const GenArgs &a, const unsigned long long *L, ui
     return false;
   }
   {
-    const unsigned long long s0 = ld(kWave + lane), s1 = ld(kWave + kWave + lane);
+    const LT s0 = ld(kWave + lane), s1 = ld(kWave + kWave + lane);
     if (probe2(s0, s1, kWave)) {
       reads = min(len, kWave + kStep);
       return true;
@@ -645,10 +679,10 @@ __device__ bool witness_search(const GenArgs &a, const unsigned long long *L, ui
     reads = len;
     return false;
   }
-  unsigned long long b0 = ld(kLong + lane), b1 = ld(kLong + kWave + lane);
+  LT b0 = ld(kLong + lane), b1 = ld(kLong + kWave + lane);
   for (uint32_t j = kLong; j < len; j += 2 * kStep) {
     // b holds labels [j, j + kStep); a is loaded with [j + kStep, j + 2 kStep)
-    const unsigned long long a0 = ld(j + kStep + lane), a1 = ld(j + kStep + kWave + lane);
+    const LT a0 = ld(j + kStep + lane), a1 = ld(j + kStep + kWave + lane);
     if (probe2(b0, b1, j)) {
       reads = min(len, j + kStep);
       return true;
@@ -717,7 +751,7 @@ constexpr uint32_t kClaimSpread = 2 * (kPruneThreads / kWave);
 // round trip per level (lane i: claimed candidate i); the first 64 labels of
 // the next claimed candidate's list are loaded before the current one's probes.
 // Instantiated once for an LDS tile and once for an arena tile.
-template <class V, class KP, class VP>
+template <class V, class LT, class KP, class VP>
 __device__ __attribute__((always_inline)) inline void prune_candidates(
     const GenArgs &a, uint32_t b, uint32_t c_beg, uint32_t c_end, const uint32_t *cands, KP keys,
     VP vals, uint32_t cap, uint32_t &sh_next, uint32_t &sh_acc_n, Item *sh_acc,
@@ -747,14 +781,14 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
     auto start_of = [&](uint32_t i) {
       const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)st, i);
       const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(st >> 32), i);
-      return a.pool + (((unsigned long long)hi << 32) | lo);
+      return stream_pool<LT>(a) + (((unsigned long long)hi << 32) | lo);
     };
-    unsigned long long f0 = first_step(start_of(0), __builtin_amdgcn_readlane(len, 0));
+    LT f0 = first_step(start_of(0), __builtin_amdgcn_readlane(len, 0));
     for (uint32_t i = 0; i < cn; ++i) {
       // next candidate's first step (the last one reloads its own, so the
       // loads are unconditional)
       const uint32_t nx = min(i + 1, cn - 1);
-      const unsigned long long g0 = first_step(start_of(nx), __builtin_amdgcn_readlane(len, nx));
+      const LT g0 = first_step(start_of(nx), __builtin_amdgcn_readlane(len, nx));
       const uint32_t yi = __builtin_amdgcn_readlane(y, i), di = __builtin_amdgcn_readlane(dy, i);
       uint32_t reads = 0;
       const bool pruned = witness_search<V>(a, start_of(i), __builtin_amdgcn_readlane(len, i), f0,
@@ -772,7 +806,8 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
   }
 }
 
-template <class V>
+// LT: label type of the stream (uint32_t: the pool's 32-bit copy).
+template <class V, class LT>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune(GenArgs a) {
   extern __shared__ uint32_t lds[];
@@ -845,7 +880,7 @@ k_prune(GenArgs a) {
         tile_b = b;
         tile_reads += a.seg_len[s];
       }
-      prune_candidates<V>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds,
+      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds,
                           (lds_ptr<const V>)(lds + lcap), lcap, sh_next, sh_acc_n, sh_acc,
                           wave_reads);
     } else {
@@ -862,7 +897,7 @@ k_prune(GenArgs a) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tile_b = kInf32;
       tile_reads += a.seg_len[s];
-      prune_candidates<V>(a, b, c_beg, c_end, cands, (glb_ptr<const uint32_t>)gkeys,
+      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (glb_ptr<const uint32_t>)gkeys,
                           (glb_ptr<const V>)(gkeys + cap), cap, sh_next, sh_acc_n, sh_acc,
                           wave_reads);
     }
@@ -895,6 +930,8 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
     }
     const unsigned long long os = a.seg_start[y];
     for (uint32_t i = lane; i < len; i += kWave) a.pool[ns + i] = a.pool[os + i];
+    if (a.pool32)
+      for (uint32_t i = lane; i < len; i += kWave) a.pool32[ns + i] = a.pool32[os + i];
     if (lane == 0) {
       a.seg_start[y] = ns;
       a.seg_cap[y] = ncap;
@@ -908,10 +945,13 @@ __global__ void __launch_bounds__(256) k_append(GenArgs a) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
     const Item it = Fn[i];
     const uint32_t p = a.seg_len[it.y] + atomicAdd(&a.add_cur[it.y], 1u);
-    if (p < a.seg_cap[it.y])
-      a.pool[a.seg_start[it.y] + p] = pack_label(a.s0 + it.b, a.h, it.d);
-    else
+    if (p < a.seg_cap[it.y]) {
+      const unsigned long long at = a.seg_start[it.y] + p;
+      a.pool[at] = pack_label(a.s0 + it.b, a.h, it.d);
+      if (a.pool32) a.pool32[at] = pack32(a, a.s0 + it.b, a.h, it.d);
+    } else {
       atomicOr(&a.ctrl->overflow, kOvfPool);
+    }
   }
 }
 
@@ -1007,7 +1047,7 @@ static int upload_graph(hsdl_graph *g, int dev) {
 
 struct Gen {
   GenArgs a{};
-  DevBuf pool, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
+  DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
   DevBuf cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
@@ -1024,7 +1064,8 @@ struct Gen {
   uint64_t n_prune = 0;
   double h_grow = 0, h_enqueue = 0, h_wait = 0;  // host-side phase timers (verbose)
   uint64_t held() const {
-    return fixed + pool.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes + gtile.bytes;
+    return fixed + pool.bytes + pool32.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes +
+           gtile.bytes;
   }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1050,15 +1091,24 @@ static int read_ctrl(Gen &g, Ctrl &c) {
 static int grow_pool(Gen &g, uint64_t used, uint64_t min_cap) {
   uint64_t ncap = g.a.pool_cap * 2;
   while (ncap < min_cap) ncap *= 2;
-  if (g.held() - g.pool.bytes + ncap * 8 > g.budget)
+  const bool p32 = g.a.pool32 != nullptr;
+  const uint64_t per_label = p32 ? 12 : 8;
+  if (g.held() - g.pool.bytes - g.pool32.bytes + ncap * per_label > g.budget)
     return fail(HSDL_ENOMEM, "label pool exceeds the HBM budget");
-  DevBuf np;
+  DevBuf np, np32;
   int rc = np.alloc(ncap * 8);
+  if (!rc && p32) rc = np32.alloc(ncap * 4);
   if (rc) return rc;
   HSDL_HIP(hipMemcpyAsync(np.p, g.pool.p, used * 8, hipMemcpyDeviceToDevice, g.st));
+  if (p32)
+    HSDL_HIP(hipMemcpyAsync(np32.p, g.pool32.p, used * 4, hipMemcpyDeviceToDevice, g.st));
   HSDL_HIP(hipStreamSynchronize(g.st));
   g.pool = std::move(np);
   g.a.pool = g.pool.as<unsigned long long>();
+  if (p32) {
+    g.pool32 = std::move(np32);
+    g.a.pool32 = g.pool32.as<uint32_t>();
+  }
   g.a.pool_cap = ncap;
   return HSDL_OK;
 }
@@ -1124,10 +1174,14 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
-    if (g.small_vals)
-      hipLaunchKernelGGL(k_prune<uint16_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
-    else
-      hipLaunchKernelGGL(k_prune<uint32_t>, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
+    {
+      // 16- or 32-bit tile values; 32- or 64-bit labels streamed
+      auto kp = g.small_vals ? (a.pool32 ? k_prune<uint16_t, uint32_t>
+                                         : k_prune<uint16_t, unsigned long long>)
+                             : (a.pool32 ? k_prune<uint32_t, uint32_t>
+                                         : k_prune<uint32_t, unsigned long long>);
+      hipLaunchKernelGGL(kp, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
+    }
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
@@ -1218,7 +1272,18 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint32_t heavy_cap = env_u32("HSDL_HEAVY_CAP", 1u << 20, 1, 1u << 20);
   g.fixed = vert_bytes + (uint64_t)Bmax * n * 4 + (Bmax + 1ull) * 16 + (2ull << 20) +
             (uint64_t)heavy_cap * sizeof(HeavyItem);
-  if (g.budget < g.fixed + pool0 * 8 + gtile0 * 4 + (uint64_t)list0 * 36)
+  // 32-bit copy of the pool for the prune's stream, when hub, hop and
+  // distance fields fit in one word (tuning knob HSDL_STREAM32=0 keeps the
+  // prune on the 8-byte pool)
+  uint32_t p32_hop_bits = 1, p32_dist_bits = 1, p32_hub_bits = 1;
+  while ((1ull << p32_hop_bits) <= K) ++p32_hop_bits;
+  while ((1ull << p32_dist_bits) <= (uint64_t)K * (uint64_t)std::max(hg->max_w, 1))
+    ++p32_dist_bits;
+  while ((1ull << p32_hub_bits) <= (uint64_t)(n - 1)) ++p32_hub_bits;
+  const bool use32 = env_u32("HSDL_STREAM32", 1, 0, 1) &&
+                     p32_hub_bits + p32_hop_bits + p32_dist_bits <= 32;
+  const uint64_t per_label = use32 ? 12 : 8;
+  if (g.budget < g.fixed + pool0 * per_label + gtile0 * 4 + (uint64_t)list0 * 36)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
   if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
@@ -1228,7 +1293,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))) ||
-      (rc = g.heavy.alloc((uint64_t)heavy_cap * sizeof(HeavyItem))))
+      (rc = g.heavy.alloc((uint64_t)heavy_cap * sizeof(HeavyItem))) ||
+      (use32 && (rc = g.pool32.alloc(pool0 * 4))))
     return rc;
   {
     const size_t need = (size_t)Bmax * n * 4;
@@ -1267,6 +1333,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.adj = static_cast<const uint2 *>(hg->d_adj);
   a.pool = g.pool.as<unsigned long long>();
   a.pool_cap = pool0;
+  a.pool32 = use32 ? g.pool32.as<uint32_t>() : nullptr;
+  a.p32_hop_bits = p32_hop_bits;
+  a.p32_dist_bits = p32_dist_bits;
   a.seg_start = g.seg_start.as<unsigned long long>();
   a.seg_len = g.seg_len.as<uint32_t>();
   a.seg_cap = g.seg_cap.as<uint32_t>();
@@ -1464,6 +1533,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   idx->stats.prune_launches = g.n_prune;
   idx->stats.prune_label_reads = c.label_reads;
   idx->stats.prune_tile_reads = c.tile_reads;
+  idx->stats.prune_label_bytes = a.pool32 ? 4 : 8;
   *out = idx;
   return HSDL_OK;
 }

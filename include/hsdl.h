@@ -176,6 +176,11 @@ typedef struct hsdl_stats {
                                        by the clean's first pass            */
   double t_parents_s;               /* parent derivation (opts.parents),
                                        included in t_generate_s            */
+  uint32_t prune_label_bytes;       /* bytes per label of L(y) the prune
+                                       streamed: 4 when hub, hop and dist
+                                       fit one word (a 32-bit copy of the
+                                       label pool), else 8                  */
+  uint32_t reserved_stats;
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 
