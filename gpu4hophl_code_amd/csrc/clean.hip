@@ -79,6 +79,41 @@ __device__ inline uint32_t tile_slot(KP keys, uint32_t cap, uint32_t hub) {
   return k == hub ? slot : kInf32;
 }
 
+// The heads' lists L(u) are streamed from the index's 8-byte labels, or from
+// a 32-bit copy of them (hub | hop | dist fields) when the fields fit one
+// word: the first pass reads far more list labels than anything else and
+// runs near the rate HBM sustains, so half the bytes is what counts.
+struct Stream64 {
+  using T = unsigned long long;
+  const unsigned long long *lab;
+  __device__ const T *list(unsigned long long b) const { return lab + b; }
+  __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
+    w = label_hub(l);
+    h = label_hop(l);
+    d = label_dist(l);
+  }
+};
+struct Stream32 {
+  using T = uint32_t;
+  const uint32_t *lab;
+  uint32_t hop_bits, dist_bits;
+  __device__ const T *list(unsigned long long b) const { return lab + b; }
+  __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
+    w = l >> (hop_bits + dist_bits);
+    h = (l >> dist_bits) & ((1u << hop_bits) - 1u);
+    d = l & ((1u << dist_bits) - 1u);
+  }
+};
+
+__global__ void k_pack32(uint64_t total, const unsigned long long *lab, uint32_t *out,
+                         uint32_t hop_bits, uint32_t dist_bits) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long l = lab[i];
+    out[i] = (label_hub(l) << (hop_bits + dist_bits)) | (label_hop(l) << dist_bits) | label_dist(l);
+  }
+}
+
 // True when the all-earlier predicate removes a label of a group whose head
 // lies in L(v)[p0, p0 + win).  The wave walks the window's heads in order
 // (uniform control flow: the group's labels are uniform scalar loads) and
@@ -87,10 +122,9 @@ __device__ inline uint32_t tile_slot(KP keys, uint32_t cap, uint32_t hub) {
 // alternate (a, b) and loads are clamped to the list (lanes past the end are
 // masked), so a step's probes run while the next step's loads are in flight.
 // KP / VP: the tile's address space (LDS or the global arena).
-template <class V, class KP, class VP>
+template <class V, class S, class KP, class VP>
 __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off,
-                               const unsigned long long *__restrict__ lab,
-                               const unsigned long long *__restrict__ Lv, uint32_t len,
+                               const S &str, const unsigned long long *__restrict__ Lv, uint32_t len,
                                uint32_t p0, uint32_t win, KP keys, VP vals, uint32_t cap,
                                uint64_t &streamed) {
   const uint32_t lane = (uint32_t)lane_id();
@@ -109,13 +143,14 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     const unsigned long long ub = off[u];
     const uint32_t ulen = (uint32_t)(off[u + 1] - ub);
     if (ulen == 0) continue;
-    const unsigned long long *Lu = lab + ub;
+    const typename S::T *Lu = str.list(ub);
     streamed += ulen;
     const uint32_t last = ulen - 1;
     // one element (w,h1,d1) of L(u) at position j against the group
-    auto check = [&](unsigned long long l, uint32_t j) {
+    auto check = [&](typename S::T l, uint32_t j) {
       if (j >= ulen) return;
-      const uint32_t w = label_hub(l), h1 = label_hop(l), d1 = label_dist(l);
+      uint32_t w, h1, d1;
+      str.dec(l, w, h1, d1);
       if (w < u) {
         // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
         if (h1 + 1 <= maxh && d1 <= maxd) {
@@ -144,9 +179,9 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       }
     };
     auto ld = [&](uint32_t j) { return Lu[min(j, last)]; };
-    unsigned long long a0 = ld(lane), a1 = ld(kWave + lane);
+    typename S::T a0 = ld(lane), a1 = ld(kWave + lane);
     for (uint32_t jb = 0; jb < ulen; jb += 4 * kWave) {
-      const unsigned long long b0 = ld(jb + 2 * kWave + lane), b1 = ld(jb + 3 * kWave + lane);
+      const typename S::T b0 = ld(jb + 2 * kWave + lane), b1 = ld(jb + 3 * kWave + lane);
       check(a0, jb + lane);
       check(a1, jb + kWave + lane);
       if (jb + 2 * kWave >= ulen) break;
@@ -162,10 +197,10 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
 // Builds the tile of L(v) (keys, vals in LDS or in the workgroup's arena
 // slice) and runs the windows of L(v) over it; true when some window removes
 // a label.  Whole workgroup.
-template <class V, class KP, class VP>
+template <class V, class S, class KP, class VP>
 __device__ __attribute__((always_inline)) inline bool clean_vertex(
-    uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off,
-    const unsigned long long *__restrict__ lab, const unsigned long long *__restrict__ Lv,
+    uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off, const S &str,
+    const unsigned long long *__restrict__ Lv,
     uint32_t len, KP keys, VP vals, uint32_t cap, bool in_arena, uint32_t win, uint32_t &sh_win,
     uint32_t &sh_removed, uint64_t &streamed) {
   const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id();
@@ -202,7 +237,7 @@ __device__ __attribute__((always_inline)) inline bool clean_vertex(
     if (lane == 0) p0 = atomicAdd(&sh_win, win);
     p0 = __builtin_amdgcn_readfirstlane(__shfl(p0, 0, kWave));
     if (p0 >= len) break;
-    if (window_removes<V>(v, K1, off, lab, Lv, len, p0, win, keys, vals, cap, streamed)) {
+    if (window_removes<V>(v, K1, off, str, Lv, len, p0, win, keys, vals, cap, streamed)) {
       if (lane == 0) sh_removed = 1;
       break;
     }
@@ -213,10 +248,10 @@ __device__ __attribute__((always_inline)) inline bool clean_vertex(
 
 // ctr[0]: redo count, ctr[2]: next vertex, ctr[4..5]: labels of L(u) read (u64).  gtile: per-workgroup arena slices
 // of gwords words for tiles above lds_slots slots.
-template <class V>
+template <class V, class S>
 __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
-             const unsigned long long *__restrict__ lab,
+             const unsigned long long *__restrict__ lab, S str,
              uint8_t *keep, uint32_t *new_len, uint32_t *redo, uint32_t *ctr, uint32_t lds_slots,
              uint32_t *gtile, uint64_t gwords, uint32_t win) {
   extern __shared__ uint32_t lds[];
@@ -251,10 +286,10 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
       const uint32_t cap = tile_cap_for(sh_heads);
       if (cap <= lds_slots) {
         const uint32_t lcap = lds_tile_cap(sh_heads, lds_slots);
-        clean_vertex<V>(v, K1, off, lab, Lv, len, (lds_ptr<uint32_t>)lds,
+        clean_vertex<V>(v, K1, off, str, Lv, len, (lds_ptr<uint32_t>)lds,
                         (lds_ptr<V>)(lds + lcap), lcap, false, win, sh_win, sh_removed, streamed);
       } else
-        clean_vertex<V>(v, K1, off, lab, Lv, len, (glb_ptr<uint32_t>)gkeys,
+        clean_vertex<V>(v, K1, off, str, Lv, len, (glb_ptr<uint32_t>)gkeys,
                         (glb_ptr<V>)(gkeys + cap), cap, true, win, sh_win, sh_removed, streamed);
     }
     if (sh_removed) {
@@ -472,18 +507,45 @@ int clean_index(hsdl_index *idx) {
   const uint64_t gwords = small_vals ? tile_words<uint16_t>(gcap, K1) : tile_words<uint32_t>(gcap, K1);
   DevBuf gtile;
   if ((rc = gtile.alloc((uint64_t)tgrid * gwords * 4))) return rc;
-  if (small_vals)
-    hipLaunchKernelGGL(k_clean_tile<uint16_t>, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n,
-                       (uint32_t)idx->K, idx->off.as<unsigned long long>(),
-                       idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
-                       new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, lds_slots,
-                       gtile.as<uint32_t>(), gwords, win);
-  else
-    hipLaunchKernelGGL(k_clean_tile<uint32_t>, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n,
-                       (uint32_t)idx->K, idx->off.as<unsigned long long>(),
-                       idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
-                       new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, lds_slots,
-                       gtile.as<uint32_t>(), gwords, win);
+  // 32-bit copy of the labels for the heads' list streams, when hub, hop and
+  // distance fields fit one word (tuning knob HSDL_STREAM32=0 keeps the
+  // 8-byte stream; so does a failed allocation of the copy)
+  uint32_t hub_bits = 1, hop_bits = 1, dist_bits = 1;
+  while ((1ull << hub_bits) <= (uint64_t)(n ? n - 1 : 0)) ++hub_bits;
+  while ((1ull << hop_bits) <= (uint64_t)idx->K) ++hop_bits;
+  while ((1ull << dist_bits) <= (uint64_t)mx[1]) ++dist_bits;
+  DevBuf lab32;
+  bool use32 = env_u32("HSDL_STREAM32", 1, 0, 1) && idx->total > 0 &&
+               hub_bits + hop_bits + dist_bits <= 32;
+  if (use32) {
+    void *p = nullptr;
+    if (hipMalloc(&p, idx->total * 4) == hipSuccess) {
+      lab32.p = p;
+      lab32.bytes = idx->total * 4;
+      hipLaunchKernelGGL(k_pack32, dim3(cus * 8), dim3(256), 0, st, (uint64_t)idx->total,
+                         idx->lab.as<unsigned long long>(), lab32.as<uint32_t>(), hop_bits,
+                         dist_bits);
+      HSDL_HIP(hipGetLastError());
+    } else {
+      (void)hipGetLastError();
+      use32 = false;
+    }
+  }
+  const Stream64 s64{idx->lab.as<unsigned long long>()};
+  const Stream32 s32{lab32.as<uint32_t>(), hop_bits, dist_bits};
+  auto launch = [&](auto kern, auto str) {
+    hipLaunchKernelGGL(kern, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n, (uint32_t)idx->K,
+                       idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(), str,
+                       keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
+                       lds_slots, gtile.as<uint32_t>(), gwords, win);
+  };
+  if (small_vals) {
+    if (use32) launch(k_clean_tile<uint16_t, Stream32>, s32);
+    else launch(k_clean_tile<uint16_t, Stream64>, s64);
+  } else {
+    if (use32) launch(k_clean_tile<uint32_t, Stream32>, s32);
+    else launch(k_clean_tile<uint32_t, Stream64>, s64);
+  }
   HSDL_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_clean_seq, dim3(grid), dim3(kCleanWaves * kWave), 0, st, (uint32_t)idx->K,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),

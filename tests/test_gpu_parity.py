@@ -204,9 +204,14 @@ def test_grid_k8_matches_oracle():
     same_labels(idx, O.clean(want).arrays())
 
 
-def test_clean_removes_exactly_the_redundant_labels():
+@pytest.mark.parametrize("stream32", ["1", "0"])
+def test_clean_removes_exactly_the_redundant_labels(monkeypatch, stream32):
     """Inputs with redundancy (the reference's group-restricted pruning): the
-    GPU clean must return the oracle clean, which equals canonical labels."""
+    GPU clean must return the oracle clean, which equals canonical labels,
+    whether its first pass streams the heads' lists from the 32-bit copy of
+    the labels (the fields fit one word here) or from the 8-byte labels
+    (HSDL_STREAM32=0)."""
+    monkeypatch.setenv("HSDL_STREAM32", stream32)
     for seed, (scale, K, group) in enumerate([(9, 3, 16), (10, 4, 64), (10, 5, 7), (11, 3, 128)]):
         g = H.Graph.rmat(scale, 8, 1000 + seed, 1, 20)
         c = csr_of(g)
