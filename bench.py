@@ -212,8 +212,8 @@ def main():
                          "a stopped build reports a lower bound")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this "
+              "build runs the generation, clean and queries on a single GPU", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -262,6 +262,15 @@ def main():
         "k_expand": (28 * stats["frontier_items"] + 12 * stats["scanned_edges"]
                      + 8 * stats["candidates"], stats["t_expand_s"]),
     }
+    def traffic_ratio(name):
+        # HBM bytes per algorithmic byte, measured with rocprofv3 FETCH_SIZE and
+        # WRITE_SIZE passes on this workload (profiles/<kernel>_<config>_k<K>.md)
+        tpath = os.path.join(ROOT, "profiles", f"{name}_{args.config}_k{K}_traffic.json")
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                return json.load(f).get("traffic_per_algorithmic_byte")
+        return None
+
     rl = {}
     for name, (nbytes, secs) in kern.items():
         achieved = nbytes / secs / 1e9 if secs > 0 else 0.0
@@ -269,13 +278,7 @@ def main():
              "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
              "kernel": name, "bytes_per_launch": nbytes / launches,
              "avg_launch_us": secs / launches * 1e6, "launches": launches, "seconds": secs}
-        # HBM bytes per algorithmic byte, measured with rocprofv3 FETCH_SIZE and
-        # WRITE_SIZE passes on this workload (profiles/<kernel>_<config>_k<K>.md)
-        tpath = os.path.join(ROOT, "profiles", f"{name}_{args.config}_k{K}_traffic.json")
-        ratio = None
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                ratio = json.load(f).get("traffic_per_algorithmic_byte")
+        ratio = traffic_ratio(name)
         if ratio:
             r["traffic"] = ratio * r["bytes_per_launch"]
             r["traffic_per_algorithmic_byte"] = ratio
@@ -285,7 +288,8 @@ def main():
     roofline_other = {k: v for k, v in rl.items() if k != top}
     # achievable copy bandwidth on this GPU (STREAM-style, read + write) and
     # the rate of random returning 32-bit atomicMin over 16 GB (the dense
-    # dedupe slots' operation: expansion is bound by it)
+    # slots' update; the expansion reads a slot first and updates it only
+    # when the edge lowers it)
     import ctypes
     gbps = ctypes.c_double()
     if H.lib().hsdl_device_copy_bench(1 << 30, 5, ctypes.byref(gbps)) == 0:
@@ -294,17 +298,23 @@ def main():
     if H.lib().hsdl_device_atomic_bench(16 << 30, 1 << 30, ctypes.byref(gops)) == 0:
         ex = rl["k_expand"]
         ex["atomic_peak_gops"] = round(gops.value, 2)
-        ex["atomic_gops"] = round(stats["scanned_edges"] / stats["t_expand_s"] / 1e9, 2) \
+        # dense-slot accesses per second (one read per scanned edge, and an
+        # atomic when the edge lowers the slot)
+        ex["slot_gops"] = round(stats["scanned_edges"] / stats["t_expand_s"] / 1e9, 2) \
             if stats["t_expand_s"] > 0 else None
-    # the clean's first pass: 8 B per label of L(v) + 8 B per label of the
-    # heads' lists L(u) it streams (SURVEY's B_clean with stored label size)
+    # the clean's first pass: 8 B per label of L(v) + 4 or 8 B per label of
+    # the heads' lists L(u) it streams (the stream's label width)
     if stats["t_clean_s"] > 0:
-        cb = 8 * stats["labels_uncleaned"] + 8 * stats["clean_label_reads"]
+        cb = 8 * stats["labels_uncleaned"] + \
+            (stats.get("clean_label_bytes") or 8) * stats["clean_label_reads"]
         ca = cb / stats["t_clean_s"] / 1e9
+        cr = traffic_ratio("k_clean_tile")
         roofline_other["k_clean_tile"] = {
             "bound": "hbm", "achieved": round(ca, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ca / HBM_PEAK_GBS, 4), "traffic": None, "bytes": cb,
+            "frac": round(ca / HBM_PEAK_GBS, 4), "traffic": cr * cb if cr else None, "bytes": cb,
             "seconds": stats["t_clean_s"], "note": "host-timed clean (both passes, compaction)"}
+        if cr:
+            roofline_other["k_clean_tile"]["traffic_per_algorithmic_byte"] = cr
 
     # queries on the device-resident cleaned index
     lens = label_lengths(idx)

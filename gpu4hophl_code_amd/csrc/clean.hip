@@ -578,6 +578,7 @@ int clean_index(hsdl_index *idx) {
   idx->stats.clean_passes = hc[0] ? 2 : 1;
   idx->stats.clean_seq_vertices = hc[0];
   idx->stats.clean_label_reads = (uint64_t)hc[4] | ((uint64_t)hc[5] << 32);
+  idx->stats.clean_label_bytes = use32 ? 4 : 8;
   idx->stats.t_clean_s = (now_ns() - t0) * 1e-9;
   return HSDL_OK;
 }

@@ -180,7 +180,8 @@ typedef struct hsdl_stats {
                                        streamed: 4 when hub, hop and dist
                                        fit one word (a 32-bit copy of the
                                        label pool), else 8                  */
-  uint32_t reserved_stats;
+  uint32_t clean_label_bytes;       /* same for the heads' lists the clean's
+                                       first pass streamed (0: no clean)   */
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 
