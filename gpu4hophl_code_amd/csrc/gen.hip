@@ -93,6 +93,8 @@ struct Ctrl {
   uint32_t expand_ctr;  // next 64-item chunk of the frontier (k_expand)
   uint32_t heavy_cnt;   // pieces pushed to the heavy list this round
   uint32_t heavy_ctr;   // next heavy piece (k_expand_heavy)
+  uint32_t row_mode;    // this round's prune takes row items (k_prune_rows)
+  uint32_t row_ctr[8];  // next row item of each XCD's queue
 };
 
 // A piece of a high-degree frontier item: up to kHeavyPiece consecutive
@@ -113,6 +115,7 @@ struct GenArgs {
   uint32_t h;
   uint32_t cur;        // frontier buffer read this round
   uint32_t tile;       // candidates per prune work item
+  uint32_t row_policy; // prune row items: 0 never, 1 when candidates are dense, 2 always
   uint32_t lds_slots;  // LDS tile capacity (slots)
   const unsigned long long *off;
   const uint2 *adj;
@@ -447,6 +450,13 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
 }
 
 // ------------------------------------------------------------ chunk scan
+// A round's prune takes row items (k_prune_rows) when its candidates fill at
+// least 1/kRowDensity of the batch's dense slots.
+constexpr uint32_t kRowDensity = 32;
+// Row items: XCDs of an MI355X (each has its own L2) and vertices per item.
+constexpr uint32_t kXcds = 8;
+constexpr uint32_t kRowBlock = 2048;
+
 // One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
 // sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
 __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
@@ -478,11 +488,22 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     run += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
     runc += a.hub_ccnt[b];
   }
+  if (t < 8) a.ctrl->row_ctr[t] = 0;
   if (t == blockDim.x - 1) {
     const bool dead = a.ctrl->overflow != 0;
+    // row items when the round's candidates fill at least 1/kRowDensity of
+    // the batch's slot rows (or always / never by the policy)
+    // (and only while an XCD queue's item count fits its 32-bit counter)
+    const uint64_t queue_items =
+        (uint64_t)a.B * (((uint64_t)a.n + kRowBlock * kXcds - 1) / (kRowBlock * kXcds));
+    const bool rows = !dead && partc[t] && queue_items < (1ull << 31) &&
+                      (a.row_policy == 2 ||
+                       (a.row_policy == 1 &&
+                        (uint64_t)partc[t] * kRowDensity >= (uint64_t)a.B * a.n));
     a.chunk_off[a.B] = part[t];
     a.hub_off[a.B] = partc[t];
-    a.ctrl->total_chunks = dead ? 0u : part[t];
+    a.ctrl->row_mode = rows ? 1u : 0u;
+    a.ctrl->total_chunks = (dead || rows) ? 0u : part[t];
     a.ctrl->work_ctr = 0;
     a.ctrl->cand_total += partc[t];
     a.ctrl->item_total += min(a.ctrl->fcnt[a.cur], a.fcap);
@@ -504,7 +525,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
   __shared__ uint32_t hkey[kGroupSlots];
   __shared__ uint32_t hcnt[kGroupSlots];
   __shared__ uint32_t hbase[kGroupSlots];
-  if (a.ctrl->overflow) return;
+  if (a.ctrl->overflow || a.ctrl->row_mode) return;
   // reserved blocks may reach past the list's end; only empty entries lie there
   const uint32_t cnt = min(a.ctrl->ccnt, a.ccap);
   const uint32_t tid = threadIdx.x;
@@ -908,6 +929,134 @@ k_prune(GenArgs a) {
   if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
 }
 
+// --------------------------------------------------------- prune, by rows
+// Row items for rounds whose candidates are dense (k_chunk_scan sets
+// row_mode): item (k, b) is hub slot b's candidates among the vertices
+// [k * kRowBlock, (k + 1) * kRowBlock), found by scanning b's dense slot row
+// there (fresh slots of the running attempt are exactly the round's
+// candidates).  Each XCD has its own queue of y-blocks k = x, x + 8, ...,
+// each with all B hubs in turn, so the B hubs' items of one y-block run at
+// the same time on one XCD and re-read the same lists L(y) from its L2
+// instead of from HBM (hub-ordered items re-read a vertex's list once per
+// hub, spread over the whole round).  Workgroups are dealt to XCDs
+// round-robin by id; a workgroup whose queue runs dry takes items from the
+// other queues in turn.  The decisions do not depend on the order: a round's
+// prune reads only labels of earlier rounds and batches.
+template <class V, class LT>
+__global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
+k_prune_rows(GenArgs a) {
+  extern __shared__ uint32_t lds[];
+  __shared__ uint32_t sh_b, sh_k, sh_q, sh_next, sh_cnt;
+  __shared__ uint32_t sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base;
+  __shared__ Item sh_acc[kAccSlots];
+  __shared__ uint32_t sh_touch[kAccSlots];
+  __shared__ uint32_t sh_cand[kPruneThreads];
+  if (!a.ctrl->row_mode) return;
+  const int lane = lane_id();
+  const uint32_t K1 = a.K + 1;
+  Item *Fn = frontier(a, a.cur ^ 1);
+  uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
+  unsigned long long wave_reads = 0, tile_reads = 0;
+  const uint64_t gwords = a.gtile_cap / gridDim.x;
+  uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
+  const uint32_t nblk = (a.n + kRowBlock - 1) / kRowBlock;
+  const uint32_t home = blockIdx.x % kXcds;
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  if (threadIdx.x == 0) {
+    sh_q = 0;
+    sh_acc_n = 0;
+    sh_touch_n = 0;
+  }
+  __syncthreads();
+  for (;;) {
+    if (threadIdx.x == 0) {
+      sh_b = kInf32;
+      while (sh_q < kXcds) {
+        const uint32_t x = (home + sh_q) % kXcds;
+        const uint32_t nb = x < nblk ? (nblk - x + kXcds - 1) / kXcds : 0u;
+        const uint64_t items = (uint64_t)nb * a.B;
+        const uint32_t j = items ? atomicAdd(&a.ctrl->row_ctr[x], 1u) : 0u;
+        if (j < items) {
+          sh_k = x + kXcds * (j / a.B);
+          sh_b = j % a.B;
+          break;
+        }
+        ++sh_q;
+      }
+    }
+    __syncthreads();
+    const uint32_t b = sh_b;
+    if (b == kInf32) break;
+    const uint32_t k = sh_k;
+    const uint32_t s = a.s0 + b;
+    const uint32_t cap = tile_cap_for(a.seg_len[s]);
+    const bool in_lds = cap <= a.lds_slots;
+    const uint32_t lcap = in_lds ? lds_tile_cap(a.seg_len[s], a.lds_slots) : cap;
+    if (!in_lds && tile_words<V>(cap, K1) > gwords) {
+      // arena too small: the host rolls the batch back, grows the arena and
+      // runs the batch again
+      if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, kOvfArena);
+      __syncthreads();
+      continue;
+    }
+    bool built = false;
+    const uint32_t y_end = min(a.n, (k + 1) * kRowBlock);
+    for (uint32_t y0 = k * kRowBlock; y0 < y_end; y0 += kPruneThreads) {
+      // this hub's candidates among [y0, y0 + 1024), compacted into sh_cand
+      if (threadIdx.x == 0) sh_cnt = 0;
+      __syncthreads();
+      const uint32_t y = y0 + threadIdx.x;
+      const uint32_t v = y < y_end ? a.cand_d[(uint64_t)b * a.n + y] : kInf32;
+      const bool cand = y < y_end && (v >> a.slot_shift) == a.tag && !(v & 1u);
+      const unsigned long long m = __ballot(cand);
+      uint32_t base = 0;
+      if (lane == 0 && m) base = atomicAdd(&sh_cnt, (uint32_t)__popcll(m));
+      base = __shfl(base, 0, kWave);
+      if (cand) sh_cand[base + (uint32_t)__popcll(m & lt_mask)] = y;
+      __syncthreads();
+      const uint32_t cnt = sh_cnt;
+      if (cnt == 0) continue;
+      if (!built) {
+        if (in_lds) {
+          if (tile_b != b) {
+            build_tile<V>(a, s, lds, reinterpret_cast<V *>(lds + lcap), lcap);
+            tile_b = b;
+            tile_reads += a.seg_len[s];
+          }
+        } else {
+          build_tile<V>(a, s, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
+          // written through L2 (atomics): drop stale L1 lines of the slice
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+          tile_b = kInf32;
+          tile_reads += a.seg_len[s];
+        }
+        built = true;
+      }
+      for (uint32_t c0 = 0; c0 < cnt; c0 += a.tile) {
+        const uint32_t c1 = min(cnt, c0 + a.tile);
+        // staged labels are published once this chunk's could overflow them
+        if (sh_acc_n + (c1 - c0) > kAccSlots)
+          publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n,
+                           sh_touch_base);
+        if (threadIdx.x == 0) sh_next = 0;
+        __syncthreads();
+        if (in_lds)
+          prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (lds_ptr<const uint32_t>)lds,
+                                  (lds_ptr<const V>)(lds + lcap), lcap, sh_next, sh_acc_n,
+                                  sh_acc, wave_reads);
+        else
+          prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (glb_ptr<const uint32_t>)gkeys,
+                                  (glb_ptr<const V>)(gkeys + cap), cap, sh_next, sh_acc_n,
+                                  sh_acc, wave_reads);
+        __syncthreads();
+      }
+    }
+  }
+  publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
+  if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
+  if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
+}
+
 // -------------------------------------------------------- reserve / append
 __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
   const int lane = lane_id();
@@ -1181,6 +1330,13 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
                              : (a.pool32 ? k_prune<uint32_t, uint32_t>
                                          : k_prune<uint32_t, unsigned long long>);
       hipLaunchKernelGGL(kp, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
+      // row items instead, when k_chunk_scan chose them (one of the two
+      // launches returns at once)
+      auto kr = g.small_vals ? (a.pool32 ? k_prune_rows<uint16_t, uint32_t>
+                                         : k_prune_rows<uint16_t, unsigned long long>)
+                             : (a.pool32 ? k_prune_rows<uint32_t, uint32_t>
+                                         : k_prune_rows<uint32_t, unsigned long long>);
+      hipLaunchKernelGGL(kr, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
     }
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
@@ -1328,6 +1484,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.K = K;
   // accepted candidates of a work item are staged in 256 LDS slots
   a.tile = opts->tile ? std::min<uint32_t>(opts->tile, 256u) : 256u;
+  // tuning knob HSDL_PRUNE_ROWS: 0 never row items, 1 when dense (default),
+  // 2 always
+  a.row_policy = env_u32("HSDL_PRUNE_ROWS", 1, 0, 2);
   a.lds_slots = lds_slots;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);

@@ -118,6 +118,32 @@ def test_prune_label_stream_widths(monkeypatch, heavy13, stream32, slots):
     same_labels(idx, want)
 
 
+@pytest.mark.parametrize("rows", ["2", "0"])
+@pytest.mark.parametrize("slots,hub_batch", [(0, 0), (64, 0), (0, 7)])
+def test_prune_row_items(monkeypatch, heavy13, rows, slots, hub_batch):
+    """Rounds whose candidates are dense are pruned by row items (a hub's
+    dense slot row scanned one block of vertices at a time, from per-XCD
+    queues); HSDL_PRUNE_ROWS=2 takes them in every round, 0 in none.  Over
+    LDS and arena tiles and several batch sizes the labels must not change."""
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    g, want = heavy13
+    idx = H.hop_constrained_two_hop_labels_generation(g, 4, lds_tile_slots=slots,
+                                                      hub_batch=hub_batch)
+    same_labels(idx, want)
+
+
+@pytest.mark.parametrize("rows", ["2", "0"])
+def test_c1_row_items_plain_and_overflow_redo(monkeypatch, c1, rows):
+    """Row items (or none) in the plain one-hub form and through the pool and
+    arena overflow path (rollback, growth, redo): the golden labels."""
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    g, z = c1
+    want = (z["lab_off"], z["hub"], z["hop"], z["dist"])
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, mode=H.MODE_PLAIN), want)
+    same_labels(H.hop_constrained_two_hop_labels_generation(
+        g, 4, hub_batch=512, pool_init_labels=4096 * 16 + 1, arena_init_words=1), want)
+
+
 def test_wide_weights_use_32bit_tiles():
     """K * max weight >= 65535 selects 32-bit tile values."""
     g = H.Graph.rmat(10, 8, 4242, 1, 40000)
