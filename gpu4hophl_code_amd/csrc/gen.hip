@@ -116,6 +116,7 @@ struct GenArgs {
   uint32_t cur;        // frontier buffer read this round
   uint32_t tile;       // candidates per prune work item
   uint32_t row_policy; // prune row items: 0 never, 1 when candidates are dense, 2 always
+  uint32_t row_density;  // "dense": candidates >= batch slots / row_density
   uint32_t lds_slots;  // LDS tile capacity (slots)
   const unsigned long long *off;
   const uint2 *adj;
@@ -451,8 +452,8 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
 
 // ------------------------------------------------------------ chunk scan
 // A round's prune takes row items (k_prune_rows) when its candidates fill at
-// least 1/kRowDensity of the batch's dense slots.
-constexpr uint32_t kRowDensity = 32;
+// least 1/row_density of the batch's dense slots (default 32: the row scan
+// then costs at most 128 B per candidate, against about 1.5 KB of list reads).
 // Row items: XCDs of an MI355X (each has its own L2) and vertices per item.
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kRowBlock = 2048;
@@ -491,7 +492,7 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
   if (t < 8) a.ctrl->row_ctr[t] = 0;
   if (t == blockDim.x - 1) {
     const bool dead = a.ctrl->overflow != 0;
-    // row items when the round's candidates fill at least 1/kRowDensity of
+    // row items when the round's candidates fill at least 1/row_density of
     // the batch's slot rows (or always / never by the policy)
     // (and only while an XCD queue's item count fits its 32-bit counter)
     const uint64_t queue_items =
@@ -499,7 +500,7 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     const bool rows = !dead && partc[t] && queue_items < (1ull << 31) &&
                       (a.row_policy == 2 ||
                        (a.row_policy == 1 &&
-                        (uint64_t)partc[t] * kRowDensity >= (uint64_t)a.B * a.n));
+                        (uint64_t)partc[t] * a.row_density >= (uint64_t)a.B * a.n));
     a.chunk_off[a.B] = part[t];
     a.hub_off[a.B] = partc[t];
     a.ctrl->row_mode = rows ? 1u : 0u;
@@ -1487,6 +1488,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   // tuning knob HSDL_PRUNE_ROWS: 0 never row items, 1 when dense (default),
   // 2 always
   a.row_policy = env_u32("HSDL_PRUNE_ROWS", 1, 0, 2);
+  a.row_density = env_u32("HSDL_ROW_DENSITY", 32, 1, 1u << 20);
   a.lds_slots = lds_slots;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);
