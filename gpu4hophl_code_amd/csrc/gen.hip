@@ -117,6 +117,7 @@ struct GenArgs {
   uint32_t tile;       // candidates per prune work item
   uint32_t row_policy; // prune row items: 0 never, 1 when candidates are dense, 2 always
   uint32_t row_density;  // "dense": candidates >= batch slots / row_density
+  uint32_t row_run;      // y-blocks per row item (one tile build per item)
   uint32_t lds_slots;  // LDS tile capacity (slots)
   const unsigned long long *off;
   const uint2 *adj;
@@ -457,6 +458,7 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
 // Row items: XCDs of an MI355X (each has its own L2) and vertices per item.
 constexpr uint32_t kXcds = 8;
 constexpr uint32_t kRowBlock = 2048;
+static_assert(kRowBlock % 1024 == 0, "row blocks are scanned 1024 vertices at a time");
 
 // One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
 // sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
@@ -932,14 +934,14 @@ k_prune(GenArgs a) {
 
 // --------------------------------------------------------- prune, by rows
 // Row items for rounds whose candidates are dense (k_chunk_scan sets
-// row_mode): item (k, b) is hub slot b's candidates among the vertices
-// [k * kRowBlock, (k + 1) * kRowBlock), found by scanning b's dense slot row
-// there (fresh slots of the running attempt are exactly the round's
-// candidates).  Each XCD has its own queue of y-blocks k = x, x + 8, ...,
-// each with all B hubs in turn, so the B hubs' items of one y-block run at
-// the same time on one XCD and re-read the same lists L(y) from its L2
-// instead of from HBM (hub-ordered items re-read a vertex's list once per
-// hub, spread over the whole round).  Workgroups are dealt to XCDs
+// row_mode): an item is hub slot b's candidates among a run of row_run
+// y-blocks of kRowBlock vertices, found by scanning b's dense slot row there
+// (fresh slots of the running attempt are exactly the round's candidates);
+// one tile build serves the run.  Each XCD has its own queue of y-blocks
+// k = x, x + 8, ..., whose runs come with all B hubs in turn, so the B hubs'
+// items of one run are processed at the same time on one XCD and re-read the
+// same lists L(y) from its L2 instead of from HBM (hub-ordered items re-read
+// a vertex's list once per hub, spread over the whole round).  Workgroups are dealt to XCDs
 // round-robin by id; a workgroup whose queue runs dry takes items from the
 // other queues in turn.  The decisions do not depend on the order: a round's
 // prune reads only labels of earlier rounds and batches.
@@ -947,7 +949,7 @@ template <class V, class LT>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune_rows(GenArgs a) {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t sh_b, sh_k, sh_q, sh_next, sh_cnt;
+  __shared__ uint32_t sh_b, sh_k, sh_kn, sh_q, sh_next, sh_cnt;
   __shared__ uint32_t sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base;
   __shared__ Item sh_acc[kAccSlots];
   __shared__ uint32_t sh_touch[kAccSlots];
@@ -973,12 +975,17 @@ k_prune_rows(GenArgs a) {
     if (threadIdx.x == 0) {
       sh_b = kInf32;
       while (sh_q < kXcds) {
+        // queue x: its blocks k = x + 8i, in runs of row_run blocks; item j
+        // is run j / B of hub slot j % B
         const uint32_t x = (home + sh_q) % kXcds;
         const uint32_t nb = x < nblk ? (nblk - x + kXcds - 1) / kXcds : 0u;
-        const uint64_t items = (uint64_t)nb * a.B;
+        const uint32_t nruns = (nb + a.row_run - 1) / a.row_run;
+        const uint64_t items = (uint64_t)nruns * a.B;
         const uint32_t j = items ? atomicAdd(&a.ctrl->row_ctr[x], 1u) : 0u;
         if (j < items) {
-          sh_k = x + kXcds * (j / a.B);
+          const uint32_t i0 = (j / a.B) * a.row_run;
+          sh_k = x + kXcds * i0;            // first block of the run
+          sh_kn = min(a.row_run, nb - i0);  // blocks in the run
           sh_b = j % a.B;
           break;
         }
@@ -1001,8 +1008,14 @@ k_prune_rows(GenArgs a) {
       continue;
     }
     bool built = false;
-    const uint32_t y_end = min(a.n, (k + 1) * kRowBlock);
-    for (uint32_t y0 = k * kRowBlock; y0 < y_end; y0 += kPruneThreads) {
+    // the run's blocks k, k + 8, ..., 1024 vertices at a time
+    constexpr uint32_t kSubs = kRowBlock / kPruneThreads;
+    const uint32_t kn = sh_kn;
+    for (uint32_t i = 0; i < kn * kSubs; ++i) {
+      const uint32_t kb = k + kXcds * (i / kSubs);
+      const uint32_t y0 = kb * kRowBlock + (i % kSubs) * kPruneThreads;
+      const uint32_t y_end = min(a.n, (kb + 1) * kRowBlock);
+      if (y0 >= y_end) continue;
       // this hub's candidates among [y0, y0 + 1024), compacted into sh_cand
       if (threadIdx.x == 0) sh_cnt = 0;
       __syncthreads();
@@ -1489,6 +1502,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   // 2 always
   a.row_policy = env_u32("HSDL_PRUNE_ROWS", 1, 0, 2);
   a.row_density = env_u32("HSDL_ROW_DENSITY", 32, 1, 1u << 20);
+  a.row_run = env_u32("HSDL_ROW_RUN", 4, 1, 64);
   a.lds_slots = lds_slots;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);

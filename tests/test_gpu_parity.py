@@ -118,14 +118,17 @@ def test_prune_label_stream_widths(monkeypatch, heavy13, stream32, slots):
     same_labels(idx, want)
 
 
-@pytest.mark.parametrize("rows", ["2", "0"])
+@pytest.mark.parametrize("rows,run", [("2", "1"), ("2", "64"), ("2", None), ("0", None)])
 @pytest.mark.parametrize("slots,hub_batch", [(0, 0), (64, 0), (0, 7)])
-def test_prune_row_items(monkeypatch, heavy13, rows, slots, hub_batch):
+def test_prune_row_items(monkeypatch, heavy13, rows, run, slots, hub_batch):
     """Rounds whose candidates are dense are pruned by row items (a hub's
-    dense slot row scanned one block of vertices at a time, from per-XCD
-    queues); HSDL_PRUNE_ROWS=2 takes them in every round, 0 in none.  Over
-    LDS and arena tiles and several batch sizes the labels must not change."""
+    dense slot row scanned over runs of vertex blocks, from per-XCD queues);
+    HSDL_PRUNE_ROWS=2 takes them in every round, 0 in none, HSDL_ROW_RUN sets
+    the blocks per item.  Over LDS and arena tiles and several batch sizes the
+    labels must not change."""
     monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    if run:
+        monkeypatch.setenv("HSDL_ROW_RUN", run)
     g, want = heavy13
     idx = H.hop_constrained_two_hop_labels_generation(g, 4, lds_tile_slots=slots,
                                                       hub_batch=hub_batch)
