@@ -49,6 +49,7 @@ constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vert
 constexpr uint32_t kTileWindow = 4;            // labels of L(v) per work unit (default)
 constexpr uint32_t kMaxWindow = 64;            // largest window
 constexpr uint32_t kTileLdsBytes = 64 * 1024 + 16;  // LDS tile budget (two workgroups per CU)
+constexpr uint32_t kGroupFast = 8;             // group sizes with the unrolled check
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -144,6 +145,18 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     const uint32_t ulen = (uint32_t)(off[u + 1] - ub);
     if (ulen == 0) continue;
     const typename S::T *Lu = str.list(ub);
+    // the group's hops and distances, decoded once per head into scalars for
+    // groups of up to kGroupFast labels (a group holds at most K + 1, one per
+    // hop): the per-element loop over them is then unrolled straight-line
+    // code instead of a scalar load, a wait and a decode per group label
+    const bool fast = glen <= kGroupFast;
+    uint32_t gh[kGroupFast], gd[kGroupFast];
+#pragma unroll
+    for (uint32_t t = 0; t < kGroupFast; ++t) {
+      const unsigned long long g = (fast && t < glen) ? Lv[p + t] : 0ull;
+      gh[t] = label_hop(g);
+      gd[t] = label_dist(g);
+    }
     streamed += ulen;
     const uint32_t last = ulen - 1;
     // one element (w,h1,d1) of L(u) at position j against the group
@@ -156,12 +169,25 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
         if (h1 + 1 <= maxh && d1 <= maxd) {
           const uint32_t slot = tile_slot(keys, cap, w);
           if (slot != kInf32) {
-            for (uint32_t t = 0; t < glen; ++t) {
-              const unsigned long long g = Lv[p + t];
-              const uint32_t hg = label_hop(g);
-              if (hg > h1) {
-                const uint32_t m = vals[mul_u24(slot, K1) + (hg - h1)];
-                if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
+            const uint32_t base = mul_u24(slot, K1);
+            if (fast) {
+#pragma unroll
+              for (uint32_t t = 0; t < kGroupFast; ++t) {
+                if (t >= glen) break;
+                // a label no deeper than h1 reads the slot's first value and
+                // is masked
+                const bool deeper = gh[t] > h1;
+                const uint32_t m = vals[base + (deeper ? gh[t] - h1 : 0u)];
+                hit |= deeper & (m != TileVal<V>::kEmpty) & (d1 + m <= gd[t]);
+              }
+            } else {
+              for (uint32_t t = 0; t < glen; ++t) {
+                const unsigned long long g = Lv[p + t];
+                const uint32_t hg = label_hop(g);
+                if (hg > h1) {
+                  const uint32_t m = vals[base + (hg - h1)];
+                  if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
+                }
               }
             }
           }
