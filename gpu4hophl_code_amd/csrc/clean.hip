@@ -161,38 +161,38 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     const uint32_t last = ulen - 1;
     // one element (w,h1,d1) of L(u) at position j against the group
     auto check = [&](typename S::T l, uint32_t j) {
-      if (j >= ulen) return;
       uint32_t w, h1, d1;
-      str.dec(l, w, h1, d1);
-      if (w < u) {
-        // a pair through w has hop >= h1 + 1 (w != v) and dist >= d1
-        if (h1 + 1 <= maxh && d1 <= maxd) {
-          const uint32_t slot = tile_slot(keys, cap, w);
-          if (slot != kInf32) {
-            const uint32_t base = mul_u24(slot, K1);
-            if (fast) {
+      str.dec(l, w, h1, d1);  // a lane past the list's end holds a copy of its last element
+      const bool in = j < ulen;
+      // a pair through w < u has hop >= h1 + 1 (w != v) and dist >= d1; the
+      // conditions are combined into one branch (one exec-mask change)
+      if (in & (w < u) & (h1 + 1 <= maxh) & (d1 <= maxd)) {
+        const uint32_t slot = tile_slot(keys, cap, w);
+        if (slot != kInf32) {
+          const uint32_t base = mul_u24(slot, K1);
+          if (fast) {
 #pragma unroll
-              for (uint32_t t = 0; t < kGroupFast; ++t) {
-                if (t >= glen) break;
-                // a label no deeper than h1 reads the slot's first value and
-                // is masked
-                const bool deeper = gh[t] > h1;
-                const uint32_t m = vals[base + (deeper ? gh[t] - h1 : 0u)];
-                hit |= deeper & (m != TileVal<V>::kEmpty) & (d1 + m <= gd[t]);
-              }
-            } else {
-              for (uint32_t t = 0; t < glen; ++t) {
-                const unsigned long long g = Lv[p + t];
-                const uint32_t hg = label_hop(g);
-                if (hg > h1) {
-                  const uint32_t m = vals[base + (hg - h1)];
-                  if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
-                }
+            for (uint32_t t = 0; t < kGroupFast; ++t) {
+              if (t >= glen) break;
+              // a label no deeper than h1 reads the slot's first value and
+              // is masked
+              const bool deeper = gh[t] > h1;
+              const uint32_t m = vals[base + (deeper ? gh[t] - h1 : 0u)];
+              hit |= deeper & (m != TileVal<V>::kEmpty) & (d1 + m <= gd[t]);
+            }
+          } else {
+            for (uint32_t t = 0; t < glen; ++t) {
+              const unsigned long long g = Lv[p + t];
+              const uint32_t hg = label_hop(g);
+              if (hg > h1) {
+                const uint32_t m = vals[base + (hg - h1)];
+                if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
               }
             }
           }
         }
-      } else if (w == u) {
+      }
+      if (in & (w == u)) {
         // through hub u: earlier labels of the group against later ones
         for (uint32_t t1 = 0; t1 < glen; ++t1) {
           const unsigned long long g1 = Lv[p + t1];
