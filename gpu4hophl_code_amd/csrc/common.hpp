@@ -21,6 +21,11 @@ struct hsdl_graph {
   // allocations that followed)
   void *d_slots = nullptr;
   size_t d_slots_bytes = 0;
+  // the last build's label pool and its 32-bit copy, kept for the next build
+  // on this graph for the same reason (and so that build starts at the size
+  // the pool grew to)
+  void *d_pool = nullptr, *d_pool32 = nullptr;
+  size_t d_pool_bytes = 0, d_pool32_bytes = 0;
   int d_device = -1;
   void (*d_free)(hsdl_graph *) = nullptr;
   ~hsdl_graph() {

@@ -1178,9 +1178,11 @@ static void free_device_graph(hsdl_graph *g) {
   if (g->d_off) (void)hipFree(g->d_off);
   if (g->d_adj) (void)hipFree(g->d_adj);
   if (g->d_slots) (void)hipFree(g->d_slots);
+  if (g->d_pool) (void)hipFree(g->d_pool);
+  if (g->d_pool32) (void)hipFree(g->d_pool32);
   if (cur >= 0 && cur != g->d_device) (void)hipSetDevice(cur);
-  g->d_off = g->d_adj = g->d_slots = nullptr;
-  g->d_slots_bytes = 0;
+  g->d_off = g->d_adj = g->d_slots = g->d_pool = g->d_pool32 = nullptr;
+  g->d_slots_bytes = g->d_pool_bytes = g->d_pool32_bytes = 0;
   g->d_device = -1;
 }
 
@@ -1411,13 +1413,13 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
 
   size_t free_b = 0, total_b = 0;
   HSDL_HIP(hipMemGetInfo(&free_b, &total_b));
-  // dense slots kept from an earlier build on this graph count as free: this
-  // build reuses or replaces them
+  // dense slots and label pools kept from an earlier build on this graph
+  // count as free: this build reuses or replaces them
   hsdl_graph *mg = const_cast<hsdl_graph *>(hg);
-  free_b += mg->d_slots_bytes;
+  free_b += mg->d_slots_bytes + mg->d_pool_bytes + mg->d_pool32_bytes;
   g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
   const uint32_t c0 = 16;  // initial label slots per vertex
-  const uint64_t pool0 = opts->pool_init_labels
+  uint64_t pool0 = opts->pool_init_labels
                              ? std::max<uint64_t>(opts->pool_init_labels, (uint64_t)n * c0)
                              : std::max<uint64_t>((uint64_t)n * c0 * 4, 64ull << 20);
   const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
@@ -1453,10 +1455,36 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const bool use32 = env_u32("HSDL_STREAM32", 1, 0, 1) &&
                      p32_hub_bits + p32_hop_bits + p32_dist_bits <= 32;
   const uint64_t per_label = use32 ? 12 : 8;
+  // the pools kept from the previous build on this graph are taken over when
+  // no initial pool size is forced and they fit the budget (no allocation,
+  // and no growth if this build needs no more); otherwise they are freed
+  {
+    const uint64_t kept = mg->d_pool_bytes / 8;
+    const bool adopt = !opts->pool_init_labels && mg->d_pool && kept >= pool0 &&
+                       (!use32 || (mg->d_pool32 && mg->d_pool32_bytes >= kept * 4)) &&
+                       g.fixed + kept * per_label + gtile0 * 4 + (uint64_t)list0 * 36 <= g.budget;
+    if (adopt) {
+      pool0 = kept;
+      g.pool.p = mg->d_pool;
+      g.pool.bytes = mg->d_pool_bytes;
+      mg->d_pool = nullptr;
+      mg->d_pool_bytes = 0;
+      if (use32) {
+        g.pool32.p = mg->d_pool32;
+        g.pool32.bytes = mg->d_pool32_bytes;
+        mg->d_pool32 = nullptr;
+        mg->d_pool32_bytes = 0;
+      }
+    }
+    if (mg->d_pool) (void)hipFree(mg->d_pool);
+    if (mg->d_pool32) (void)hipFree(mg->d_pool32);
+    mg->d_pool = mg->d_pool32 = nullptr;
+    mg->d_pool_bytes = mg->d_pool32_bytes = 0;
+  }
   if (g.budget < g.fixed + pool0 * per_label + gtile0 * 4 + (uint64_t)list0 * 36)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
-  if ((rc = g.pool.alloc(pool0 * 8)) || (rc = g.seg_start.alloc(n * 8ull)) ||
+  if ((!g.pool.p && (rc = g.pool.alloc(pool0 * 8))) || (rc = g.seg_start.alloc(n * 8ull)) ||
       (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
       (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
       (rc = g.touched.alloc(n * 4ull)) ||
@@ -1464,7 +1492,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))) ||
       (rc = g.heavy.alloc((uint64_t)heavy_cap * sizeof(HeavyItem))) ||
-      (use32 && (rc = g.pool32.alloc(pool0 * 4))))
+      (use32 && !g.pool32.p && (rc = g.pool32.alloc(pool0 * 4))))
     return rc;
   {
     const size_t need = (size_t)Bmax * n * 4;
@@ -1678,10 +1706,16 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   hipLaunchKernelGGL(k_gather, dim3(g.grid), dim3(256), 0, g.st, a,
                      idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>());
   if (hipStreamSynchronize(g.st) != hipSuccess) return cleanup(fail(HSDL_EDEVICE, "gather failed"));
-  g.pool.release();
   const uint64_t t_sort0 = now_ns();
-  if ((rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st)))
-    return cleanup(rc);
+  // the pools stay for the graph's next build; they are released first only
+  // when the sort cannot get its buffers otherwise
+  rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st);
+  if (rc == HSDL_ENOMEM) {
+    g.pool.release();
+    g.pool32.release();
+    rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st);
+  }
+  if (rc) return cleanup(rc);
   const uint64_t t_sorted = now_ns();
   // parents (optional): derived from the sorted uncleaned labels and the graph
   if (opts->parents && (rc = derive_parents(idx, hg, g.st))) return cleanup(rc);
@@ -1709,6 +1743,17 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   idx->stats.prune_label_reads = c.label_reads;
   idx->stats.prune_tile_reads = c.tile_reads;
   idx->stats.prune_label_bytes = a.pool32 ? 4 : 8;
+  // the pools stay with the graph for its next build (a pool released for
+  // the sort leaves nothing to keep: the next build allocates anew)
+  if (!g.pool.p) g.pool32.release();
+  mg->d_pool = g.pool.p;
+  mg->d_pool_bytes = g.pool.bytes;
+  g.pool.p = nullptr;
+  g.pool.bytes = 0;
+  mg->d_pool32 = g.pool32.p;
+  mg->d_pool32_bytes = g.pool32.bytes;
+  g.pool32.p = nullptr;
+  g.pool32.bytes = 0;
   *out = idx;
   return HSDL_OK;
 }
