@@ -96,9 +96,10 @@ int hsdl_graph_grid(uint32_t rows, uint32_t cols, uint64_t seed, int32_t wmin,
  * hsdl_generate uploads it itself otherwise.  Freed with the graph. */
 int hsdl_graph_upload(hsdl_graph *g, int32_t device);
 /* Frees the graph's device state: the CSR copy and the dense dedupe slots
- * that hsdl_generate keeps with it for the next build (hub batch x n x 4 B,
- * up to a quarter of the build's HBM budget).  hsdl_free_graph does the
- * same.  (No counterpart in the reference.) */
+ * and label pool that hsdl_generate keeps with it for the next build (hub
+ * batch x n x 4 B, up to a quarter of the build's HBM budget; 8-12 B per
+ * label slot).  hsdl_free_graph does the same.  (No counterpart in the
+ * reference.) */
 int hsdl_graph_release_device(hsdl_graph *g);
 
 /* n and stored adjacency entries (2*undirected edges, self loops once). */
@@ -143,9 +144,9 @@ void hsdl_opts_init(hsdl_opts *opts);
  * hop_constrained_two_hop_labels_generation before its canonical repair
  * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:513-604).
  * The index stays resident in HBM.  K must be 1..15.  The dense dedupe slots
- * stay with the graph's device copy for the next build on this graph
- * (hsdl_graph_release_device frees them); calls on one graph must not
- * overlap. */
+ * and the label pool stay with the graph's device copy for the next build
+ * on this graph (hsdl_graph_release_device frees them); calls on one graph
+ * must not overlap. */
 int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts, hsdl_index **out);
 
 /* Canonical repair in place, on the GPU.  Replaces hop_constrained_clean_L
