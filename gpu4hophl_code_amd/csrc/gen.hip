@@ -499,9 +499,11 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     // (and only while an XCD queue's item count fits its 32-bit counter)
     const uint64_t queue_items =
         (uint64_t)a.B * (((uint64_t)a.n + kRowBlock * kXcds - 1) / (kRowBlock * kXcds));
+    // (a batch of one hub has no list reads to share: the plain form never
+    // takes row items unless forced)
     const bool rows = !dead && partc[t] && queue_items < (1ull << 31) &&
                       (a.row_policy == 2 ||
-                       (a.row_policy == 1 &&
+                       (a.row_policy == 1 && a.B > 1 &&
                         (uint64_t)partc[t] * a.row_density >= (uint64_t)a.B * a.n));
     a.chunk_off[a.B] = part[t];
     a.hub_off[a.B] = partc[t];
@@ -1347,12 +1349,15 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
                                          : k_prune<uint32_t, unsigned long long>);
       hipLaunchKernelGGL(kp, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
       // row items instead, when k_chunk_scan chose them (one of the two
-      // launches returns at once)
-      auto kr = g.small_vals ? (a.pool32 ? k_prune_rows<uint16_t, uint32_t>
-                                         : k_prune_rows<uint16_t, unsigned long long>)
-                             : (a.pool32 ? k_prune_rows<uint32_t, uint32_t>
-                                         : k_prune_rows<uint32_t, unsigned long long>);
-      hipLaunchKernelGGL(kr, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
+      // launches returns at once); never for a one-hub batch unless forced,
+      // so the plain form pays no second launch per round
+      if (B > 1 || a.row_policy == 2) {
+        auto kr = g.small_vals ? (a.pool32 ? k_prune_rows<uint16_t, uint32_t>
+                                           : k_prune_rows<uint16_t, unsigned long long>)
+                               : (a.pool32 ? k_prune_rows<uint32_t, uint32_t>
+                                           : k_prune_rows<uint32_t, unsigned long long>);
+        hipLaunchKernelGGL(kr, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
+      }
     }
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
