@@ -175,3 +175,28 @@ def test_generation_needs_gpu_no_cpu_fallback():
     with pytest.raises(H.HSDLError) as e:
         H.hop_constrained_two_hop_labels_generation(g, 3)
     assert e.value.code == -8
+
+
+@pytest.mark.parametrize("name", ["hsdl_opts", "hsdl_stats", "hsdl_server_opts",
+                                  "hsdl_server_stats"])
+def test_ctypes_structs_match_the_header(tmp_path, name):
+    """The Python mirrors of the ABI structs have the C header's size and
+    field offsets (a C program compiled against include/hsdl.h prints them)."""
+    import ctypes
+    import subprocess
+    st = getattr(H, name)
+    fields = [f for f, _ in st._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "hsdl.h"', "int main(void) {",
+             f'  printf("size %zu\\n", sizeof({name}));']
+    lines += [f'  printf("{f} %zu\\n", offsetof({name}, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src),
+                           "-o", str(exe)])
+    out = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    assert int(out["size"]) == ctypes.sizeof(st)
+    for f in fields:
+        assert int(out[f]) == getattr(st, f).offset, f
+
