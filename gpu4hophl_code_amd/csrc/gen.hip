@@ -118,6 +118,7 @@ struct GenArgs {
   uint32_t row_policy; // prune row items: 0 never, 1 when candidates are dense, 2 always
   uint32_t row_density;  // "dense": candidates >= batch slots / row_density
   uint32_t row_run;      // y-blocks per row item (one tile build per item)
+  uint32_t row_block;    // vertices per y-block (a multiple of 1024)
   uint32_t lds_slots;  // LDS tile capacity (slots)
   const unsigned long long *off;
   const uint2 *adj;
@@ -457,8 +458,8 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
 // then costs at most 128 B per candidate, against about 1.5 KB of list reads).
 // Row items: XCDs of an MI355X (each has its own L2) and vertices per item.
 constexpr uint32_t kXcds = 8;
-constexpr uint32_t kRowBlock = 2048;
-static_assert(kRowBlock % 1024 == 0, "row blocks are scanned 1024 vertices at a time");
+// (vertices per row block: a.row_block, a multiple of the 1024 a workgroup
+// scans at a time)
 
 // One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
 // sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
@@ -498,7 +499,8 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
     // the batch's slot rows (or always / never by the policy)
     // (and only while an XCD queue's item count fits its 32-bit counter)
     const uint64_t queue_items =
-        (uint64_t)a.B * (((uint64_t)a.n + kRowBlock * kXcds - 1) / (kRowBlock * kXcds));
+        (uint64_t)a.B * (((uint64_t)a.n + (uint64_t)a.row_block * kXcds - 1) /
+                         ((uint64_t)a.row_block * kXcds));
     // (a batch of one hub has no list reads to share: the plain form never
     // takes row items unless forced)
     const bool rows = !dead && partc[t] && queue_items < (1ull << 31) &&
@@ -937,7 +939,7 @@ k_prune(GenArgs a) {
 // --------------------------------------------------------- prune, by rows
 // Row items for rounds whose candidates are dense (k_chunk_scan sets
 // row_mode): an item is hub slot b's candidates among a run of row_run
-// y-blocks of kRowBlock vertices, found by scanning b's dense slot row there
+// y-blocks of row_block vertices, found by scanning b's dense slot row there
 // (fresh slots of the running attempt are exactly the round's candidates);
 // one tile build serves the run.  Each XCD has its own queue of y-blocks
 // k = x, x + 8, ..., whose runs come with all B hubs in turn, so the B hubs'
@@ -964,7 +966,7 @@ k_prune_rows(GenArgs a) {
   unsigned long long wave_reads = 0, tile_reads = 0;
   const uint64_t gwords = a.gtile_cap / gridDim.x;
   uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
-  const uint32_t nblk = (a.n + kRowBlock - 1) / kRowBlock;
+  const uint32_t nblk = (uint32_t)(((uint64_t)a.n + a.row_block - 1) / a.row_block);
   const uint32_t home = blockIdx.x % kXcds;
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   if (threadIdx.x == 0) {
@@ -1011,12 +1013,12 @@ k_prune_rows(GenArgs a) {
     }
     bool built = false;
     // the run's blocks k, k + 8, ..., 1024 vertices at a time
-    constexpr uint32_t kSubs = kRowBlock / kPruneThreads;
+    const uint32_t kSubs = a.row_block / kPruneThreads;
     const uint32_t kn = sh_kn;
     for (uint32_t i = 0; i < kn * kSubs; ++i) {
       const uint32_t kb = k + kXcds * (i / kSubs);
-      const uint32_t y0 = kb * kRowBlock + (i % kSubs) * kPruneThreads;
-      const uint32_t y_end = min(a.n, (kb + 1) * kRowBlock);
+      const uint32_t y0 = kb * a.row_block + (i % kSubs) * kPruneThreads;
+      const uint32_t y_end = min(a.n, (kb + 1) * a.row_block);
       if (y0 >= y_end) continue;
       // this hub's candidates among [y0, y0 + 1024), compacted into sh_cand
       if (threadIdx.x == 0) sh_cnt = 0;
@@ -1536,6 +1538,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   a.row_policy = env_u32("HSDL_PRUNE_ROWS", 1, 0, 2);
   a.row_density = env_u32("HSDL_ROW_DENSITY", 32, 1, 1u << 20);
   a.row_run = env_u32("HSDL_ROW_RUN", 4, 1, 64);
+  // tuning knob HSDL_ROW_BLOCK: vertices per y-block, rounded to 1024
+  a.row_block = ((env_u32("HSDL_ROW_BLOCK", 2048, 1024, 1u << 20) + 1023) / 1024) * 1024;
   a.lds_slots = lds_slots;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);
