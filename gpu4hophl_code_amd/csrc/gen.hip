@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
 // A round's prune takes row items (k_prune_rows) when its candidates fill at
 // least 1/row_density of the batch's dense slots (default 32: the row scan
 // then costs at most 128 B per candidate, against about 1.5 KB of list reads).
-// Row items: XCDs of an MI355X (each has its own L2) and vertices per item.
+// Row items: XCDs of an MI355X (each has its own L2).
 constexpr uint32_t kXcds = 8;
 // (vertices per row block: a.row_block, a multiple of the 1024 a workgroup
 // scans at a time)
@@ -945,9 +945,9 @@ k_prune(GenArgs a) {
 // k = x, x + 8, ..., whose runs come with all B hubs in turn, so the B hubs'
 // items of one run are processed at the same time on one XCD and re-read the
 // same lists L(y) from its L2 instead of from HBM (hub-ordered items re-read
-// a vertex's list once per hub, spread over the whole round).  Workgroups are dealt to XCDs
-// round-robin by id; a workgroup whose queue runs dry takes items from the
-// other queues in turn.  The decisions do not depend on the order: a round's
+// a vertex's list once per hub, spread over the whole round).  Workgroups
+// are dealt to XCDs round-robin by id; a workgroup whose queue runs dry takes
+// items from the other queues in turn.  The decisions do not depend on the order: a round's
 // prune reads only labels of earlier rounds and batches.
 template <class V, class LT>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
