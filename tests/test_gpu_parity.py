@@ -104,14 +104,17 @@ def test_list_and_heavy_overflow_paths(monkeypatch, heavy13, list_init, heavy_ca
     same_labels(idx, want)
 
 
+@pytest.mark.parametrize("rows", ["1", "2"])
 @pytest.mark.parametrize("stream32", ["1", "0"])
 @pytest.mark.parametrize("slots", [0, 64])
-def test_prune_label_stream_widths(monkeypatch, heavy13, stream32, slots):
+def test_prune_label_stream_widths(monkeypatch, heavy13, stream32, slots, rows):
     """The prune streams L(y) from the pool's 32-bit copy when hub, hop and
     distance fields fit one word (here 13 + 3 + 9 bits) and from the 8-byte
     pool otherwise (HSDL_STREAM32=0 forces it), over LDS tiles and over arena
-    tiles (lds_tile_slots=64).  The labels must not change."""
+    tiles (lds_tile_slots=64), with row items where the candidates are dense
+    and in every round (HSDL_PRUNE_ROWS=2).  The labels must not change."""
     monkeypatch.setenv("HSDL_STREAM32", stream32)
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
     g, want = heavy13
     idx = H.hop_constrained_two_hop_labels_generation(g, 4, lds_tile_slots=slots)
     assert idx.stats()["prune_label_bytes"] == (4 if stream32 == "1" else 8)
@@ -147,8 +150,11 @@ def test_c1_row_items_plain_and_overflow_redo(monkeypatch, c1, rows):
         g, 4, hub_batch=512, pool_init_labels=4096 * 16 + 1, arena_init_words=1), want)
 
 
-def test_wide_weights_use_32bit_tiles():
-    """K * max weight >= 65535 selects 32-bit tile values."""
+@pytest.mark.parametrize("rows", ["1", "2"])
+def test_wide_weights_use_32bit_tiles(monkeypatch, rows):
+    """K * max weight >= 65535 selects 32-bit tile values (list items, and
+    row items in every round with HSDL_PRUNE_ROWS=2)."""
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
     g = H.Graph.rmat(10, 8, 4242, 1, 40000)
     c = csr_of(g)
     for slots in (0, 64):
