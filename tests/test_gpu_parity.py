@@ -178,6 +178,29 @@ def test_slot_tags_wrap_with_wide_weights(mode, hub_batch):
     same_labels(idx, want.arrays())
 
 
+@pytest.fixture(scope="module")
+def rmat15():
+    """R-MAT scale 15, edge factor 16, K=5 (32,768 vertices, the CPU-baseline
+    sample of the bench) and the oracle's uncleaned and cleaned label sets."""
+    g = H.Graph.rmat(15, 16, 0x5EED0002, 1, 100)
+    threads = min(16, os.cpu_count() or 4)
+    lab = O.kround(csr_of(g), 5, threads=threads)
+    return g, lab.arrays(), O.clean(lab, 1, threads=threads).arrays()
+
+
+@pytest.mark.parametrize("rows", ["1", "2"])
+def test_rmat15_k5_labels_match_oracle(monkeypatch, rmat15, rows):
+    """The full label set at R-MAT scale 15, K=5 (about 9 million labels):
+    the batched build and its clean are bit-exact against the oracle, with
+    row items where a round's candidates are dense and in every round."""
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    g, want, want_clean = rmat15
+    idx = H.hop_constrained_two_hop_labels_generation(g, 5)
+    same_labels(idx, want)
+    idx.clean()
+    same_labels(idx, want_clean)
+
+
 def test_config2_full_size_queries_match_bellman_ford():
     """Config 2 at its full size (R-MAT scale 18, edge factor 16, K=5, the
     bench workload): the batched build's cleaned index answers sampled
