@@ -200,3 +200,27 @@ def test_ctypes_structs_match_the_header(tmp_path, name):
     for f in fields:
         assert int(out[f]) == getattr(st, f).offset, f
 
+
+
+def test_argument_errors_come_before_the_device():
+    """Bad arguments are refused before any device work, with the codes the
+    header documents (so these run without a GPU): K outside 1..15, an empty
+    graph, K · max weight beyond the 27-bit label distance, and edge lists
+    with an endpoint out of range or a weight below 1."""
+    one = lambda a: np.asarray(a)
+    g = H.Graph.from_edges(4, one([0, 1]), one([1, 2]), one([3, 5]))
+    for K in (0, 16):
+        with pytest.raises(H.HSDLError) as e:
+            H.hop_constrained_two_hop_labels_generation(g, K)
+        assert e.value.code == -1
+    with pytest.raises(H.HSDLError) as e:
+        H.hop_constrained_two_hop_labels_generation(H.Graph.from_edges(0, [], [], []), 3)
+    assert e.value.code == -1
+    wide = H.Graph.from_edges(3, one([0]), one([1]), one([(1 << 26) + 1]))
+    with pytest.raises(H.HSDLError) as e:
+        H.hop_constrained_two_hop_labels_generation(wide, 2)
+    assert e.value.code == -7
+    for u, v, w in (([0], [4], [1]), ([4], [0], [1]), ([0], [1], [0]), ([0], [1], [-3])):
+        with pytest.raises(H.HSDLError) as e:
+            H.Graph.from_edges(4, one(u), one(v), one(w))
+        assert e.value.code == -1

@@ -522,3 +522,48 @@ def test_parents_survive_save_and_load(tmp_path):
     plain.save(str(p))
     with pytest.raises(H.HSDLError):
         H.Index.load(str(p), 4).parents()
+
+
+def degenerate_graphs():
+    """Small graphs at the edges of the input space: one vertex, vertices
+    without edges, a star whose centre's 600 edges make the expansion cut
+    heavy pieces (and 50 isolated vertices: vertex count not a multiple of any
+    block), and two disjoint weighted paths at the largest K."""
+    rng = np.random.default_rng(0x0DD)
+    yield "one_vertex", H.Graph.from_edges(1, [], [], []), (1, 5)
+    yield "isolated", H.Graph.from_edges(9, [], [], []), (1, 3)
+    leaves = np.arange(1, 601)
+    yield "star", H.Graph.from_edges(651, np.zeros(600), leaves, rng.integers(1, 10, 600)), (1, 2, 3)
+    u = np.concatenate([np.arange(0, 39), np.arange(40, 79)])
+    yield "two_paths", H.Graph.from_edges(80, u, u + 1, rng.integers(1, 51, len(u))), (15,)
+
+
+@pytest.mark.parametrize("rows", ["1", "2"])
+@pytest.mark.parametrize("mode", [H.MODE_BATCHED, H.MODE_PLAIN])
+def test_degenerate_graphs_match_oracle(monkeypatch, mode, rows):
+    """Labels, clean and queries on the graphs above, batched and plain, with
+    row items allowed and forced (a single, partial vertex block): equal to
+    the oracle; queries over every pair at every hop bound 0..K+1, or a
+    random sample for the star."""
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    rng = np.random.default_rng(5)
+    for name, g, Ks in degenerate_graphs():
+        g.reorder_by_degree()
+        c = csr_of(g)
+        n = c.n
+        for K in Ks:
+            print(f"[degenerate] {name} n={n} K={K} mode={mode} rows={rows}", flush=True)
+            want = O.kround(c, K)
+            idx = H.hop_constrained_two_hop_labels_generation(g, K, mode=mode)
+            same_labels(idx, want.arrays())
+            idx.clean()
+            cl = O.clean(want)
+            same_labels(idx, cl.arrays())
+            if n <= 100:
+                s, t, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(K + 2), indexing="ij")
+                s, t, k = s.ravel(), t.ravel(), k.ravel()
+            else:
+                s, t = rng.integers(0, n, 20000), rng.integers(0, n, 20000)
+                k = rng.integers(0, K + 2, 20000)
+            got = H.hop_constrained_extract_distance(idx, s, t, k)
+            assert np.array_equal(got, O.query(cl, s, t, k)), (name, K)
