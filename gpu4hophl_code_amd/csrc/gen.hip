@@ -1584,7 +1584,13 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
                      kInf32);
   HSDL_HIP(hipGetLastError());
 
-  BatchTuner tuner{plain ? 1u : std::min<uint32_t>(Bmax, 64u), Bmax, 0.025};
+  // HSDL_BATCH_TARGET_US: the time the tuner aims one batch at (default 50 ms:
+  // R-MAT scale 20 K=5 generates in 5.11 s against 5.71 s at 25 ms and 6.74 s
+  // at 12.5 ms; scale 18 takes the same at 25 and 50 ms, longer at 100 ms);
+  // HSDL_HUB_BATCH_MAX: the largest batch it may choose (default: the slots')
+  const uint32_t tune_cap = env_u32("HSDL_HUB_BATCH_MAX", Bmax, 1, Bmax);
+  const double tune_target = env_u32("HSDL_BATCH_TARGET_US", 50000, 100, 10000000) * 1e-6;
+  BatchTuner tuner{plain ? 1u : std::min<uint32_t>(tune_cap, 64u), tune_cap, tune_target};
   if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
   uint64_t batches = 0, redos = 0;
   uint32_t s0 = 0;
