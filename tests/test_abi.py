@@ -224,3 +224,38 @@ def test_argument_errors_come_before_the_device():
         with pytest.raises(H.HSDLError) as e:
             H.Graph.from_edges(4, one(u), one(v), one(w))
         assert e.value.code == -1
+
+
+def test_label_import_and_handle_errors(tmp_path):
+    """Imported labels are validated on the host before any device work
+    (offsets, hub range, hop ≤ K, 27-bit distances), missing files give
+    HSDL_EIO, and null handles HSDL_EINVAL, all without a GPU."""
+    import ctypes
+    L = H.lib()
+    off = np.array([0, 1, 3], np.uint64)
+    hub = np.array([0, 0, 1], np.uint32)
+    hop = np.array([0, 1, 0], np.uint8)
+    dist = np.array([0, 4, 0], np.int32)
+
+    def code(fn, *a):
+        with pytest.raises(H.HSDLError) as e:
+            fn(*a)
+        return e.value.code
+
+    I = H.Index.from_labels
+    assert code(I, 2, 0, off, hub, hop, dist) == -1                                   # K = 0
+    assert code(I, 2, 16, off, hub, hop, dist) == -1                                  # K > 15
+    assert code(I, 2, 3, np.array([1, 1, 3], np.uint64), hub, hop, dist) == -1        # off[0] != 0
+    assert code(I, 2, 3, np.array([0, 2, 1], np.uint64), hub, hop, dist) == -1        # decreasing
+    assert code(I, 2, 3, off, np.array([0, 2, 1], np.uint32), hop, dist) == -1        # hub >= n
+    assert code(I, 2, 1, off, hub, np.array([0, 2, 0], np.uint8), dist) == -7         # hop > K
+    assert code(I, 2, 3, off, hub, hop, np.array([0, -1, 0], np.int32)) == -7         # dist < 0
+    assert code(I, 2, 3, off, hub, hop, np.array([0, 1 << 27, 0], np.int32)) == -7    # dist >= 2^27
+    assert code(H.Index.load, str(tmp_path / "missing"), 3) == -3
+    assert code(H.Graph.load_txt, str(tmp_path / "missing.txt")) == -3
+    nul = ctypes.c_void_p()
+    assert L.hsdl_clean(None) == -1
+    u32, i8, i32 = np.zeros(1, np.uint32), np.zeros(1, np.int8), np.zeros(1, np.int32)
+    assert L.hsdl_query(None, 1, u32, u32, i8, i32) == -1
+    assert L.hsdl_extract_paths(None, 1, u32, u32, i8, 0, None, u32) == -1
+    assert L.hsdl_server_create(None, None, ctypes.byref(nul)) == -1
