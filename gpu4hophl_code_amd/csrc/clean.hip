@@ -611,7 +611,11 @@ int clean_index(hsdl_index *idx) {
 
 }  // namespace hsdl
 
-extern "C" int hsdl_clean(hsdl_index *idx) {
+static int clean_body(hsdl_index *idx) {
   if (!idx) return hsdl::fail(HSDL_EINVAL, "null index");
   return hsdl::clean_index(idx);
+}
+
+extern "C" int hsdl_clean(hsdl_index *idx) {
+  return hsdl::guarded([&] { return clean_body(idx); });
 }

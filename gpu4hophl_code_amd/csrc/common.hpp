@@ -2,6 +2,8 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -42,6 +44,23 @@ namespace hsdl {
 // Thread-local error detail for hsdl_last_error().
 void set_error(const std::string &msg);
 int fail(int code, const std::string &msg);
+
+// Runs the body of a C ABI entry point: a host allocation failure
+// (std::bad_alloc, e.g. a vector sized from a corrupt file) or any other C++
+// exception becomes a status code, since nothing may unwind through the ABI
+// into a C, ctypes or cgo caller.
+template <class F>
+int guarded(F &&body) noexcept {
+  try {
+    return body();
+  } catch (const std::bad_alloc &) {
+    return fail(HSDL_ENOMEM, "host allocation failed");
+  } catch (const std::exception &e) {
+    return fail(HSDL_EINVAL, std::string("host error: ") + e.what());
+  } catch (...) {
+    return fail(HSDL_EINVAL, "unexpected host error");
+  }
+}
 
 // Wall clock in ns (CLOCK_MONOTONIC).
 uint64_t now_ns();

@@ -1781,7 +1781,7 @@ extern "C" int hsdl_graph_release_device(hsdl_graph *g) {
   return HSDL_OK;
 }
 
-extern "C" int hsdl_graph_upload(hsdl_graph *g, int32_t device) {
+static int graph_upload_body(hsdl_graph *g, int32_t device) {
   if (!g) return hsdl::fail(HSDL_EINVAL, "null graph");
   int dev = 0;
   int rc = hsdl::use_device(device, &dev);
@@ -1789,7 +1789,11 @@ extern "C" int hsdl_graph_upload(hsdl_graph *g, int32_t device) {
   return hsdl::upload_graph(g, dev);
 }
 
-extern "C" int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
+extern "C" int hsdl_graph_upload(hsdl_graph *g, int32_t device) {
+  return hsdl::guarded([&] { return graph_upload_body(g, device); });
+}
+
+static int generate_body(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
                              hsdl_index **out) {
   if (!g || !out) return hsdl::fail(HSDL_EINVAL, "null argument");
   hsdl_opts o;
@@ -1798,4 +1802,9 @@ extern "C" int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *op
   else
     hsdl_opts_init(&o);
   return hsdl::generate(g, K, &o, out);
+}
+
+extern "C" int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
+                             hsdl_index **out) {
+  return hsdl::guarded([&] { return generate_body(g, K, opts, out); });
 }

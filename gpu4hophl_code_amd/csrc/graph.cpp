@@ -199,7 +199,7 @@ const char *hsdl_strerror(int code) {
   }
 }
 
-int hsdl_graph_from_edges(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
+static int graph_from_edges_body(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
                           const int32_t *w, uint32_t flags, hsdl_graph **out) {
   if (!out || (m && (!u || !v || !w))) return fail(HSDL_EINVAL, "null argument");
   hsdl_graph *g = new hsdl_graph();
@@ -210,6 +210,11 @@ int hsdl_graph_from_edges(uint32_t n, uint64_t m, const uint32_t *u, const uint3
   }
   *out = g;
   return HSDL_OK;
+}
+
+int hsdl_graph_from_edges(uint32_t n, uint64_t m, const uint32_t *u, const uint32_t *v,
+                          const int32_t *w, uint32_t flags, hsdl_graph **out) {
+  return hsdl::guarded([&] { return graph_from_edges_body(n, m, u, v, w, flags, out); });
 }
 
 static bool parse_int(const char *&p, long long &x) {
@@ -230,7 +235,7 @@ static bool parse_double(const char *&p, double &x) {
   return true;
 }
 
-int hsdl_graph_load_txt(const char *path, hsdl_graph **out) {
+static int graph_load_txt_body(const char *path, hsdl_graph **out) {
   if (!path || !out) return fail(HSDL_EINVAL, "null argument");
   std::ifstream in(path);
   if (!in.is_open()) return fail(HSDL_EIO, std::string("cannot open ") + path);
@@ -261,7 +266,11 @@ int hsdl_graph_load_txt(const char *path, hsdl_graph **out) {
   return hsdl_graph_from_edges((uint32_t)n, us.size(), us.data(), vs.data(), ws.data(), 0, out);
 }
 
-int hsdl_graph_save_txt(const hsdl_graph *g, const char *path) {
+int hsdl_graph_load_txt(const char *path, hsdl_graph **out) {
+  return hsdl::guarded([&] { return graph_load_txt_body(path, out); });
+}
+
+static int graph_save_txt_body(const hsdl_graph *g, const char *path) {
   if (!g || !path) return fail(HSDL_EINVAL, "null argument");
   FILE *f = fopen(path, "w");
   if (!f) return fail(HSDL_EIO, std::string("cannot open ") + path);
@@ -277,7 +286,11 @@ int hsdl_graph_save_txt(const hsdl_graph *g, const char *path) {
   return fclose(f) == 0 ? HSDL_OK : fail(HSDL_EIO, "write failed");
 }
 
-int hsdl_graph_load_edgelist(const char *path, uint32_t flags, hsdl_graph **out) {
+int hsdl_graph_save_txt(const hsdl_graph *g, const char *path) {
+  return hsdl::guarded([&] { return graph_save_txt_body(g, path); });
+}
+
+static int graph_load_edgelist_body(const char *path, uint32_t flags, hsdl_graph **out) {
   if (!path || !out) return fail(HSDL_EINVAL, "null argument");
   std::ifstream in(path);
   if (!in.is_open()) return fail(HSDL_EIO, std::string("cannot open ") + path);
@@ -316,7 +329,11 @@ int hsdl_graph_load_edgelist(const char *path, uint32_t flags, hsdl_graph **out)
   return hsdl_graph_from_edges((uint32_t)n, us.size(), us.data(), vs.data(), ws.data(), flags, out);
 }
 
-int hsdl_graph_reorder_by_degree(hsdl_graph *g, uint32_t *new_to_old) {
+int hsdl_graph_load_edgelist(const char *path, uint32_t flags, hsdl_graph **out) {
+  return hsdl::guarded([&] { return graph_load_edgelist_body(path, flags, out); });
+}
+
+static int graph_reorder_by_degree_body(hsdl_graph *g, uint32_t *new_to_old) {
   if (!g) return fail(HSDL_EINVAL, "null graph");
   g->drop_device_copy();
   std::vector<uint32_t> n2o;
@@ -325,7 +342,11 @@ int hsdl_graph_reorder_by_degree(hsdl_graph *g, uint32_t *new_to_old) {
   return HSDL_OK;
 }
 
-int hsdl_graph_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+int hsdl_graph_reorder_by_degree(hsdl_graph *g, uint32_t *new_to_old) {
+  return hsdl::guarded([&] { return graph_reorder_by_degree_body(g, new_to_old); });
+}
+
+static int graph_rmat_body(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
                     uint64_t seed, int32_t wmin, int32_t wmax, hsdl_graph **out) {
   if (!out || scale < 1 || scale > 31 || edge_factor < 1 || wmin < 1 || wmax < wmin ||
       a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
@@ -387,7 +408,12 @@ int hsdl_graph_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, do
   return HSDL_OK;
 }
 
-int hsdl_graph_grid(uint32_t rows, uint32_t cols, uint64_t seed, int32_t wmin, int32_t wmax,
+int hsdl_graph_rmat(uint32_t scale, uint32_t edge_factor, double a, double b, double c,
+                    uint64_t seed, int32_t wmin, int32_t wmax, hsdl_graph **out) {
+  return hsdl::guarded([&] { return graph_rmat_body(scale, edge_factor, a, b, c, seed, wmin, wmax, out); });
+}
+
+static int graph_grid_body(uint32_t rows, uint32_t cols, uint64_t seed, int32_t wmin, int32_t wmax,
                     hsdl_graph **out) {
   if (!out || rows < 1 || cols < 1 || (uint64_t)rows * cols > 0xFFFFFFFFull || wmin < 1 ||
       wmax < wmin)
@@ -412,6 +438,11 @@ int hsdl_graph_grid(uint32_t rows, uint32_t cols, uint64_t seed, int32_t wmin, i
   reorder_by_degree(*g, n2o);
   *out = g;
   return HSDL_OK;
+}
+
+int hsdl_graph_grid(uint32_t rows, uint32_t cols, uint64_t seed, int32_t wmin, int32_t wmax,
+                    hsdl_graph **out) {
+  return hsdl::guarded([&] { return graph_grid_body(rows, cols, seed, wmin, wmax, out); });
 }
 
 int hsdl_graph_info(const hsdl_graph *g, uint32_t *n, uint64_t *nnz) {

@@ -106,7 +106,7 @@ int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st) {
   return HSDL_OK;
 }
 
-int hsdl_export_labels(const hsdl_index *idx, uint64_t *off, uint32_t *hub, uint8_t *hop,
+static int export_labels_body(const hsdl_index *idx, uint64_t *off, uint32_t *hub, uint8_t *hop,
                        int32_t *dist) {
   if (!idx || !off) return fail(HSDL_EINVAL, "null argument");
   int rc = use_device(idx->device, nullptr);
@@ -123,7 +123,12 @@ int hsdl_export_labels(const hsdl_index *idx, uint64_t *off, uint32_t *hub, uint
   return HSDL_OK;
 }
 
-int hsdl_index_from_labels(uint32_t n, uint8_t K, const uint64_t *off, const uint32_t *hub,
+int hsdl_export_labels(const hsdl_index *idx, uint64_t *off, uint32_t *hub, uint8_t *hop,
+                       int32_t *dist) {
+  return hsdl::guarded([&] { return export_labels_body(idx, off, hub, hop, dist); });
+}
+
+static int index_from_labels_body(uint32_t n, uint8_t K, const uint64_t *off, const uint32_t *hub,
                            const uint8_t *hop, const int32_t *dist, int32_t device,
                            hsdl_index **out) {
   if (!out || !off || K < 1 || K > kMaxK) return fail(HSDL_EINVAL, "bad argument");
@@ -174,9 +179,15 @@ int hsdl_index_from_labels(uint32_t n, uint8_t K, const uint64_t *off, const uin
   return HSDL_OK;
 }
 
+int hsdl_index_from_labels(uint32_t n, uint8_t K, const uint64_t *off, const uint32_t *hub,
+                           const uint8_t *hop, const int32_t *dist, int32_t device,
+                           hsdl_index **out) {
+  return hsdl::guarded([&] { return index_from_labels_body(n, K, off, hub, hop, dist, device, out); });
+}
+
 // Checkpoint layout of Clean_pc/main.cu:11-57 (save_labels / load_labels):
 // size_t n; per vertex: size_t count, then count x {int hub, parent, hop, dist}.
-int hsdl_save_labels(const hsdl_index *idx, const char *path) {
+static int save_labels_body(const hsdl_index *idx, const char *path) {
   if (!idx || !path) return fail(HSDL_EINVAL, "null argument");
   std::vector<uint64_t> off(idx->n + 1ull);
   std::vector<uint32_t> hub(idx->total);
@@ -210,12 +221,29 @@ int hsdl_save_labels(const hsdl_index *idx, const char *path) {
   return ok ? HSDL_OK : fail(HSDL_EIO, "write failed");
 }
 
-int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **out) {
+int hsdl_save_labels(const hsdl_index *idx, const char *path) {
+  return hsdl::guarded([&] { return save_labels_body(idx, path); });
+}
+
+static int load_labels_body(const char *path, uint8_t K, int32_t device, hsdl_index **out) {
   if (!path || !out) return fail(HSDL_EINVAL, "null argument");
+  if (K < 1 || K > kMaxK) return fail(HSDL_EINVAL, "K must be in 1..15");
   FILE *f = fopen(path, "rb");
   if (!f) return fail(HSDL_EIO, std::string("cannot open ") + path);
+  // the file's size bounds every count read from it, so a corrupt count is
+  // refused before anything is sized from it
+  uint64_t left = 0;
+  if (fseek(f, 0, SEEK_END) == 0) {
+    const long sz = ftell(f);
+    left = sz > 0 ? (uint64_t)sz : 0;
+  }
+  if (fseek(f, 0, SEEK_SET) != 0) left = 0;
   size_t n = 0;
-  bool ok = fread(&n, sizeof(size_t), 1, f) == 1 && n <= 0xFFFFFFFFull;
+  bool ok = left >= sizeof(size_t) && fread(&n, sizeof(size_t), 1, f) == 1;
+  if (ok) left -= sizeof(size_t);
+  // each vertex holds at least its count word
+  ok = ok && n <= 0xFFFFFFFFull && (uint64_t)n <= left / sizeof(size_t);
+  int bad = HSDL_EIO;
   std::vector<uint64_t> off(1, 0);
   std::vector<uint32_t> hub, par;
   std::vector<uint8_t> hop;
@@ -227,15 +255,25 @@ int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **o
   bool any_parent = false;
   for (size_t v = 0; ok && v < n; ++v) {
     size_t c = 0;
-    ok = fread(&c, sizeof(size_t), 1, f) == 1 && c < (1ull << 40);
+    ok = left >= sizeof(size_t) && fread(&c, sizeof(size_t), 1, f) == 1;
+    if (ok) left -= sizeof(size_t);
+    ok = ok && (uint64_t)c <= left / 16;  // 16 B per record
     if (!ok) break;
+    left -= (uint64_t)c * 16;
     rec.resize(4 * c);
     if (c) ok = fread(rec.data(), sizeof(int32_t), 4 * c, f) == 4 * c;
     row.clear();
     for (size_t i = 0; ok && i < c; ++i) {
-      if (rec[4 * i] < 0 || rec[4 * i + 2] < 0 || rec[4 * i + 2] > 255 || rec[4 * i + 3] < 0 ||
+      if (rec[4 * i] < 0 || rec[4 * i + 2] < 0 || rec[4 * i + 3] < 0 ||
           (uint32_t)rec[4 * i + 3] > kMaxDist) {
         ok = false;
+        break;
+      }
+      // the hop field of a packed label holds 0..31: a hop above K is
+      // refused here, before packing could carry it into the hub bits
+      if (rec[4 * i + 2] > (int32_t)K) {
+        ok = false;
+        bad = HSDL_ERANGE;
         break;
       }
       row.push_back({pack_label((uint32_t)rec[4 * i], (uint32_t)rec[4 * i + 2],
@@ -252,6 +290,7 @@ int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **o
     off.push_back(hub.size());
   }
   fclose(f);
+  if (!ok && bad == HSDL_ERANGE) return fail(HSDL_ERANGE, std::string("hop above K in ") + path);
   if (!ok) return fail(HSDL_EIO, std::string("malformed label file ") + path);
   int rc = hsdl_index_from_labels((uint32_t)n, K, off.data(), hub.data(), hop.data(), dist.data(),
                                   device, out);
@@ -261,6 +300,10 @@ int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **o
     *out = nullptr;
   }
   return rc;
+}
+
+int hsdl_load_labels(const char *path, uint8_t K, int32_t device, hsdl_index **out) {
+  return hsdl::guarded([&] { return load_labels_body(path, K, device, out); });
 }
 
 void hsdl_free_index(hsdl_index *idx) {

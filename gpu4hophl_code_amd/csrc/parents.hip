@@ -219,7 +219,7 @@ using namespace hsdl;
 
 extern "C" {
 
-int hsdl_export_parents(const hsdl_index *idx, uint32_t *parent) {
+static int export_parents_body(const hsdl_index *idx, uint32_t *parent) {
   if (!idx || !parent) return fail(HSDL_EINVAL, "null argument");
   if (!idx->has_parents) return fail(HSDL_EINVAL, "index has no parents (build with opts.parents = 1)");
   int rc = use_device(idx->device, nullptr);
@@ -229,7 +229,11 @@ int hsdl_export_parents(const hsdl_index *idx, uint32_t *parent) {
   return HSDL_OK;
 }
 
-int hsdl_index_set_parents(hsdl_index *idx, const uint32_t *parent) {
+int hsdl_export_parents(const hsdl_index *idx, uint32_t *parent) {
+  return hsdl::guarded([&] { return export_parents_body(idx, parent); });
+}
+
+static int index_set_parents_body(hsdl_index *idx, const uint32_t *parent) {
   if (!idx || (!parent && idx->total)) return fail(HSDL_EINVAL, "null argument");
   for (uint64_t i = 0; i < idx->total; ++i)
     if (parent[i] >= idx->n) return fail(HSDL_EINVAL, "parent id out of range");
@@ -243,7 +247,11 @@ int hsdl_index_set_parents(hsdl_index *idx, const uint32_t *parent) {
   return HSDL_OK;
 }
 
-int hsdl_extract_paths(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+int hsdl_index_set_parents(hsdl_index *idx, const uint32_t *parent) {
+  return hsdl::guarded([&] { return index_set_parents_body(idx, parent); });
+}
+
+static int extract_paths_body(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
                        const int8_t *k, uint32_t cap, uint32_t *edges, uint32_t *counts) {
   if (!idx) return fail(HSDL_EINVAL, "null index");
   if (n == 0) return HSDL_OK;
@@ -273,6 +281,11 @@ int hsdl_extract_paths(const hsdl_index *idx, size_t n, const uint32_t *s, const
   if (cap) HSDL_HIP(hipMemcpy(edges, de.p, (size_t)2 * cap * n * 4, hipMemcpyDeviceToHost));
   HSDL_HIP(hipMemcpy(counts, dc.p, n * 4, hipMemcpyDeviceToHost));
   return HSDL_OK;
+}
+
+int hsdl_extract_paths(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, uint32_t cap, uint32_t *edges, uint32_t *counts) {
+  return hsdl::guarded([&] { return extract_paths_body(idx, n, s, t, k, cap, edges, counts); });
 }
 
 }  // extern "C"

@@ -326,7 +326,7 @@ int hsdl_query_device(const hsdl_index *idx, size_t n, const uint32_t *d_s, cons
   return launch_query(idx, n, d_s, d_t, d_k, d_out, (hipStream_t)hip_stream);
 }
 
-int hsdl_query(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+static int query_body(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
                const int8_t *k, int32_t *out) {
   if (!idx) return fail(HSDL_EINVAL, "null index");
   if (n == 0) return HSDL_OK;
@@ -348,5 +348,11 @@ int hsdl_query(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_
   HSDL_HIP(hipMemcpy(out, dout.p, n * 4, hipMemcpyDeviceToHost));
   return HSDL_OK;
 }
+
+int hsdl_query(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+               const int8_t *k, int32_t *out) {
+  return hsdl::guarded([&] { return query_body(idx, n, s, t, k, out); });
+}
+
 
 }  // extern "C"

@@ -219,7 +219,7 @@ using namespace hsdl;
 
 extern "C" {
 
-int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl_server **out) {
+static int server_create_body(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl_server **out) {
   if (!idx || !out) return fail(HSDL_EINVAL, "null argument");
   hsdl_server_opts o;
   if (opts) o = *opts; else hsdl_server_opts_init(&o);
@@ -296,8 +296,16 @@ int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl
   return HSDL_OK;
 }
 
-int hsdl_server_submit(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
-                       const int8_t *k, const uint64_t *arrival_ns, uint64_t *first_id) {
+int hsdl_server_create(const hsdl_index *idx, const hsdl_server_opts *opts, hsdl_server **out) {
+  return hsdl::guarded([&] { return server_create_body(idx, opts, out); });
+}
+
+// Enqueues requests [0, n) when `partial` is false (all or nothing:
+// HSDL_EBUSY when they do not all fit), or the longest prefix that fits when
+// it is true; *taken receives the count enqueued.
+static int enqueue_requests(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+                            const int8_t *k, const uint64_t *arrival_ns, uint64_t *first_id,
+                            bool partial, size_t *taken) {
   if (!srv) return fail(HSDL_EINVAL, "null server");
   if (n && (!s || !t || !k)) return fail(HSDL_EINVAL, "null request arrays");
   for (size_t i = 0; i < n; ++i)
@@ -306,18 +314,31 @@ int hsdl_server_submit(hsdl_server *srv, size_t n, const uint32_t *s, const uint
   const uint64_t now = now_ns();
   std::lock_guard<std::mutex> g(srv->mu);
   // pending, in flight and unpolled requests share the capacity
-  if (srv->q_tail - srv->r_head.load(std::memory_order_acquire) + n > cap)
-    return fail(HSDL_EBUSY, "request queue full");
+  const uint64_t used = srv->q_tail - srv->r_head.load(std::memory_order_acquire);
+  const uint64_t room = used < cap ? cap - used : 0;
+  if (!partial && n > room) return fail(HSDL_EBUSY, "request queue full");
+  const size_t m = (size_t)std::min<uint64_t>(n, room);
   if (first_id) *first_id = srv->q_tail;
-  for (size_t i = 0; i < n; ++i) {
+  for (size_t i = 0; i < m; ++i) {
     const uint64_t p = (srv->q_tail + i) % cap;
     srv->q_s[p] = s[i];
     srv->q_t[p] = t[i];
     srv->q_k[p] = k[i];
     srv->q_arr[p] = arrival_ns ? arrival_ns[i] : now;
   }
-  srv->q_tail += n;
+  srv->q_tail += m;
+  if (taken) *taken = m;
   return HSDL_OK;
+}
+
+static int server_submit_body(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+                              const int8_t *k, const uint64_t *arrival_ns, uint64_t *first_id) {
+  return enqueue_requests(srv, n, s, t, k, arrival_ns, first_id, false, nullptr);
+}
+
+int hsdl_server_submit(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, const uint64_t *arrival_ns, uint64_t *first_id) {
+  return hsdl::guarded([&] { return server_submit_body(srv, n, s, t, k, arrival_ns, first_id); });
 }
 
 int hsdl_server_poll(hsdl_server *srv, size_t cap, uint64_t *ids, int32_t *answers,
@@ -391,7 +412,7 @@ void hsdl_server_destroy(hsdl_server *srv) {
   delete srv;
 }
 
-int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+static int server_replay_body(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
                        const int8_t *k, const uint64_t *arrival_offset_ns, int32_t *answers,
                        uint64_t *latency_ns) {
   if (!srv || (n && (!s || !t || !k || !arrival_offset_ns))) return fail(HSDL_EINVAL, "null argument");
@@ -422,9 +443,14 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
         srv->note(TraceEvent{'R', 0, {now, now - start - arrival_offset_ns[sent]}});
       arr.resize(upto - sent);
       for (size_t i = sent; i < upto; ++i) arr[i - sent] = start + arrival_offset_ns[i];
-      int rc = hsdl_server_submit(srv, upto - sent, s + sent, t + sent, k + sent, arr.data(), nullptr);
-      if (rc == HSDL_OK) sent = upto;
-      else if (rc != HSDL_EBUSY) return rc;
+      // as many of the due requests as the queue has room for: a backlog
+      // larger than the queue (a burst, or a server that fell behind) goes in
+      // as results are drained below, instead of being retried all or nothing
+      size_t taken = 0;
+      int rc = enqueue_requests(srv, upto - sent, s + sent, t + sent, k + sent, arr.data(), nullptr,
+                                true, &taken);
+      if (rc != HSDL_OK) return rc;
+      sent += taken;
     }
     size_t fresh = 0;
     {
@@ -447,6 +473,12 @@ int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint
     if (fresh == 0 && upto == sent) std::this_thread::yield();
   }
   return HSDL_OK;
+}
+
+int hsdl_server_replay(hsdl_server *srv, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, const uint64_t *arrival_offset_ns, int32_t *answers,
+                       uint64_t *latency_ns) {
+  return hsdl::guarded([&] { return server_replay_body(srv, n, s, t, k, arrival_offset_ns, answers, latency_ns); });
 }
 
 }  // extern "C"

@@ -259,7 +259,14 @@ typedef struct hsdl_server_opts {
   uint32_t max_wait_us;    /* batching window (default 200)                 */
   uint32_t inflight;       /* concurrent batches on the device (default 4)  */
   uint64_t queue_capacity; /* submitted, not yet polled (default 1<<22)     */
-  uint64_t hbm_budget;     /* device bytes for batch buffers; 0 = derive    */
+  uint64_t hbm_budget;     /* staging bytes for batch slots (13 B per
+                              request per slot); 0 = a quarter of the free
+                              HBM at creation.  Admission by memory is a
+                              static cap: the slot count is fixed at
+                              creation as min(inflight, budget / slot
+                              size), and a closed batch waits for a free
+                              slot.  Mapped staging holds the slots in
+                              pinned host memory, copy staging in HBM.     */
   uint32_t staging;        /* HSDL_STAGE_MAPPED (default) or HSDL_STAGE_COPY */
   uint32_t reserved[7];
 } hsdl_server_opts;

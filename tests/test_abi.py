@@ -259,3 +259,85 @@ def test_label_import_and_handle_errors(tmp_path):
     assert L.hsdl_query(None, 1, u32, u32, i8, i32) == -1
     assert L.hsdl_extract_paths(None, 1, u32, u32, i8, 0, None, u32) == -1
     assert L.hsdl_server_create(None, None, ctypes.byref(nul)) == -1
+
+
+def _dump_bytes(rows):
+    """The reference's label dump (Clean_pc/main.cu:11-57) written with
+    numpy: size_t n, then per vertex size_t count and count records of four
+    int32 {hub, parent, hop, dist}."""
+    out = [np.array([len(rows)], np.uint64).tobytes()]
+    for r in rows:
+        out.append(np.array([len(r)], np.uint64).tobytes())
+        if r:
+            out.append(np.array(r, np.int32).reshape(-1).tobytes())
+    return b"".join(out)
+
+
+def test_label_dump_errors_before_the_device(tmp_path):
+    """Malformed dumps are refused on the host: a hop above K (a hop of 32
+    or more would otherwise spill into the packed hub field) gives
+    HSDL_ERANGE, a count larger than the file or a truncated record
+    HSDL_EIO -- never an abort from a huge allocation."""
+    def code(path, K=5):
+        with pytest.raises(H.HSDLError) as e:
+            H.Index.load(str(path), K)
+        return e.value.code
+
+    ok_rows = [[(0, 0, 0, 0)], [(0, 0, 1, 7), (1, 0, 0, 0)]]
+    p = tmp_path / "hop33.bin"
+    p.write_bytes(_dump_bytes([[(0, 0, 0, 0)], [(0, 0, 33, 7), (1, 0, 0, 0)]]))
+    assert code(p) == -7
+    p = tmp_path / "hop6.bin"
+    p.write_bytes(_dump_bytes([[(0, 0, 0, 0)], [(0, 0, 6, 7), (1, 0, 0, 0)]]))
+    assert code(p) == -7
+    # a per-vertex count of 2^39 records in a file of a few bytes
+    good = _dump_bytes(ok_rows)
+    bad = bytearray(good)
+    bad[8:16] = np.array([1 << 39], np.uint64).tobytes()
+    p = tmp_path / "count.bin"
+    p.write_bytes(bytes(bad))
+    assert code(p) == -3
+    # a vertex count beyond what the file can hold
+    bad = bytearray(good)
+    bad[0:8] = np.array([1 << 31], np.uint64).tobytes()
+    p = tmp_path / "n.bin"
+    p.write_bytes(bytes(bad))
+    assert code(p) == -3
+    p = tmp_path / "trunc.bin"
+    p.write_bytes(good[:-3])
+    assert code(p) == -3
+    p = tmp_path / "empty.bin"
+    p.write_bytes(b"")
+    assert code(p) == -3
+    with pytest.raises(H.HSDLError) as e:
+        H.Index.load(str(tmp_path / "hop6.bin"), 0)
+    assert e.value.code == -1
+
+
+def test_reorder_tie_rule_is_degree_desc_then_id():
+    """The degree renaming (reference
+    graph_v_of_v_update_vertexIDs_by_degrees_large_to_small.h:14-40, whose
+    std::sort leaves ties unordered) is fixed here as (degree desc, old id
+    asc): the oracle's and the library's new->old maps both equal numpy's
+    lexsort of (id, -degree), on graphs with many equal degrees."""
+    rng = np.random.default_rng(11)
+    for trial in range(20):
+        n = int(rng.integers(2, 60))
+        m = int(rng.integers(0, 3 * n))
+        u = rng.integers(0, n, m)
+        v = rng.integers(0, n, m)
+        keep = u != v
+        u, v = u[keep], v[keep]
+        w = rng.integers(1, 9, len(u))
+        g = H.Graph.from_edges(n, u, v, w, H.DROP_SELF_LOOPS)
+        off, col, ww = g.csr()
+        deg = np.diff(off.astype(np.int64))
+        want = np.lexsort((np.arange(n), -deg))
+        og = O.CSR(n, off.astype(np.int64), col.astype(np.int32), ww)
+        _, on2o = O.reorder(og)
+        assert np.array_equal(on2o, want.astype(np.int32)), trial
+        n2o = g.reorder_by_degree()
+        assert np.array_equal(n2o.astype(np.int64), want), trial
+        # a planted mistake (ties by id descending) differs on these graphs
+        if trial == 0:
+            assert not np.array_equal(np.lexsort((-np.arange(n), -deg)), want) or n < 3

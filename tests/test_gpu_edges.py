@@ -1,0 +1,250 @@
+"""GPU edge cases: the exact 32-bit label packing, prune block sizes, HBM
+budget failures, server back-pressure and the label dump's byte layout.
+Every comparison with the oracle is exact (integer work)."""
+import numpy as np
+import pytest
+
+import gpu4hophl_code_amd as H
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def csr_of(g):
+    off, col, w = g.csr()
+    return O.CSR(len(off) - 1, off.astype(np.int64), col.astype(np.int32), w)
+
+
+def same_labels(idx, want):
+    off, hub, hop, dist = idx.labels()
+    woff, whub, whop, wdist = want
+    assert np.array_equal(off.astype(np.int64), np.asarray(woff, np.int64))
+    assert np.array_equal(hub, whub)
+    assert np.array_equal(hop, whop)
+    assert np.array_equal(dist, wdist)
+
+
+def bits(x):
+    """Bits of an unsigned field that holds 0..x (at least 1)."""
+    b = 1
+    while (1 << b) <= x:
+        b += 1
+    return b
+
+
+@pytest.fixture(scope="module")
+def wide32():
+    """R-MAT scale 12, edge factor 8, K=5, weights U[20000, 26000]: 12 hub
+    bits + 3 hop bits + 17 distance bits = exactly 32, both for the
+    generator's 32-bit pool copy (K * max weight = 130,000 < 2^17) and for
+    the clean's 32-bit copy (every label of 4 or more hops is at least
+    80,000 >= 2^16, so the largest label distance needs 17 bits) -- the
+    packing R-MAT scale 20 at K=5 uses (20 + 3 + 9)."""
+    g = H.Graph.rmat(12, 8, 0x5EED0032, 20000, 26000)
+    c = csr_of(g)
+    K = 5
+    kr = O.kround(c, K)
+    want = kr.arrays()
+    assert int(c.w.max()) * K < (1 << 17) and int(c.w.max()) * K >= (1 << 16)
+    assert bits(c.n - 1) + bits(K) + bits(int(c.w.max()) * K) == 32
+    assert bits(c.n - 1) + bits(K) + bits(int(want[3].max())) == 32
+    return g, c, K, kr, want
+
+
+@pytest.mark.parametrize("rows", ["1", "2", "0"])
+@pytest.mark.parametrize("mode,hub_batch", [("batched", 0), ("batched", 333), ("plain", 0)])
+def test_exact_32bit_packing_generation_clean_queries(monkeypatch, wide32, rows, mode, hub_batch):
+    """At exactly 32 packed bits the prune streams the 32-bit pool copy
+    (prune_label_bytes 4) and the clean streams its 32-bit copy
+    (clean_label_bytes 4); labels, cleaned labels and queries equal the
+    oracle's."""
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    g, c, K, kr, want = wide32
+    m = H.MODE_PLAIN if mode == "plain" else H.MODE_BATCHED
+    idx = H.hop_constrained_two_hop_labels_generation(g, K, mode=m, hub_batch=hub_batch)
+    st = idx.stats()
+    assert st["prune_label_bytes"] == 4
+    same_labels(idx, want)
+    idx.clean()
+    assert idx.stats()["clean_label_bytes"] == 4
+    cl = O.clean(kr)
+    same_labels(idx, cl.arrays())
+    rng = np.random.default_rng(5)
+    s = rng.integers(0, c.n, 20000).astype(np.uint32)
+    t = rng.integers(0, c.n, 20000).astype(np.uint32)
+    k = rng.integers(-1, K + 2, 20000).astype(np.int8)
+    assert np.array_equal(idx.query(s, t, k), O.query(cl, s, t, k.astype(np.int32)))
+
+
+def test_exact_32bit_packing_clean_of_redundant_labels(wide32):
+    """The clean at 32 packed bits on labels it must thin: the reference's
+    vertex-group generation (pruning sees only same-group hubs) imported and
+    cleaned on the GPU equals the oracle's clean of the same labels."""
+    g, c, K, kr, want = wide32
+    red = O.kround_grouped(c, K, 256)
+    off, hub, hop, dist = red.arrays()
+    assert len(hub) > len(want[1])
+    idx = H.Index.from_labels(c.n, K, off.astype(np.uint64), hub, hop, dist)
+    idx.clean()
+    assert idx.stats()["clean_label_bytes"] == 4
+    same_labels(idx, O.clean(red).arrays())
+
+
+@pytest.fixture(scope="module")
+def ragged_grid():
+    """A 61 x 83 grid (5,063 vertices: no multiple of any vertex block) at
+    K=6 with its oracle labels."""
+    g = H.Graph.grid(61, 83, 0x5EED0061, 1, 1000)
+    return g, O.kround(csr_of(g), 6).arrays()
+
+
+@pytest.fixture(scope="module")
+def heavy13():
+    g = H.Graph.rmat(13, 16, 77, 1, 100)
+    return g, O.kround(csr_of(g), 4).arrays()
+
+
+@pytest.mark.parametrize("block", ["1024", "4096"])
+@pytest.mark.parametrize("rows", ["1", "2"])
+def test_row_block_sizes(monkeypatch, heavy13, ragged_grid, block, rows):
+    """HSDL_ROW_BLOCK (vertices per y-block of a row item) at 1024 and 4096,
+    beside the default 2048 the other tests use: on R-MAT scale 13 and on a
+    grid whose vertex count is no multiple of the block, with row items where
+    dense and in every round."""
+    monkeypatch.setenv("HSDL_ROW_BLOCK", block)
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    g, want = heavy13
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4), want)
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=100), want)
+    gg, gwant = ragged_grid
+    same_labels(H.hop_constrained_two_hop_labels_generation(gg, 6), gwant)
+
+
+def test_hbm_budget_failures_are_clean(golden_dir):
+    """A build under a device budget too small for it returns HSDL_ENOMEM
+    (at set-up, or when the label pool, arena or lists must grow past it)
+    and leaves the graph usable: over a sweep of budgets every build either
+    fails with HSDL_ENOMEM or gives the golden labels, both outcomes occur,
+    and a build without a budget afterwards is exact."""
+    import os
+    z = np.load(os.path.join(golden_dir, "c1_rmat12_k4.npz"))
+    want = (z["lab_off"], z["hub"], z["hop"], z["dist"])
+    g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    outcomes = []
+    # set-up needs about 440 MB here (the dense slots, lists, tile arena and
+    # heavy-piece list); budgets just above it fail later, when the label
+    # pool must grow
+    for mb in [1, 96] + list(range(430, 472, 2)) + [512]:
+        try:
+            idx = H.hop_constrained_two_hop_labels_generation(
+                g, 4, hbm_budget=mb << 20, pool_init_labels=4096 * 16)
+        except H.HSDLError as e:
+            assert e.code == -2, (mb, e)
+            outcomes.append("enomem")
+            continue
+        same_labels(idx, want)
+        outcomes.append("ok")
+        idx.free()
+    assert "enomem" in outcomes and "ok" in outcomes, outcomes
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4), want)
+
+
+@pytest.fixture(scope="module")
+def c1_index(golden_dir):
+    import os
+    z = np.load(os.path.join(golden_dir, "c1_rmat12_k4.npz"))
+    g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
+    return idx, z
+
+
+def test_server_queue_full_and_backlog_larger_than_queue(c1_index):
+    """Back-pressure: a submit that does not fit the queue is refused whole
+    (HSDL_EBUSY, nothing enqueued); a replay whose due requests exceed the
+    queue capacity (all 20,000 arrive at once into a queue of 1,000)
+    finishes, feeding the queue as results drain, with every answer equal to
+    hsdl_query's."""
+    idx, z = c1_index
+    n = 4096
+    rng = np.random.default_rng(9)
+    s = rng.integers(0, n, 20000).astype(np.uint32)
+    t = rng.integers(0, n, 20000).astype(np.uint32)
+    k = rng.integers(1, 5, 20000).astype(np.int8)
+    want = idx.query(s, t, k)
+    srv = H.Server(idx, max_batch=256, max_wait_us=50, inflight=2, queue_capacity=1000)
+    try:
+        assert srv.submit(s[:1001], t[:1001], k[:1001]) is None  # EBUSY
+        first = srv.submit(s[:1000], t[:1000], k[:1000])
+        assert first == 0
+        assert srv.submit(s[:1], t[:1], k[:1]) is None  # full until polled
+        srv.drain()
+        ids, ans, _ = srv.poll(2000)
+        order = np.argsort(ids)
+        assert np.array_equal(ids[order], np.arange(1000))
+        assert np.array_equal(ans[order], want[:1000])
+        got, lat = srv.replay(s, t, k, np.zeros(len(s), np.uint64))
+        assert np.array_equal(got, want)
+        assert (lat > 0).all()
+    finally:
+        srv.close()
+
+
+def test_server_one_slot_many_small_batches(c1_index):
+    """Slot starvation: one in-flight slot and batches of 64 -- requests wait
+    for the slot, none is lost or answered twice, and the batches number at
+    least requests / 64."""
+    idx, z = c1_index
+    rng = np.random.default_rng(10)
+    nq = 30000
+    s = rng.integers(0, 4096, nq).astype(np.uint32)
+    t = rng.integers(0, 4096, nq).astype(np.uint32)
+    k = rng.integers(0, 6, nq).astype(np.int8)
+    want = idx.query(s, t, k)
+    srv = H.Server(idx, max_batch=64, max_wait_us=0, inflight=1, queue_capacity=4096)
+    try:
+        gaps = rng.exponential(200.0, nq)  # ~5M q/s offered
+        got, _ = srv.replay(s, t, k, np.cumsum(gaps).astype(np.uint64))
+        st = srv.stats()
+        assert np.array_equal(got, want)
+        assert st["completed"] == nq and st["batches"] >= nq // 64
+    finally:
+        srv.close()
+
+
+def _dump_bytes(off, hub, hop, dist, par=None):
+    """The reference's label dump (Clean_pc/main.cu:11-57) built with
+    numpy: size_t n; per vertex size_t count, then count records of four
+    int32 {hub, parent, hop, dist}."""
+    n = len(off) - 1
+    parts = [np.array([n], np.uint64).tobytes()]
+    rec = np.zeros((len(hub), 4), np.int32)
+    rec[:, 0] = hub
+    rec[:, 1] = 0 if par is None else par
+    rec[:, 2] = hop
+    rec[:, 3] = dist
+    for v in range(n):
+        parts.append(np.array([off[v + 1] - off[v]], np.uint64).tobytes())
+        parts.append(rec[off[v]:off[v + 1]].tobytes())
+    return b"".join(parts)
+
+
+def test_label_dump_byte_layout(tmp_path, c1_index):
+    """hsdl_save_labels writes the reference's dump byte for byte (a numpy
+    packing of the exported labels, parent 0 without parents), and a dump
+    written by numpy in that layout -- records in reverse order, as the
+    reference's uncleaned export leaves them -- loads back sorted."""
+    idx, z = c1_index
+    off, hub, hop, dist = idx.labels()
+    off = off.astype(np.int64)
+    p = tmp_path / "c1.bin"
+    idx.save(str(p))
+    assert p.read_bytes() == _dump_bytes(off, hub, hop, dist)
+    # reversed per vertex, with parents 0
+    rh, rp, rd = hub.copy(), hop.copy(), dist.copy()
+    for v in range(len(off) - 1):
+        a, b = off[v], off[v + 1]
+        rh[a:b], rp[a:b], rd[a:b] = hub[a:b][::-1], hop[a:b][::-1], dist[a:b][::-1]
+    q = tmp_path / "rev.bin"
+    q.write_bytes(_dump_bytes(off, rh, rp, rd))
+    back = H.Index.load(str(q), 4)
+    same_labels(back, (off, hub, hop, dist))
