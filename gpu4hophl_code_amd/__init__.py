@@ -37,6 +37,7 @@ MODE_BATCHED = 0
 MODE_PLAIN = 1
 DROP_SELF_LOOPS = 0x1
 DEDUP_MIN = 0x2
+NOALLOC_MAX_QUERIES = 1 << 20
 INT32_MAX = 2147483647
 
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
@@ -103,6 +104,7 @@ _SIGNATURES = [
     ("hsdl_index_stats", ctypes.c_int, [_vp, ctypes.POINTER(hsdl_stats)]),
     ("hsdl_query", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
     ("hsdl_query_device", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp]),
+    ("hsdl_query_noalloc", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
     ("hsdl_extract_paths", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, ctypes.c_uint32, _vp, _u32p]),
     ("hsdl_export_parents", ctypes.c_int, [_vp, _u32p]),
     ("hsdl_index_set_parents", ctypes.c_int, [_vp, _u32p]),
@@ -357,6 +359,18 @@ class Index:
             raise ValueError("s, t, k must have equal length")
         out = np.zeros(len(s), np.int32)
         _check(lib().hsdl_query(self.h, len(s), s, t, k, out))
+        return out
+
+    def query_noalloc(self, s, t, k):
+        """hsdl_query without device allocation (at most NOALLOC_MAX_QUERIES
+        per call; the library's fixed staging array, shared per GPU)."""
+        s = np.ascontiguousarray(s, np.uint32)
+        t = np.ascontiguousarray(t, np.uint32)
+        k = np.ascontiguousarray(k, np.int8)
+        if not (len(s) == len(t) == len(k)):
+            raise ValueError("s, t, k must have equal length")
+        out = np.zeros(len(s), np.int32)
+        _check(lib().hsdl_query_noalloc(self.h, len(s), s, t, k, out))
         return out
 
     def query_device(self, d_s, d_t, d_k, d_out, n, stream=0):

@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <mutex>
 #include <vector>
 
 #include "device.hpp"
@@ -354,5 +355,79 @@ int hsdl_query(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_
   return hsdl::guarded([&] { return query_body(idx, n, s, t, k, out); });
 }
 
+}  // extern "C"
+
+// ------------------------------------------------------- no-allocation query
+// hsdl_query_noalloc stages the requests and answers in `scratch`, a
+// file-scope device array that every GPU the library runs on holds until the
+// process ends, instead of allocating device buffers per call: for callers
+// that cannot allocate.  Layout for n <= kNoAllocMaxQueries requests:
+//   [s: u32 x max][t: u32 x max][out: i32 x max][k: i8 x max]
+// Nothing in it must start at zero (the query kernel writes every answer).
+// Every call on one GPU shares that GPU's array: calls that overlap on one
+// GPU race and must be serialised by the caller.
+namespace {
+constexpr size_t kNoAllocMaxQueries = (size_t)1 << 20;
+constexpr size_t kNoAllocBytes = kNoAllocMaxQueries * 13;
+static_assert(HSDL_NOALLOC_MAX_QUERIES == kNoAllocMaxQueries, "header and library disagree");
+constexpr int kNoAllocDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+// Device address of `scratch` per device: the host-side name is only a
+// stand-in, so host code uses these (looked up once per device, with that
+// device current).
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kNoAllocDevices] = {};
+
+int scratch_for_device(int dev, uint8_t **out) {
+  if (dev < 0 || dev >= kNoAllocDevices) return fail(HSDL_ENODEV, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_scratch_ptr[dev] = p;
+  }
+  *out = static_cast<uint8_t *>(g_scratch_ptr[dev]);
+  return HSDL_OK;
+}
+}  // namespace
+
+extern "C" {
+
+static int query_noalloc_body(const hsdl_index *idx, size_t n, const uint32_t *s,
+                              const uint32_t *t, const int8_t *k, int32_t *out) {
+  if (!idx) return fail(HSDL_EINVAL, "null index");
+  if (n > kNoAllocMaxQueries)
+    return fail(HSDL_EINVAL, "hsdl_query_noalloc takes at most HSDL_NOALLOC_MAX_QUERIES (2^20) "
+                             "queries per call");
+  if (n == 0) return HSDL_OK;
+  if (!s || !t || !k || !out) return fail(HSDL_EINVAL, "null query arrays");
+  for (size_t i = 0; i < n; ++i)
+    if (s[i] >= idx->n || t[i] >= idx->n) return fail(HSDL_EINVAL, "query vertex out of range");
+  int dev = 0;
+  int rc = use_device(idx->device, &dev);
+  if (rc) return rc;
+  uint8_t *base = nullptr;
+  if ((rc = scratch_for_device(dev, &base))) return rc;
+  uint32_t *ds = reinterpret_cast<uint32_t *>(base);
+  uint32_t *dt = ds + kNoAllocMaxQueries;
+  int32_t *dout = reinterpret_cast<int32_t *>(dt + kNoAllocMaxQueries);
+  int8_t *dk = reinterpret_cast<int8_t *>(dout + kNoAllocMaxQueries);
+  HSDL_HIP(hipMemcpy(ds, s, n * 4, hipMemcpyHostToDevice));
+  HSDL_HIP(hipMemcpy(dt, t, n * 4, hipMemcpyHostToDevice));
+  HSDL_HIP(hipMemcpy(dk, k, n, hipMemcpyHostToDevice));
+  if ((rc = launch_query(idx, n, ds, dt, dk, dout, nullptr))) return rc;
+  HSDL_HIP(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
+  return HSDL_OK;
+}
+
+int hsdl_query_noalloc(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, int32_t *out) {
+  return hsdl::guarded([&] { return query_noalloc_body(idx, n, s, t, k, out); });
+}
 
 }  // extern "C"

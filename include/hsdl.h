@@ -202,6 +202,19 @@ int hsdl_query_device(const hsdl_index *idx, size_t n, const uint32_t *d_s,
                       const uint32_t *d_t, const int8_t *d_k, int32_t *d_out,
                       void *hip_stream);
 
+/* hsdl_query without any allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * answers and blocking behaviour as hsdl_query, at most
+ * HSDL_NOALLOC_MAX_QUERIES queries per call (HSDL_EINVAL beyond).  The
+ * requests and answers are staged in a fixed device array of the library
+ * (13 B per query, 13.6 MB per GPU, held until the process ends) instead of
+ * buffers allocated per call; calls that overlap on one GPU share that array
+ * and race, so the caller serialises them. */
+#define HSDL_NOALLOC_MAX_QUERIES ((size_t)1 << 20)
+int hsdl_query_noalloc(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
+                       const int8_t *k, int32_t *out);
+
 /* Batched hop-constrained shortest paths from host arrays; blocks until
  * done.  Replaces hop_constrained_extract_shortest_path
  * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319) and
