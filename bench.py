@@ -13,8 +13,13 @@ Default workload: R-MAT scale 18, edge factor 16, (a,b,c,d) =
 Prints one JSON line.  Extra fields: build_s (batched), plain_build_s and
 batched_vs_plain, query throughput on device buffers, serving p50/p99 at the
 requested Poisson rates, the roofline of the prune kernel (the build's
-dominant kernel) and the CPU baseline (the oracle's K-round build on a
-bounded sample, timed on this host).
+dominant kernel) against the reference algorithm's bytes (9 B per label of
+L_{<h}(y) read per candidate and of L_{<h}(s) per active hub, counted by the
+oracle and committed with the full-size fixtures), the CPU baseline (the
+reference's threaded CPU_HopHL generation on the same graph, time-boxed) and,
+by default, an R-MAT scale-20 leg (batched build + clean, labels before and
+after, serving at the same rates on that index, and a deadline-bounded plain
+build for a lower bound on the speedup).
 """
 import argparse
 import json
@@ -140,24 +145,42 @@ def plain_per_query(H, idx, n, torch, count=2000):
             "p99_us": float(np.percentile(lat, 99)) * 1e6, "queries": count}
 
 
-def cpu_baseline(scale=15, K=5):
-    """The oracle's K-round build + clean (restated reference CPU twin) on a
-    bounded sample: an R-MAT graph of the same family at a smaller scale."""
+def oracle_csr(g):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import gpu4hophl_code_amd as H
     import oracle_lib as O
-    g = H.Graph.rmat(scale, 16, 0x5EED0002, 1, 100)
     off, col, w = g.csr()
-    c = O.CSR(len(off) - 1, off.astype(np.int64), col.astype(np.int32), w)
+    return O, O.CSR(len(off) - 1, off.astype(np.int64), col.astype(np.int32), w)
+
+
+def cpu_baseline(g, K, n, seconds=20.0):
+    """The reference's CPU path (CPU_HopHL: threaded per-hub Dijkstra
+    generation, restated by the oracle) on the bench graph itself, stopped
+    after `seconds`: labels per second over the hubs it finished.  The full
+    generation does not finish in a bench's time (a DNF at the deadline, with
+    the fraction of hubs done); the K-round twin's full build + clean at
+    R-MAT scale 15 is reported beside it."""
+    O, c = oracle_csr(g)
     threads = cpu_threads()
+    hubs, labels, sec = O.dijkstra_timed(c, K, threads=threads, seconds=seconds)
+    out = {"value": labels / sec, "unit": "labels/s", "cores": threads, "kind": "port",
+           "sample": f"CPU_HopHL threaded Dijkstra generation on the bench graph (n={n}), "
+                     f"stopped at {seconds:.0f} s: {hubs} of {n} hubs in id order "
+                     f"({100.0 * hubs / n:.2f}%, DNF), {labels} uncleaned labels in {sec:.2f} s",
+           "hubs_done": hubs, "hubs_total": n, "completed": hubs == n,
+           "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+    # the K-round twin (the GPU's own algorithm on the CPU) on a graph of the
+    # same family small enough to finish
+    g15 = __import__("gpu4hophl_code_amd").Graph.rmat(15, 16, 0x5EED0002, 1, 100)
+    _, c15 = oracle_csr(g15)
     t0 = time.perf_counter()
-    lab = O.kround(c, K, threads=threads)
+    lab = O.kround(c15, K, threads=threads)
     cl = O.clean(lab, 1, threads=threads)
-    sec = time.perf_counter() - t0
-    return {"value": cl.total() / sec, "unit": "labels/s", "cores": threads, "kind": "port",
-            "sample": f"R-MAT scale {scale} (same generator, edge factor 16, K={K}), full "
-                      f"K-round build + clean by the oracle, {cl.total()} labels in {sec:.2f} s",
-            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+    s15 = time.perf_counter() - t0
+    out["kround_scale15"] = {"labels_per_s": cl.total() / s15, "seconds": s15,
+                             "labels": cl.total(), "threads": threads,
+                             "sample": "R-MAT scale 15 (same generator, edge factor 16, K=5): "
+                                       "full K-round build + clean by the oracle"}
+    return out
 
 
 def cpu_threads():
@@ -194,6 +217,210 @@ def cpu_query_baseline(idx, n, K):
     return out
 
 
+# The reference algorithm's per-build counts (the oracle's K-round build of
+# the same graph, committed with the full-size fixtures): frontier items,
+# their full adjacency, candidates, labels of L_{<h}(y) per candidate and of
+# L_{<h}(s) per active hub, accepted labels.
+ALG_FIXTURES = {"rmat18": "full_c2_rmat18_k5.npz", "grid1024": "full_c3_grid1024_k8.npz"}
+
+
+def ref_counters(config):
+    name = ALG_FIXTURES.get(config)
+    path = os.path.join(ROOT, "tests", "golden", name) if name else None
+    if not path or not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return {k: int(z[k]) for k in ("frontier_items", "scanned_edges", "candidates",
+                                   "cand_label_reads", "hub_label_reads", "accepted")}
+
+
+def clean_reference_bytes(H, g, K):
+    """Algorithmic bytes of the reference clean on this graph's labels: 9 B
+    per label of L(v), 9 B per label of L(u) for every label (u,.,.) of L(v)
+    with u != v (the reference reads L(u) once per label), 9 B per kept
+    label.  From an untimed build's labels."""
+    idx = H.hop_constrained_two_hop_labels_generation(g, K)
+    off, hub = idx.hubs()
+    off = off.astype(np.int64)
+    lens = np.diff(off)
+    own = np.repeat(np.arange(len(lens), dtype=np.int64), lens)
+    reads = int(lens[hub].astype(np.int64)[hub.astype(np.int64) != own].sum())
+    idx.clean()
+    kept = idx.label_count()
+    idx.free()
+    return 9 * int(off[-1]) + 9 * reads + 9 * kept
+
+
+def profile_traffic(name, config, K):
+    """HBM bytes per build of a kernel, from the rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes on this workload (profiles/<kernel>_<config>_k<K>_traffic.json,
+    each counter scaled by its reading on a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", f"{name}_{config}_k{K}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def rooflines(H, stats, config, K, ref, clean_bytes):
+    """Rooflines of the build kernels, timed with HIP events on the build
+    stream (summed over a build's launches).  `achieved` divides the
+    reference algorithm's bytes per launch (ref: the oracle's counts) by the
+    average launch time:
+      k_prune:  9 B x (labels of L_{<h}(y) per candidate + labels of
+                L_{<h}(s) per active hub)  -- the reference's prune reads
+                every label of both lists; this kernel stops at the first
+                witness and serves repeated lists from L2, so its rate can
+                pass the HBM peak
+      k_expand: 8 B per frontier item + 8 B per entry of its whole adjacency
+      clean:    9 B per label of L(v) + 9 B x |L(u)| per label (u,.,.) of
+                L(v), u != v + 9 B per kept label
+    Without reference counts (no fixture for the workload) the kernels' own
+    streamed bytes are used.  `kernel_stream` gives those bytes for every
+    kernel, and `traffic` the HBM bytes per launch measured by rocprofv3."""
+    launches = max(1, stats["prune_launches"])  # one expand and one prune per round
+    own = {
+        # 4 or 8 B per label of L(y) streamed (up to the step with the
+        # witness) + 8 B per label of L(s) staged + 8 B per candidate
+        "k_prune": (stats.get("prune_label_bytes") or 8) * stats["prune_label_reads"]
+        + 8 * stats["prune_tile_reads"] + 8 * stats["candidates"],
+        # 28 B per frontier item + 12 B per scanned edge (8 B (col, w) + the
+        # 4 B dense slot) + 8 B per candidate appended
+        "k_expand": 28 * stats["frontier_items"] + 12 * stats["scanned_edges"]
+        + 8 * stats["candidates"],
+    }
+    alg = None
+    if ref:
+        alg = {"k_prune": 9 * (ref["cand_label_reads"] + ref["hub_label_reads"]),
+               "k_expand": 8 * (ref["frontier_items"] + ref["scanned_edges"])}
+    secs = {"k_prune": stats["t_prune_s"], "k_expand": stats["t_expand_s"]}
+    rl = {}
+    for name in ("k_prune", "k_expand"):
+        nbytes = alg[name] if alg else own[name]
+        t = secs[name]
+        achieved = nbytes / t / 1e9 if t > 0 else 0.0
+        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+             "kernel": name, "bytes": "reference algorithm" if alg else "kernel stream",
+             "bytes_per_launch": nbytes / launches, "avg_launch_us": t / launches * 1e6,
+             "launches": launches, "seconds": t,
+             "kernel_stream": {"bytes_per_launch": own[name] / launches,
+                               "achieved": round(own[name] / t / 1e9, 1) if t > 0 else None,
+                               "frac": round(own[name] / t / 1e9 / HBM_PEAK_GBS, 4)
+                               if t > 0 else None}}
+        tr = profile_traffic(name, config, K)
+        if tr and tr.get("hbm_bytes_per_build"):
+            r["traffic"] = tr["hbm_bytes_per_build"] / tr.get("launches_per_build", launches)
+            r["traffic_per_reference_byte"] = round(r["traffic"] / r["bytes_per_launch"], 4)
+        rl[name] = r
+    top = max(rl, key=lambda x: rl[x]["seconds"])
+    roofline = rl[top]
+    other = {k: v for k, v in rl.items() if k != top}
+    # achievable copy bandwidth on this GPU (STREAM-style, read + write) and
+    # the rate of random returning 32-bit atomicMin over 16 GB (the dense
+    # slots' update; the expansion reads a slot first and updates it only
+    # when the edge lowers it)
+    import ctypes
+    gbps = ctypes.c_double()
+    if H.lib().hsdl_device_copy_bench(1 << 30, 5, ctypes.byref(gbps)) == 0:
+        roofline["copy_gbps"] = round(gbps.value, 1)
+    gops = ctypes.c_double()
+    if H.lib().hsdl_device_atomic_bench(16 << 30, 1 << 30, ctypes.byref(gops)) == 0:
+        ex = rl["k_expand"]
+        ex["atomic_peak_gops"] = round(gops.value, 2)
+        ex["slot_gops"] = round(stats["scanned_edges"] / stats["t_expand_s"] / 1e9, 2) \
+            if stats["t_expand_s"] > 0 else None
+    if stats["t_clean_s"] > 0:
+        cown = 8 * stats["labels_uncleaned"] + \
+            (stats.get("clean_label_bytes") or 8) * stats["clean_label_reads"]
+        cb = clean_bytes if clean_bytes else cown
+        ca = cb / stats["t_clean_s"] / 1e9
+        other["clean"] = {
+            "bound": "hbm", "achieved": round(ca, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ca / HBM_PEAK_GBS, 4), "bytes": "reference algorithm" if clean_bytes
+            else "kernel stream", "bytes_per_build": cb, "seconds": stats["t_clean_s"],
+            "kernel_stream": {"bytes_per_build": cown,
+                              "frac": round(cown / stats["t_clean_s"] / 1e9 / HBM_PEAK_GBS, 4)},
+            "note": "host-timed clean (both passes, compaction)"}
+        tr = profile_traffic("k_clean_tile", config, K)
+        if tr and tr.get("hbm_bytes_per_build"):
+            other["clean"]["traffic"] = tr["hbm_bytes_per_build"]
+    return roofline, other
+
+
+def plain_build(H, g, K, deadline, labels):
+    """The plain one-hub-per-launch form of the same build (generation +
+    clean), stopped at `deadline` seconds (0: to the end); a stopped build
+    gives a lower bound."""
+    os.environ.setdefault("HSDL_PROGRESS", "30")  # the plain build reports as it goes
+    t0 = time.perf_counter()
+    try:
+        pidx = H.hop_constrained_two_hop_labels_generation(g, K, mode=H.MODE_PLAIN,
+                                                           deadline_s=deadline)
+        pidx.clean()
+        psec = time.perf_counter() - t0
+        out = {"build_s": psec, "labels": pidx.label_count(),
+               "same_label_count": pidx.label_count() == labels, "completed": True}
+        pidx.free()
+    except H.HSDLError as e:
+        if e.code != -5:  # HSDL_ETIMEDOUT
+            raise
+        out = {"build_s": None, "completed": False, "lower_bound_s": time.perf_counter() - t0}
+    return out
+
+
+def scale20_leg(H, torch, args, warmup=1, steps=3):
+    """R-MAT scale 20, edge factor 16, weights U[1,100], K=5: batched build +
+    clean (1 untimed + 3 timed), labels before and after the clean, the
+    build rooflines, serving on the cleaned index at the same rates, and the
+    plain form stopped at a deadline (a lower bound on batched vs plain)."""
+    cfg = CONFIGS["rmat20"]
+    K = cfg["K"]
+    t0 = time.perf_counter()
+    g = make_graph(H, cfg)
+    n, nnz = g.info()
+    g.upload(0)
+    log(f"rmat20: graph n={n} nnz={nnz} in {time.perf_counter() - t0:.1f} s")
+    for i in range(warmup):
+        idx, sec = build(H, g, K, H.MODE_BATCHED, False)
+        log(f"rmat20 warmup {i}: {sec:.3f} s")
+        idx.free()
+    times, stats = [], None
+    for i in range(steps):
+        torch.cuda.synchronize()
+        idx, sec = build(H, g, K, H.MODE_BATCHED, True)
+        torch.cuda.synchronize()
+        times.append(sec)
+        stats = idx.stats()
+        log(f"rmat20 step {i}: {sec:.3f} s; stats {stats}")
+        if i + 1 < steps:
+            idx.free()
+    build_s = float(np.mean(times))
+    rl, rl_other = rooflines(H, stats, "rmat20", K, None, None)
+    out = {"workload": cfg["desc"], "vertices": n, "csr_entries": nnz, "steps": steps,
+           "warmup": warmup, "build_s": build_s, "build_s_each": times,
+           "labels_uncleaned": stats["labels_uncleaned"],
+           "labels_cleaned": stats["labels_cleaned"], "generate_s": stats["t_generate_s"],
+           "clean_s": stats["t_clean_s"], "batches": stats["batches"],
+           "labels_per_s": stats["labels_cleaned"] / build_s, "roofline": rl,
+           "roofline_other_kernels": rl_other}
+    if args.serve == "on":
+        rates = [float(x) for x in args.rates.split(",")]
+        out["serving"] = {"rates": serving(H, idx, n, rates, args.queries),
+                          "plain_per_query": plain_per_query(H, idx, n, torch)}
+    labels = stats["labels_cleaned"]
+    idx.free()
+    if args.plain == "on":
+        p = plain_build(H, g, K, args.s20_plain_deadline, labels)
+        out["plain"] = p
+        out["batched_vs_plain"] = p["build_s"] / build_s if p["completed"] else None
+        out["batched_vs_plain_at_least"] = (p["lower_bound_s"] / build_s
+                                            if not p["completed"] else None)
+        log(f"rmat20 plain build: {p}")
+    g.release_device()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +437,11 @@ def main():
     ap.add_argument("--plain-deadline", type=float, default=0.0,
                     help="stop the plain build after this many seconds (0: run to the end); "
                          "a stopped build reports a lower bound")
+    ap.add_argument("--scale20", default="on", choices=["on", "off"],
+                    help="also measure R-MAT scale 20 (build, clean, serving, bounded plain "
+                         "build) after the headline workload")
+    ap.add_argument("--s20-plain-deadline", type=float, default=60.0)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this "
@@ -246,75 +478,9 @@ def main():
     labels = stats["labels_cleaned"]
     value = labels / (ms * 1e-3)
 
-    # rooflines of the two build kernels (HIP events on the build stream,
-    # summed over their launches); the slower one is reported as `roofline`.
-    #   k_prune:  4 or 8 B per label streamed from L(y) (the stream's label
-    #             width, prune_label_bytes) + 8 B per label staged from L(s)
-    #             + 8 B per candidate (its list entry and dense slot)
-    #   k_expand: 28 B per frontier item (item + its two CSR offsets)
-    #             + 12 B per scanned edge ((col, w) + the 4 B dense slot,
-    #             read, and updated by an atomic when the edge lowers it)
-    #             + 8 B per candidate appended
-    launches = max(1, stats["prune_launches"])  # one expand and one prune per round
-    kern = {
-        "k_prune": ((stats.get("prune_label_bytes") or 8) * stats["prune_label_reads"]
-                    + 8 * stats["prune_tile_reads"] + 8 * stats["candidates"], stats["t_prune_s"]),
-        "k_expand": (28 * stats["frontier_items"] + 12 * stats["scanned_edges"]
-                     + 8 * stats["candidates"], stats["t_expand_s"]),
-    }
-    def traffic_ratio(name):
-        # HBM bytes per algorithmic byte, measured with rocprofv3 FETCH_SIZE and
-        # WRITE_SIZE passes on this workload (profiles/<kernel>_<config>_k<K>.md)
-        tpath = os.path.join(ROOT, "profiles", f"{name}_{args.config}_k{K}_traffic.json")
-        if os.path.exists(tpath):
-            with open(tpath) as f:
-                return json.load(f).get("traffic_per_algorithmic_byte")
-        return None
-
-    rl = {}
-    for name, (nbytes, secs) in kern.items():
-        achieved = nbytes / secs / 1e9 if secs > 0 else 0.0
-        r = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-             "kernel": name, "bytes_per_launch": nbytes / launches,
-             "avg_launch_us": secs / launches * 1e6, "launches": launches, "seconds": secs}
-        ratio = traffic_ratio(name)
-        if ratio:
-            r["traffic"] = ratio * r["bytes_per_launch"]
-            r["traffic_per_algorithmic_byte"] = ratio
-        rl[name] = r
-    top = max(rl, key=lambda x: rl[x]["seconds"])
-    roofline = rl[top]
-    roofline_other = {k: v for k, v in rl.items() if k != top}
-    # achievable copy bandwidth on this GPU (STREAM-style, read + write) and
-    # the rate of random returning 32-bit atomicMin over 16 GB (the dense
-    # slots' update; the expansion reads a slot first and updates it only
-    # when the edge lowers it)
-    import ctypes
-    gbps = ctypes.c_double()
-    if H.lib().hsdl_device_copy_bench(1 << 30, 5, ctypes.byref(gbps)) == 0:
-        roofline["copy_gbps"] = round(gbps.value, 1)
-    gops = ctypes.c_double()
-    if H.lib().hsdl_device_atomic_bench(16 << 30, 1 << 30, ctypes.byref(gops)) == 0:
-        ex = rl["k_expand"]
-        ex["atomic_peak_gops"] = round(gops.value, 2)
-        # dense-slot accesses per second (one read per scanned edge, and an
-        # atomic when the edge lowers the slot)
-        ex["slot_gops"] = round(stats["scanned_edges"] / stats["t_expand_s"] / 1e9, 2) \
-            if stats["t_expand_s"] > 0 else None
-    # the clean's first pass: 8 B per label of L(v) + 4 or 8 B per label of
-    # the heads' lists L(u) it streams (the stream's label width)
-    if stats["t_clean_s"] > 0:
-        cb = 8 * stats["labels_uncleaned"] + \
-            (stats.get("clean_label_bytes") or 8) * stats["clean_label_reads"]
-        ca = cb / stats["t_clean_s"] / 1e9
-        cr = traffic_ratio("k_clean_tile")
-        roofline_other["k_clean_tile"] = {
-            "bound": "hbm", "achieved": round(ca, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ca / HBM_PEAK_GBS, 4), "traffic": cr * cb if cr else None, "bytes": cb,
-            "seconds": stats["t_clean_s"], "note": "host-timed clean (both passes, compaction)"}
-        if cr:
-            roofline_other["k_clean_tile"]["traffic_per_algorithmic_byte"] = cr
+    roofline, roofline_other = rooflines(H, stats, args.config, K, ref_counters(args.config),
+                                         clean_reference_bytes(H, g, K) if args.config in
+                                         ALG_FIXTURES else None)
 
     # queries on the device-resident cleaned index
     lens = label_lengths(idx)
@@ -362,27 +528,19 @@ def main():
     plain = None
     if args.plain == "on":
         idx.free()
-        os.environ.setdefault("HSDL_PROGRESS", "30")  # the plain build reports as it goes
-        t0 = time.perf_counter()
-        try:
-            pidx = H.hop_constrained_two_hop_labels_generation(
-                g, K, mode=H.MODE_PLAIN, deadline_s=args.plain_deadline)
-            pidx.clean()
-            psec = time.perf_counter() - t0
-            plain = {"build_s": psec, "labels": pidx.label_count(),
-                     "same_label_count": pidx.label_count() == labels, "completed": True}
-            pidx.free()
-        except H.HSDLError as e:
-            if e.code != -5:  # HSDL_ETIMEDOUT
-                raise
-            plain = {"build_s": None, "completed": False,
-                     "lower_bound_s": time.perf_counter() - t0}
+        plain = plain_build(H, g, K, args.plain_deadline, labels)
         log(f"plain build: {plain}")
 
-    cpu = cpu_baseline() if args.cpu_baseline == "on" else None
+    cpu = cpu_baseline(g, K, n, args.cpu_seconds) if args.cpu_baseline == "on" else None
     if cpu:
         cpu["query"] = cpu_q
         log(f"cpu baseline: {cpu}")
+
+    s20 = None
+    if args.scale20 == "on" and args.config == "rmat18":
+        g.release_device()
+        del g
+        s20 = scale20_leg(H, torch, args)
 
     line = {
         "metric": METRIC, "value": value, "unit": "labels/s", "n_gpus": 1, "steps": args.steps,
@@ -399,7 +557,7 @@ def main():
         "batched_vs_plain_at_least": (plain["lower_bound_s"] / (ms / 1e3))
         if plain and not plain["completed"] else None,
         "query": query, "paths": paths, "serving": serve, "roofline": roofline,
-        "roofline_other_kernels": roofline_other, "cpu_baseline": cpu,
+        "roofline_other_kernels": roofline_other, "cpu_baseline": cpu, "rmat20": s20,
     }
     print(json.dumps(line), flush=True)
 

@@ -100,6 +100,8 @@ _SIGNATURES = [
     ("hsdl_free_graph", None, [_vp]),
     ("hsdl_opts_init", None, [ctypes.POINTER(hsdl_opts)]),
     ("hsdl_generate", ctypes.c_int, [_vp, ctypes.c_uint8, ctypes.POINTER(hsdl_opts), ctypes.POINTER(_vp)]),
+    ("hsdl_generate_group", ctypes.c_int, [_vp, ctypes.c_uint8, ctypes.POINTER(hsdl_opts), _u32p,
+                                           ctypes.c_uint64, ctypes.POINTER(_vp)]),
     ("hsdl_clean", ctypes.c_int, [_vp]),
     ("hsdl_index_stats", ctypes.c_int, [_vp, ctypes.POINTER(hsdl_stats)]),
     ("hsdl_query", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
@@ -124,6 +126,7 @@ _SIGNATURES = [
     ("hsdl_server_destroy", None, [_vp]),
     ("hsdl_server_replay", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _u64p, _i32p, _u64p]),
     ("hsdl_device_copy_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
+    ("hsdl_device_gather_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     ("hsdl_device_atomic_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     ("hsdl_strerror", ctypes.c_char_p, [ctypes.c_int]),
     ("hsdl_last_error", ctypes.c_char_p, []),
@@ -340,6 +343,14 @@ class Index:
                                         counts))
         return edges, counts
 
+    def hubs(self):
+        """(off[n+1], hub) only."""
+        n, _ = self.info()
+        off = np.zeros(n + 1, np.uint64)
+        hub = np.zeros(self.label_count(), np.uint32)
+        _check(lib().hsdl_export_labels(self.h, off, hub.ctypes.data_as(_vp), None, None))
+        return off, hub
+
     def offsets(self):
         """off[n+1] only (label list of v = [off[v], off[v+1]))."""
         n, _ = self.info()
@@ -402,14 +413,21 @@ def hop_constrained_two_hop_labels_generation(graph, K, mode=MODE_BATCHED, hub_b
                                               hbm_budget=0, deadline_s=0.0, device=-1,
                                               clean=False, pool_init_labels=0,
                                               arena_init_words=0, kernel_timing=False,
-                                              lds_tile_slots=0, parents=False):
+                                              lds_tile_slots=0, parents=False, hubs=None):
     """Generates the K-hop label index of `graph` on the GPU (uncleaned unless
     clean=True, which also applies hop_constrained_clean_L); parents=True
-    also gives every label its parent vertex (for path extraction)."""
+    also gives every label its parent vertex (for path extraction).  `hubs`
+    (ascending vertex ids) restricts the build to one hub group whose
+    pruning sees only the group's labels (hsdl_generate_group)."""
     o = _opts(mode, hub_batch, tile, hbm_budget, deadline_s, device, pool_init_labels,
               arena_init_words, kernel_timing, lds_tile_slots, parents)
     h = _vp()
-    _check(lib().hsdl_generate(graph.h, K, ctypes.byref(o), ctypes.byref(h)))
+    if hubs is None:
+        _check(lib().hsdl_generate(graph.h, K, ctypes.byref(o), ctypes.byref(h)))
+    else:
+        hubs = np.ascontiguousarray(hubs, np.uint32)
+        _check(lib().hsdl_generate_group(graph.h, K, ctypes.byref(o), hubs, len(hubs),
+                                         ctypes.byref(h)))
     idx = Index(h)
     if clean:
         idx.clean()

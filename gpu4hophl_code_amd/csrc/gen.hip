@@ -110,7 +110,7 @@ struct HeavyItem {
 struct GenArgs {
   uint32_t n;
   uint32_t K;
-  uint32_t s0;
+  uint32_t s0;         // the batch's first entry of the hub sequence
   uint32_t B;
   uint32_t h;
   uint32_t cur;        // frontier buffer read this round
@@ -120,6 +120,9 @@ struct GenArgs {
   uint32_t row_run;      // y-blocks per row item (one tile build per item)
   uint32_t row_block;    // vertices per y-block (a multiple of 1024)
   uint32_t lds_slots;  // LDS tile capacity (slots)
+  // hub ids of a group build, ascending (hub slot b of the batch is
+  // hubs[s0 + b]); nullptr: every vertex is a hub (slot b is hub s0 + b)
+  const uint32_t *hubs;
   const unsigned long long *off;
   const uint2 *adj;
   unsigned long long *pool;
@@ -156,6 +159,11 @@ struct GenArgs {
 };
 
 __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
+
+// vertex id of hub slot b of the running batch
+__device__ inline uint32_t hub_of(const GenArgs &a, uint32_t b) {
+  return a.hubs ? a.hubs[a.s0 + b] : a.s0 + b;
+}
 
 // 32-bit labels of the pool copy: hub << (hop_bits + dist_bits) | hop <<
 // dist_bits | dist.
@@ -233,7 +241,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t val) {
 __global__ void k_batch_begin(GenArgs a) {
   Item *F = frontier(a, a.cur);
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x)
-    F[b] = Item{b, a.s0 + b, 0u};
+    F[b] = Item{b, hub_of(a, b), 0u};
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->fcnt[a.cur] = a.B;
 }
 
@@ -381,7 +389,7 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
       ib = it.b;
       id = it.d;
       const unsigned long long e0 = a.off[it.y], e1 = a.off[it.y + 1];
-      beg = first_above(a.adj, e0, e1, a.s0 + it.b);  // rank pruning y > s
+      beg = first_above(a.adj, e0, e1, hub_of(a, it.b));  // rank pruning y > s
       deg = (uint32_t)(e1 - beg);
       if (deg > kHeavyDeg) {
         const uint32_t pieces = (deg + kHeavyPiece - 1) / kHeavyPiece;
@@ -892,7 +900,7 @@ k_prune(GenArgs a) {
       hi = min(hi, lo + step);
     }
     const uint32_t b = lo;
-    const uint32_t s = a.s0 + b;
+    const uint32_t s = hub_of(a, b);
     const uint32_t c_beg = (w - a.chunk_off[b]) * a.tile;
     const uint32_t c_end = min(a.hub_ccnt[b], c_beg + a.tile);
     // the staged labels of earlier items are published once this item's
@@ -1000,7 +1008,7 @@ k_prune_rows(GenArgs a) {
     const uint32_t b = sh_b;
     if (b == kInf32) break;
     const uint32_t k = sh_k;
-    const uint32_t s = a.s0 + b;
+    const uint32_t s = hub_of(a, b);
     const uint32_t cap = tile_cap_for(a.seg_len[s]);
     const bool in_lds = cap <= a.lds_slots;
     const uint32_t lcap = in_lds ? lds_tile_cap(a.seg_len[s], a.lds_slots) : cap;
@@ -1114,8 +1122,9 @@ __global__ void __launch_bounds__(256) k_append(GenArgs a) {
     const uint32_t p = a.seg_len[it.y] + atomicAdd(&a.add_cur[it.y], 1u);
     if (p < a.seg_cap[it.y]) {
       const unsigned long long at = a.seg_start[it.y] + p;
-      a.pool[at] = pack_label(a.s0 + it.b, a.h, it.d);
-      if (a.pool32) a.pool32[at] = pack32(a, a.s0 + it.b, a.h, it.d);
+      const uint32_t hub = hub_of(a, it.b);
+      a.pool[at] = pack_label(hub, a.h, it.d);
+      if (a.pool32) a.pool32[at] = pack32(a, hub, a.h, it.d);
     } else {
       atomicOr(&a.ctrl->overflow, kOvfPool);
     }
@@ -1142,17 +1151,19 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
   if (mx) atomicMax(&a.ctrl->maxlen, mx);
 }
 
-// Removes the labels of hubs [s0, s0+B) (the batch being rolled back): they
-// sit at the tail of every segment, after all labels of earlier batches; the
-// self label at position 0 stays.  The redo runs under a new slot tag, so the
-// batch's dense slots need no refill.
+// Removes the labels of the batch's hubs (the batch being rolled back): they
+// sit at the tail of every segment, after all labels of earlier batches,
+// whose hubs are all smaller (hubs run in ascending order); the self label at
+// position 0 stays.  The redo runs under a new slot tag, so the batch's dense
+// slots need no refill.
 __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
+  const uint32_t lo = hub_of(a, 0), hi = hub_of(a, a.B - 1);
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
     uint32_t len = a.seg_len[v];
     const unsigned long long st = a.seg_start[v];
     while (len > 1) {
       const uint32_t hub = label_hub(a.pool[st + len - 1]);
-      if (hub < a.s0 || hub - a.s0 >= a.B) break;
+      if (hub < lo || hub > hi) break;
       --len;
     }
     a.seg_len[v] = len;
@@ -1171,6 +1182,17 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
     const unsigned long long src = a.seg_start[v], dst = out_off[v];
     const uint32_t len = a.seg_len[v];
     for (uint32_t i = lane; i < len; i += kWave) out[dst + i] = a.pool[src + i];
+  }
+}
+
+// A group build keeps only labels of its own hubs: the self label (v,0,0) at
+// position 0 of a vertex outside the group is dropped from its segment
+// before the export (in_group: one byte per vertex).
+__global__ void __launch_bounds__(256) k_drop_foreign_self(GenArgs a, const uint8_t *in_group) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
+    if (in_group[v]) continue;
+    a.seg_start[v] += 1;
+    a.seg_len[v] -= 1;
   }
 }
 
@@ -1323,10 +1345,10 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
 }
 
 // Enqueues one batch (K rounds) of hubs [s0, s0+B).
-static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K) {
+static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t nhubs) {
   // k_batch_begin writes the B hubs into the first frontier unchecked
-  if (B == 0 || B > g.a.fcap || s0 + (uint64_t)B > g.a.n)
-    return fail(HSDL_EINVAL, "hub batch does not fit the frontier list or the graph");
+  if (B == 0 || B > g.a.fcap || s0 + (uint64_t)B > nhubs)
+    return fail(HSDL_EINVAL, "hub batch does not fit the frontier list or the hub sequence");
   GenArgs a = g.a;
   a.s0 = s0;
   a.B = B;
@@ -1386,10 +1408,19 @@ struct BatchTuner {
   }
 };
 
-int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index **out) {
+int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint32_t *group,
+             uint64_t ngroup, hsdl_index **out) {
   const uint32_t n = hg->n;
   if (K < 1 || K > kMaxK) return fail(HSDL_EINVAL, "K must be in 1..15");
   if (n == 0) return fail(HSDL_EINVAL, "empty graph");
+  if (group) {
+    // a group build: hub ids ascending, no repeats, inside the graph
+    if (ngroup == 0 || ngroup > n) return fail(HSDL_EINVAL, "a hub group holds 1..n vertices");
+    for (uint64_t i = 0; i < ngroup; ++i)
+      if (group[i] >= n || (i && group[i] <= group[i - 1]))
+        return fail(HSDL_EINVAL, "group hub ids must be ascending vertex ids without repeats");
+  }
+  const uint32_t nhubs = group ? (uint32_t)ngroup : n;
   if ((uint64_t)K * (uint64_t)std::max(hg->max_w, 1) > kMaxDist)
     return fail(HSDL_ERANGE, "K * max edge weight exceeds the 27-bit label distance");
   int dev = 0;
@@ -1541,6 +1572,16 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   // tuning knob HSDL_ROW_BLOCK: vertices per y-block, rounded to 1024
   a.row_block = ((env_u32("HSDL_ROW_BLOCK", 2048, 1024, 1u << 20) + 1023) / 1024) * 1024;
   a.lds_slots = lds_slots;
+  DevBuf hubs_buf, in_group;
+  if (group) {
+    std::vector<uint8_t> mark(n, 0);
+    for (uint64_t i = 0; i < ngroup; ++i) mark[group[i]] = 1;
+    if ((rc = hubs_buf.alloc(ngroup * 4)) || (rc = in_group.alloc(n))) return rc;
+    HSDL_HIP(hipMemcpyAsync(hubs_buf.p, group, ngroup * 4, hipMemcpyHostToDevice, g.st));
+    HSDL_HIP(hipMemcpyAsync(in_group.p, mark.data(), n, hipMemcpyHostToDevice, g.st));
+    HSDL_HIP(hipStreamSynchronize(g.st));
+  }
+  a.hubs = group ? hubs_buf.as<uint32_t>() : nullptr;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);
   a.pool = g.pool.as<unsigned long long>();
@@ -1600,8 +1641,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
   const uint32_t tag_top = (uint32_t)((1ull << (32 - g.a.slot_shift)) - 2);
   uint64_t tag_next = tag_top;
   uint32_t rows_used = 0;
-  while (s0 < n) {
-    const uint32_t B = std::min<uint32_t>(tuner.B, n - s0);
+  while (s0 < nhubs) {
+    const uint32_t B = std::min<uint32_t>(tuner.B, nhubs - s0);
     const uint64_t tb = now_ns();
     if (tag_next > tag_top) {  // wrapped below zero
       hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, g.a.cand_d,
@@ -1612,7 +1653,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     }
     g.a.tag = (uint32_t)tag_next--;
     rows_used = std::max(rows_used, B);
-    if ((rc = enqueue_batch(g, s0, B, K))) return rc;
+    if ((rc = enqueue_batch(g, s0, B, K, nhubs))) return rc;
     const uint64_t te = now_ns();
     if ((rc = read_ctrl(g, c))) return rc;
     g.h_enqueue += (te - tb) * 1e-9;
@@ -1685,7 +1726,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
       const uint64_t now = now_ns();
       if (!next_progress) next_progress = t_csr + (uint64_t)(progress_s * 1e9);
       if (now >= next_progress) {
-        fprintf(stderr, "[hsdl] progress: %u of %u hubs, %llu batches, %.1f s\n", s0, n,
+        fprintf(stderr, "[hsdl] progress: %u of %u hubs, %llu batches, %.1f s\n", s0, nhubs,
                 (unsigned long long)batches, (now - t_csr) * 1e-9);
         fflush(stderr);
         next_progress = now + (uint64_t)(progress_s * 1e9);
@@ -1693,6 +1734,11 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, hsdl_index 
     }
     if (opts->deadline_s > 0 && (now_ns() - t_start) * 1e-9 > opts->deadline_s)
       return fail(HSDL_ETIMEDOUT, "generation passed its deadline");
+  }
+  if (group) {
+    hipLaunchKernelGGL(k_drop_foreign_self, dim3(1024), dim3(256), 0, g.st, a,
+                       in_group.as<uint8_t>());
+    HSDL_HIP(hipGetLastError());
   }
   const uint64_t t_gen = now_ns();
   // release batch buffers before the export needs its own
@@ -1801,10 +1847,23 @@ static int generate_body(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
     o = *opts;
   else
     hsdl_opts_init(&o);
-  return hsdl::generate(g, K, &o, out);
+  return hsdl::generate(g, K, &o, nullptr, 0, out);
 }
 
 extern "C" int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
                              hsdl_index **out) {
   return hsdl::guarded([&] { return generate_body(g, K, opts, out); });
+}
+
+extern "C" int hsdl_generate_group(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
+                                   const uint32_t *hubs, uint64_t nhubs, hsdl_index **out) {
+  return hsdl::guarded([&] {
+    if (!g || !out || !hubs) return hsdl::fail(HSDL_EINVAL, "null argument");
+    hsdl_opts o;
+    if (opts)
+      o = *opts;
+    else
+      hsdl_opts_init(&o);
+    return hsdl::generate(g, K, &o, hubs, nhubs, out);
+  });
 }

@@ -149,6 +149,19 @@ void hsdl_opts_init(hsdl_opts *opts);
  * must not overlap. */
 int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts, hsdl_index **out);
 
+/* Generation for one group of hubs, as the reference's hybrid driver runs
+ * it per vertex group: label_gen(csr, info, L, nid_vec, nid_vec_id)
+ * (Hybrid_Generation_Clean/src/label_gen.cu:603-837) clears the device
+ * labels (:647, clear_L) and seeds only the group's vertices (:183-190,
+ * init_T), so each hub's pruning sees only labels of hubs in its own group
+ * and the union over the groups holds redundant labels for the clean
+ * (Hybrid_Generation_Clean/src/test.cu:356-410).  hubs[0..nhubs) are the
+ * group's vertex ids, ascending without repeats; the index holds exactly the
+ * labels whose hub is in the group (a vertex outside it has no self label),
+ * sorted per vertex.  Same options and resident index as hsdl_generate. */
+int hsdl_generate_group(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
+                        const uint32_t *hubs, uint64_t nhubs, hsdl_index **out);
+
 /* Canonical repair in place, on the GPU.  Replaces hop_constrained_clean_L
  * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:422-511)
  * and gpu_clean (Hybrid_Generation_Clean/src/label_clean.cu:244-329;
@@ -329,6 +342,12 @@ int hsdl_device_copy_bench(uint64_t bytes, uint32_t iters, double *gbps);
  * *gops = atomics per second / 1e9 (the dense dedupe slots' operation; no
  * counterpart in the reference, used for the expansion roofline). */
 int hsdl_device_atomic_bench(uint64_t bytes, uint64_t ops, double *gops);
+
+/* Random 4-byte gathers (no counterpart in the reference): `reads` loads at
+ * uniformly random words of a `bytes` array; *gbps = 4 B per read / time.
+ * The known-traffic kernel for calibrating PMC byte counters on random
+ * access, where each load moves a whole memory sector. */
+int hsdl_device_gather_bench(uint64_t bytes, uint64_t reads, double *gbps);
 
 const char *hsdl_strerror(int code);
 const char *hsdl_last_error(void);

@@ -13,6 +13,22 @@
 //   gpu_query(...)                       (Query_mdnd/src/gpu_query.cu:84)
 //   hop_constrained_extract_shortest_path(L, s, t, k)
 //       (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels.h:224-319)
+//   hop_constrained_two_hop_labels_generation(graph, case_info, uncleaned_L)
+//       (Clean_pc/include/HBPLL/hop_constrained_two_hop_labels_generation.h:514)
+// and the GPU path of the hybrid driver (Hybrid_Generation_Clean/src/
+// test.cu:219-478):
+//   LDBC<int>, graph_v_of_v_to_LDBC, toCSR -> CSR_graph<int>
+//       (Hybrid_Generation_Clean/include/graph/ldbc.hpp:31-116,
+//        include/graph/csr_graph.hpp:13-114, src/test.cu:55-65)
+//   hop_constrained_case_info_v2::init / set_nid / destroy_L_cuda
+//       (Hybrid_Generation_Clean/include/label/global_labels_v2.cuh:62-200)
+//   label_gen(csr, info, L, nid_vec, nid_vec_id)
+//       (Hybrid_Generation_Clean/src/label_gen.cu:603-837)
+//   Graph_pool<int> (Hybrid_Generation_Clean/include/memoryManagement/graph_pool.hpp)
+//   gpu_clean_init / gpu_clean(graph, info, res, thread_num, group)
+//       (Hybrid_Generation_Clean/src/label_clean.cu:166-329)
+//   double gpu_clean(graph, input_L, res, tc, K)   (Clean_pc/src/gpu_clean.cu:140)
+//   double gpu_query(graph, L, n, que, ans, K)     (Query_mdnd/src/gpu_query.cu:84)
 // Including this header instead (namespace hsdl_compat) keeps those calls;
 // the work runs in libhsdl.so on the GPU.  Differences a caller can see:
 //   - parent_vertex is the smallest vertex before the label's vertex on a
@@ -28,6 +44,7 @@
 #define HSDL_COMPAT_HPP
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdint>
 #include <stdexcept>
@@ -304,6 +321,34 @@ void hop_constrained_two_hop_labels_generation(graph_v_of_v<weight_type> &input_
   export_labels(idx, case_info.L);
 }
 
+// The uncleaned-output form (Clean_pc/include/HBPLL/
+// hop_constrained_two_hop_labels_generation.h:514-613): uncleaned_L receives
+// the labels before the canonical repair, case_info.L those after it (when
+// case_info.use_canonical_repair).
+template <typename weight_type>
+void hop_constrained_two_hop_labels_generation(
+    graph_v_of_v<weight_type> &input_graph, hop_constrained_case_info &case_info,
+    std::vector<std::vector<hop_constrained_two_hop_label>> &uncleaned_L) {
+  const bool repair = case_info.use_canonical_repair;
+  case_info.use_canonical_repair = false;
+  hop_constrained_two_hop_labels_generation(input_graph, case_info);
+  case_info.use_canonical_repair = repair;
+  uncleaned_L = case_info.L;
+  if (!repair) return;
+  const int rc = hsdl_clean(case_info.index);
+  check(rc, "hsdl_clean");
+  hsdl_stats st;
+  hsdl_index_stats(case_info.index, &st);
+  case_info.time_canonical_repair = st.t_clean_s;
+  case_info.label_size_after_canonical_repair = (double)st.labels_cleaned;
+  case_info.canonical_repair_remove_label_ratio =
+      st.labels_uncleaned
+          ? (double)(st.labels_uncleaned - st.labels_cleaned) / st.labels_uncleaned
+          : 0.0;
+  case_info.time_total += st.t_clean_s;
+  export_labels(case_info.index, case_info.L);
+}
+
 // Canonical repair of case_info.L (any labels, e.g. from another generator).
 inline void hop_constrained_clean_L(hop_constrained_case_info &case_info, int /*thread_num*/) {
   int K = 0;
@@ -411,6 +456,345 @@ inline std::vector<std::pair<int, int>> hop_constrained_extract_shortest_path(
   hop_constrained_case_info tmp;
   tmp.index = import_labels(L, K);
   return hop_constrained_extract_shortest_path(tmp, source, terminal, hop_cst);
+}
+
+// ======================================================================
+// The GPU path of the hybrid driver (Hybrid_Generation_Clean).
+// ======================================================================
+
+// Graph in the LDBC form the driver converts to CSR
+// (Hybrid_Generation_Clean/include/graph/ldbc.hpp:31-116): OUTs / INs
+// adjacency sorted by neighbour id, add_edge overwriting a weight.
+template <typename weight_type = int>
+class LDBC {
+ public:
+  int V = 0;
+  long long E = 0;
+  std::vector<std::vector<std::pair<int, weight_type>>> OUTs, INs;
+  bool is_directed = true;
+  bool is_weight = false;
+  LDBC() = default;
+  explicit LDBC(int n) : V(n), OUTs(n), INs(n) {}
+  int size() { return V; }
+  void add_edge(int e1, int e2, weight_type ec) {
+    put(OUTs[e1], e2, ec);
+    put(INs[e2], e1, ec);
+    if (!is_directed) {
+      put(OUTs[e2], e1, ec);
+      put(INs[e1], e2, ec);
+    }
+  }
+  void clear() {
+    OUTs.clear();
+    INs.clear();
+    V = 0;
+    E = 0;
+  }
+
+ private:
+  static void put(std::vector<std::pair<int, weight_type>> &a, int key, weight_type w) {
+    auto it = std::lower_bound(a.begin(), a.end(), key,
+                               [](const std::pair<int, weight_type> &x, int k) { return x.first < k; });
+    if (it != a.end() && it->first == key)
+      it->second = w;
+    else
+      a.insert(it, {key, w});
+  }
+};
+
+// Hybrid_Generation_Clean/src/test.cu:55-65: every adjacency entry of the
+// (undirected, both directions stored) graph becomes an OUT edge.
+template <typename weight_type>
+void graph_v_of_v_to_LDBC(LDBC<weight_type> &graph, graph_v_of_v<int> &input_graph) {
+  const int N = input_graph.size();
+  for (int i = 0; i < N; ++i)
+    for (auto &e : input_graph[i]) graph.add_edge(i, e.first, (weight_type)e.second);
+}
+
+// The CSR the reference's GPU generation reads
+// (Hybrid_Generation_Clean/include/graph/csr_graph.hpp:13-114): host vectors
+// of the OUT / IN / ALL adjacency, the raw pointers the reference kept in
+// managed memory (here pointing at the host vectors), and the library graph,
+// resident in HBM, that label_gen generates from.
+template <typename weight_type = int>
+class CSR_graph {
+ public:
+  std::vector<int> INs_Neighbor_start_pointers, OUTs_Neighbor_start_pointers, ALL_start_pointers;
+  std::vector<int> INs_Edges, OUTs_Edges, all_Edges;
+  std::vector<weight_type> INs_Edge_weights, OUTs_Edge_weights;
+  int *in_pointer = nullptr, *out_pointer = nullptr, *in_edge = nullptr, *out_edge = nullptr,
+      *all_pointer = nullptr, *all_edge = nullptr;
+  int *in_edge_weight = nullptr, *out_edge_weight = nullptr;
+  int E_all = 0;
+  hsdl_graph *graph = nullptr;  // device-resident copy (hsdl_graph_upload)
+
+  void destroy_csr_graph() {
+    if (graph) hsdl_free_graph(graph);
+    graph = nullptr;
+  }
+  void point_at_vectors() {
+    in_pointer = INs_Neighbor_start_pointers.data();
+    out_pointer = OUTs_Neighbor_start_pointers.data();
+    all_pointer = ALL_start_pointers.data();
+    in_edge = INs_Edges.data();
+    out_edge = OUTs_Edges.data();
+    all_edge = all_Edges.data();
+    in_edge_weight = INs_Edge_weights.data();
+    out_edge_weight = OUTs_Edge_weights.data();
+  }
+};
+
+// toCSR (Hybrid_Generation_Clean/include/graph/csr_graph.hpp:38-114): the
+// three adjacency CSRs, and the undirected graph uploaded to HBM.  The GPU
+// generation needs OUTs symmetric (an undirected graph); a directed one is
+// refused.
+template <typename weight_type>
+CSR_graph<weight_type> toCSR(LDBC<weight_type> &graph) {
+  CSR_graph<weight_type> a;
+  const int V = graph.size();
+  a.INs_Neighbor_start_pointers.assign(V + 1, 0);
+  a.OUTs_Neighbor_start_pointers.assign(V + 1, 0);
+  a.ALL_start_pointers.assign(V + 1, 0);
+  for (int i = 0; i < V; ++i) {
+    a.INs_Neighbor_start_pointers[i] = (int)a.INs_Edges.size();
+    for (auto &x : graph.INs[i]) {
+      a.INs_Edges.push_back(x.first);
+      a.INs_Edge_weights.push_back(x.second);
+    }
+    a.OUTs_Neighbor_start_pointers[i] = (int)a.OUTs_Edges.size();
+    for (auto &x : graph.OUTs[i]) {
+      a.OUTs_Edges.push_back(x.first);
+      a.OUTs_Edge_weights.push_back(x.second);
+    }
+    // ALL: the union of both directions' neighbours, sorted, no repeats
+    a.ALL_start_pointers[i] = (int)a.all_Edges.size();
+    std::vector<int> nb;
+    for (auto &x : graph.INs[i]) nb.push_back(x.first);
+    for (auto &x : graph.OUTs[i]) nb.push_back(x.first);
+    std::sort(nb.begin(), nb.end());
+    nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+    a.all_Edges.insert(a.all_Edges.end(), nb.begin(), nb.end());
+  }
+  a.INs_Neighbor_start_pointers[V] = (int)a.INs_Edges.size();
+  a.OUTs_Neighbor_start_pointers[V] = (int)a.OUTs_Edges.size();
+  a.ALL_start_pointers[V] = (int)a.all_Edges.size();
+  a.E_all = (int)a.all_Edges.size();
+  a.point_at_vectors();
+  std::vector<uint32_t> u, v;
+  std::vector<int32_t> w;
+  for (int i = 0; i < V; ++i)
+    for (auto &x : graph.OUTs[i]) {
+      // the reverse entry must exist with the same weight
+      auto &back = graph.OUTs[x.first];
+      auto it = std::lower_bound(back.begin(), back.end(), i,
+                                 [](const std::pair<int, weight_type> &y, int k) { return y.first < k; });
+      if (it == back.end() || it->first != i || it->second != x.second)
+        throw error(HSDL_EINVAL, "toCSR: the GPU generation needs an undirected graph");
+      if (i <= x.first) {
+        u.push_back((uint32_t)i);
+        v.push_back((uint32_t)x.first);
+        w.push_back((int32_t)x.second);
+      }
+    }
+  check(hsdl_graph_from_edges((uint32_t)V, u.size(), u.data(), v.data(), w.data(), 0, &a.graph),
+        "hsdl_graph_from_edges");
+  const int rc = hsdl_graph_upload(a.graph, -1);
+  if (rc != HSDL_OK) {
+    a.destroy_csr_graph();
+    check(rc, "hsdl_graph_upload");
+  }
+  return a;
+}
+
+// Vertex groups of the hybrid driver
+// (Hybrid_Generation_Clean/include/memoryManagement/graph_pool.hpp).
+template <typename T>
+class Graph_pool {
+ public:
+  std::vector<std::vector<T>> graph_group, graph_group_bfs;
+  Graph_pool() = default;
+  explicit Graph_pool(int groups) : graph_group(groups) {}
+  int size() { return (int)graph_group.size(); }
+};
+
+// Device state of the GPU path (Hybrid_Generation_Clean/include/label/
+// global_labels_v2.cuh:14-200): the reference's pools, hash tables and
+// frontier vectors become the library's own buffers, which each call sizes;
+// init / set_nid keep the parameters and the vertex groups, destroy_L_cuda
+// releases the clean's device index (the generation's device buffers stay
+// with the CSR graph for its next group and go with destroy_csr_graph).
+class hop_constrained_case_info_v2 {
+ public:
+  int thread_num = 1;
+  int hop_cst = 0;
+  int Distributed_Graph_Num = 0;
+  int use_2023WWW_GPU_version = 0;
+  int use_new_algo = 0;
+  double time_initialization = 0, time_generate_labels = 0, time_traverse_labels = 0,
+         time_total = 0, label_size = 0;
+  int L_size = 0;
+  int G_max = 0;
+  std::vector<std::vector<int>> nid;  // vertex groups (set_nid)
+  std::vector<int> nid_size;
+  // the clean's device index (gpu_clean_init)
+  hsdl_index *clean_index = nullptr;
+  bool clean_done = false;
+
+  hop_constrained_case_info_v2() = default;
+  hop_constrained_case_info_v2(const hop_constrained_case_info_v2 &) = delete;
+  hop_constrained_case_info_v2 &operator=(const hop_constrained_case_info_v2 &) = delete;
+  ~hop_constrained_case_info_v2() { destroy_L_cuda(); }
+
+  void init(int V, int K, int g_max, int threads, std::vector<std::vector<int>> graph_group) {
+    L_size = V;
+    hop_cst = K;
+    G_max = g_max;
+    thread_num = threads;
+    if (!graph_group.empty()) set_nid((int)graph_group.size(), graph_group);
+  }
+  void set_nid(int distributed_graph_num, std::vector<std::vector<int>> graph_group) {
+    Distributed_Graph_Num = distributed_graph_num;
+    nid.assign(graph_group.begin(), graph_group.begin() + distributed_graph_num);
+    nid_size.clear();
+    for (auto &g : nid) nid_size.push_back((int)g.size());
+  }
+  int cuda_vector_size() { return L_size; }
+  void destroy_L_cuda() {
+    if (clean_index) hsdl_free_index(clean_index);
+    clean_index = nullptr;
+    clean_done = false;
+  }
+};
+
+// label_gen (Hybrid_Generation_Clean/src/label_gen.cu:603-837): the labels of
+// one vertex group's hubs, each hub's pruning seeing only labels of the
+// group (the reference clears the device labels per group, :647), appended
+// to L with hub ids in the graph's numbering and parent_vertex 0; L is
+// sorted by the caller afterwards, as in the reference's driver.
+inline void label_gen(CSR_graph<int> &input_graph, hop_constrained_case_info_v2 *info,
+                      std::vector<std::vector<hop_constrained_two_hop_label>> &L,
+                      std::vector<int> &nid_vec, int /*nid_vec_id*/) {
+  if (!input_graph.graph) throw error(HSDL_ESTATE, "label_gen: CSR graph not built by toCSR");
+  if (info->hop_cst < 1 || info->hop_cst > 15)
+    throw error(HSDL_EINVAL, "label_gen: hop_cst must be in 1..15 on the GPU path");
+  std::vector<uint32_t> hubs(nid_vec.begin(), nid_vec.end());
+  std::sort(hubs.begin(), hubs.end());
+  hubs.erase(std::unique(hubs.begin(), hubs.end()), hubs.end());
+  if (hubs.empty()) return;
+  uint32_t n = 0;
+  uint64_t nnz = 0;
+  check(hsdl_graph_info(input_graph.graph, &n, &nnz), "hsdl_graph_info");
+  if (L.size() < n) L.resize(n);
+  hsdl_opts o;
+  hsdl_opts_init(&o);
+  hsdl_index *idx = nullptr;
+  check(hsdl_generate_group(input_graph.graph, (uint8_t)info->hop_cst, &o, hubs.data(),
+                            hubs.size(), &idx),
+        "hsdl_generate_group");
+  hsdl_stats st;
+  hsdl_index_stats(idx, &st);
+  info->time_generate_labels += st.t_generate_s;
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<std::vector<hop_constrained_two_hop_label>> part;
+  try {
+    export_labels(idx, part);
+  } catch (...) {
+    hsdl_free_index(idx);
+    throw;
+  }
+  hsdl_free_index(idx);
+  uint64_t added = 0;
+  for (uint32_t v = 0; v < n; ++v) {
+    L[v].insert(L[v].end(), part[v].begin(), part[v].end());
+    added += part[v].size();
+  }
+  info->time_traverse_labels +=
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  info->label_size += (double)added / (double)n;
+}
+
+// gpu_clean_init (Hybrid_Generation_Clean/src/label_clean.cu:166-242): the
+// (uncleaned, per-vertex sorted) labels of every vertex go to the device
+// once; the groups' gpu_clean calls then hand out their vertices' cleaned
+// lists.  tc and K are the reference's hash sizing; K is also the hop bound.
+inline void gpu_clean_init(graph_v_of_v<int> & /*input_graph*/,
+                           std::vector<std::vector<hop_constrained_two_hop_label>> &input_L,
+                           hop_constrained_case_info_v2 *info_gpu, Graph_pool<int> &graph_pool,
+                           int /*tc*/, int K) {
+  if (info_gpu->clean_index) hsdl_free_index(info_gpu->clean_index);
+  info_gpu->clean_index = import_labels(input_L, K < 1 ? 1 : K);
+  info_gpu->clean_done = false;
+  info_gpu->set_nid((int)graph_pool.graph_group.size(), graph_pool.graph_group);
+}
+
+// gpu_clean (Hybrid_Generation_Clean/src/label_clean.cu:244-329): res[v] for
+// every vertex v of group nid_vec_id becomes v's canonically repaired list.
+// The repair of the whole index runs on the first call (a vertex's repair
+// reads only the input labels, so the groups' results do not depend on the
+// order or grouping of the calls).
+inline void gpu_clean(graph_v_of_v<int> & /*input_graph*/, hop_constrained_case_info_v2 *info_gpu,
+                      std::vector<std::vector<hop_constrained_two_hop_label>> &res,
+                      int /*thread_num*/, int nid_vec_id) {
+  if (!info_gpu->clean_index) throw error(HSDL_ESTATE, "gpu_clean: call gpu_clean_init first");
+  if (nid_vec_id < 0 || nid_vec_id >= (int)info_gpu->nid.size())
+    throw error(HSDL_EINVAL, "gpu_clean: no such vertex group");
+  if (!info_gpu->clean_done) {
+    check(hsdl_clean(info_gpu->clean_index), "hsdl_clean");
+    info_gpu->clean_done = true;
+  }
+  std::vector<std::vector<hop_constrained_two_hop_label>> all;
+  export_labels(info_gpu->clean_index, all);
+  if (res.size() < all.size()) res.resize(all.size());
+  for (int v : info_gpu->nid[nid_vec_id]) res[v] = all[v];
+}
+
+// Clean_pc/src/gpu_clean.cu:140: canonical repair of input_L into res on the
+// GPU; returns the GPU time in seconds.  tc is the reference's batch width.
+inline double gpu_clean(graph_v_of_v<int> & /*input_graph*/,
+                        std::vector<std::vector<hop_constrained_two_hop_label>> &input_L,
+                        std::vector<std::vector<hop_constrained_two_hop_label>> &res, int /*tc*/,
+                        int K) {
+  hsdl_index *idx = import_labels(input_L, K < 1 ? 1 : K);
+  int rc = hsdl_clean(idx);
+  if (rc != HSDL_OK) {
+    hsdl_free_index(idx);
+    check(rc, "hsdl_clean");
+  }
+  hsdl_stats st;
+  hsdl_index_stats(idx, &st);
+  try {
+    export_labels(idx, res);
+  } catch (...) {
+    hsdl_free_index(idx);
+    throw;
+  }
+  hsdl_free_index(idx);
+  return st.t_clean_s;
+}
+
+// The query module's label (Query_mdnd/include/HBPLL/cuda_vector.cuh:9-11):
+// hub v, hop h, distance d.
+struct label {
+  int v, h, d;
+};
+
+// Query_mdnd/src/gpu_query.cu:84-177: query_num (s, t, h) queries against
+// the per-vertex labels input_L (any order; K is the index's hop bound);
+// answers in ans[], INT_MAX when unreachable; returns the GPU time in
+// seconds (upload of the labels excluded, as in the reference, which times
+// the query launches only).
+inline double gpu_query(graph_v_of_v<int> & /*input_graph*/, std::vector<std::vector<label>> &input_L,
+                        int query_num, query_info *que, int *ans, int K) {
+  std::vector<std::vector<hop_constrained_two_hop_label>> L(input_L.size());
+  for (size_t v = 0; v < input_L.size(); ++v) {
+    L[v].reserve(input_L[v].size());
+    for (auto &x : input_L[v]) L[v].emplace_back(x.v, 0, x.h, x.d);
+  }
+  hop_constrained_case_info tmp;
+  tmp.index = import_labels(L, K < 1 ? 1 : K);
+  const auto t0 = std::chrono::steady_clock::now();
+  gpu_query(tmp, query_num, que, ans);
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
 }  // namespace hsdl_compat

@@ -32,14 +32,17 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 namespace {
@@ -167,7 +170,12 @@ int64_t pair_min(const std::vector<Label> &Lx, const HubTable &T, int32_t hop,
 // group, Hybrid_Generation_Clean/src/test.cu:310-328 cuts hubs into groups of
 // G_max ids): a hub's pruning query only sees labels whose hub lies in its own
 // group, which leaves redundant labels for the clean to remove.
-Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) {
+// `gid` (one group id per vertex) generalises the contiguous groups to the
+// reference's arbitrary vertex groups (Hybrid_Generation_Clean/src/test.cu:
+// 338-341, generate_Group_CDLP): a hub's query sees only labels whose hub has
+// its group id.
+Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0,
+               const int32_t *gid = nullptr) {
   const int32_t n = g.n;
   Labels *res = new Labels();
   LabelSet &L = res->L;
@@ -203,7 +211,8 @@ Labels *kround(const Graph &g, int K, int threads, int mutation, int group = 0) 
       if (F[s].empty()) return;
       // snapshot L(s) into the hub table (reference :164-170)
       for (const Label &x : L[s])
-        if (group <= 0 || x.hub / group == s / group) S.T.add(x.hub, x.dist, x.hop);
+        if (gid ? gid[x.hub] == gid[s] : (group <= 0 || x.hub / group == s / group))
+          S.T.add(x.hub, x.dist, x.hop);
       S.c_hub_reads += (int64_t)L[s].size();
       // expansion with min-dedupe (reference :172-194)
       for (const auto &fx : F[s]) {
@@ -461,6 +470,97 @@ Labels *dijkstra(const Graph &g, int K, int mutation) {
   res->accepted = 0;
   for (auto &v : L) res->accepted += (int64_t)v.size();
   return res;
+}
+
+// ---------------------------------------------------------------------------
+// The threaded CPU_HopHL generation as the reference runs it, for timing:
+// hop_constrained_two_hop_labels_generation's thread pool over hubs in id
+// order (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:
+// 560-575) running HSDL_thread_function (:45-224), with one mutex per vertex
+// guarding its label list while a hub queries or extends it (:84-91,
+// :128-146, :160-167).  Workers stop taking hubs once `seconds` have passed
+// (0: no limit) and finish the hubs they hold.  The uncleaned set depends on
+// the schedule (as the reference's does); only the time and the counts are
+// used: *hubs_done (hubs finished, a prefix in id order up to the ones still
+// in flight at the deadline) and *labels.
+// ---------------------------------------------------------------------------
+void dijkstra_timed(const Graph &g, int K, int threads, double seconds, int64_t *hubs_done,
+                    int64_t *labels) {
+  const int32_t n = g.n;
+  LabelSet L(n);
+  std::vector<std::mutex> mtx(n);
+  threads = nthreads_or_default(threads);
+  const int K1 = K + 1;
+  std::atomic<int64_t> next{0}, done{0}, made{0};
+  std::atomic<bool> stop{false};
+  const auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&]() {
+    HubTable T(n);
+    IndexedHeap Q(n, K);
+    std::vector<int64_t> changed;
+    for (;;) {
+      if (seconds > 0 && std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                                 .count() > seconds)
+        stop = true;
+      if (stop) break;
+      const int64_t vki = next.fetch_add(1);
+      if (vki >= n) break;
+      const int32_t vk = (int32_t)vki;
+      {
+        std::lock_guard<std::mutex> lk(mtx[vk]);
+        L[vk].push_back({vk, 0, 0, 0});
+        for (const Label &x : L[vk]) T.add(x.hub, x.dist, x.hop);
+      }
+      int64_t mine = 1;
+      const int64_t st0 = (int64_t)vk * K1;
+      Q.push(st0, 0);
+      changed.push_back(st0);
+      while (!Q.heap.empty()) {
+        const int64_t st = Q.pop();
+        const int32_t u = (int32_t)(st / K1);
+        const int32_t uh = (int32_t)(st % K1);
+        const int32_t P = Q.key_dist[st];
+        if (vk > u) continue;
+        int64_t q;
+        bool add;
+        {
+          std::lock_guard<std::mutex> lk(mtx[u]);
+          q = pair_min(L[u], T, uh, 0);
+          add = P < q;
+          if (add && u != vk) {
+            L[u].push_back({vk, uh, P, 0});
+            ++mine;
+          }
+        }
+        if ((add || q == 0) && uh + 1 <= K) {
+          for (int64_t e = g.off[u]; e < g.off[u + 1]; ++e) {
+            const int32_t y = g.col[e];
+            const int32_t nd = P + g.w[e];
+            const int64_t sy = (int64_t)y * K1 + (uh + 1);
+            if (Q.key_dist[sy] != INF32) {
+              if (Q.key_dist[sy] > nd && Q.pos[sy] >= 0) Q.decrease(sy, nd);
+            } else {
+              Q.push(sy, nd);
+              changed.push_back(sy);
+            }
+          }
+        }
+      }
+      for (int64_t x : changed) {
+        Q.key_dist[x] = INF32;
+        Q.pos[x] = -1;
+      }
+      changed.clear();
+      T.clear();
+      done += 1;
+      made += mine;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
+  for (auto &th : pool) th.join();
+  *hubs_done = done;
+  *labels = made;
 }
 
 // ---------------------------------------------------------------------------
@@ -794,6 +894,137 @@ Labels *wrap(LabelSet &&L) {
 }  // namespace
 
 // ------------------------------- C interface -------------------------------
+// ---------------------------------------------------------------------------
+// Checks of a finished label set too large to rebuild on the CPU (flat CSR
+// arrays, each vertex's labels sorted by (hub, hop, dist)).
+//
+// Replay of hub s: the K rounds of the K-round twin
+// (Hybrid_Generation_Clean/include/HBPLL/hop_constrained_two_hop_labels_generation.h:172-222,
+// the GPU round of Hybrid_Generation_Clean/src/label_gen.cu:477-593,87-110)
+// recomputed for s alone, with the queries reading the given set: in s's
+// round-h query only labels of hubs a < s (final once every smaller hub has
+// run) and s's own labels of hops < h can pair (a hop-h label pairs only with
+// a self label, which rank pruning rules out), so the set fixes what s's
+// round h must produce.  Returns, per sampled hub, 0 when every round's
+// accepted (y, dist) equal the set's (s, h, .) labels, else the first
+// differing round.
+// ---------------------------------------------------------------------------
+struct FlatLabels {
+  const int64_t *off;
+  const uint32_t *hub;
+  const uint8_t *hop;
+  const int32_t *dist;
+};
+
+void replay_hubs(const Graph &g, int K, const FlatLabels &L, int64_t nh, const int32_t *hubs,
+                 int threads, int32_t *bad) {
+  const int32_t n = g.n;
+  // the set's labels of the sampled hubs: per sample, (hop, y, dist)
+  std::vector<int32_t> slot(n, -1);
+  for (int64_t i = 0; i < nh; ++i) slot[hubs[i]] = (int32_t)i;
+  std::vector<std::vector<std::tuple<int32_t, int32_t, int32_t>>> have(nh);
+  for (int32_t y = 0; y < n; ++y)
+    for (int64_t j = L.off[y]; j < L.off[y + 1]; ++j) {
+      const int32_t sl = slot[L.hub[j]];
+      if (sl >= 0 && L.hop[j] > 0) have[sl].emplace_back(L.hop[j], y, L.dist[j]);
+    }
+  for (auto &v : have) std::sort(v.begin(), v.end());
+  threads = nthreads_or_default(threads);
+  struct Scratch {
+    std::vector<int32_t> cand;
+    std::vector<int32_t> touched;
+    HubTable T;
+    explicit Scratch(int32_t n) : cand(n, INF32), T(n) {}
+  };
+  std::vector<Scratch *> scr;
+  for (int t = 0; t < threads; ++t) scr.push_back(new Scratch(n));
+  parallel_for(nh, threads, [&](int tid, int64_t i) {
+    Scratch &S = *scr[tid];
+    const int32_t s = hubs[i];
+    for (int64_t j = L.off[s]; j < L.off[s + 1]; ++j) S.T.add((int32_t)L.hub[j], L.dist[j], L.hop[j]);
+    std::vector<std::pair<int32_t, int32_t>> F{{s, 0}}, acc;
+    size_t pos = 0;  // position in have[i] (sorted by hop, y)
+    bad[i] = 0;
+    for (int h = 1; h <= K && !bad[i]; ++h) {
+      for (const auto &fx : F)
+        for (int64_t e = g.off[fx.first]; e < g.off[fx.first + 1]; ++e) {
+          const int32_t y = g.col[e];
+          if (y <= s) continue;
+          const int32_t nd = fx.second + g.w[e];
+          if (nd < S.cand[y]) {
+            if (S.cand[y] == INF32) S.touched.push_back(y);
+            S.cand[y] = nd;
+          }
+        }
+      std::sort(S.touched.begin(), S.touched.end());
+      acc.clear();
+      for (int32_t y : S.touched) {
+        const int32_t dy = S.cand[y];
+        S.cand[y] = INF32;
+        int64_t q = INF32;
+        for (int64_t j = L.off[y]; j < L.off[y + 1]; ++j) {
+          const int32_t a = (int32_t)L.hub[j];
+          if (a > s || (a == s && L.hop[j] >= h)) break;
+          for (const auto &tb : S.T.t[a])
+            if (L.hop[j] + tb.second <= h) q = std::min<int64_t>(q, (int64_t)L.dist[j] + tb.first);
+        }
+        if (dy < q) acc.emplace_back(y, dy);
+      }
+      S.touched.clear();
+      // compare with the set's hop-h labels of s
+      size_t e = pos;
+      while (e < have[i].size() && std::get<0>(have[i][e]) == h) ++e;
+      bool same = (e - pos) == acc.size();
+      for (size_t k = 0; same && k < acc.size(); ++k)
+        same = std::get<1>(have[i][pos + k]) == acc[k].first &&
+               std::get<2>(have[i][pos + k]) == acc[k].second;
+      if (!same) bad[i] = h;
+      pos = e;
+      F.swap(acc);
+    }
+    if (!bad[i] && pos != have[i].size()) bad[i] = K + 1;  // labels above hop K
+    S.T.clear();
+  });
+  for (auto *x : scr) delete x;
+}
+
+// The normative clean (hop_constrained_clean_L, CPU_HopHL/include/HBPLL/
+// hop_constrained_two_hop_labels_generation.h:451-482) of sampled vertices
+// of an uncleaned set `in`, reading L(u) from the input (lu_mode 1 of
+// clean()), compared with the cleaned set `out`: bad[i] = 1 when vertex
+// verts[i]'s kept labels differ from out's list.
+void clean_check(int32_t n, const FlatLabels &in, const FlatLabels &out, int64_t nv,
+                 const int32_t *verts, int threads, int32_t *bad) {
+  threads = nthreads_or_default(threads);
+  std::vector<HubTable *> tabs;
+  for (int t = 0; t < threads; ++t) tabs.push_back(new HubTable(n));
+  parallel_for(nv, threads, [&](int tid, int64_t i) {
+    HubTable &T = *tabs[tid];
+    const int32_t v = verts[i];
+    std::vector<std::tuple<uint32_t, uint8_t, int32_t>> kept;
+    for (int64_t j = in.off[v]; j < in.off[v + 1]; ++j) {
+      const int32_t u = (int32_t)in.hub[j], h = in.hop[j], d = in.dist[j];
+      int64_t m = INF32;
+      for (int64_t k = in.off[u]; k < in.off[u + 1]; ++k)
+        for (const auto &tb : T.t[in.hub[k]])
+          if (in.hop[k] + tb.second <= h) m = std::min<int64_t>(m, (int64_t)in.dist[k] + tb.first);
+      if (m > d) {
+        kept.emplace_back(in.hub[j], in.hop[j], d);
+        T.add(u, d, h);
+      }
+    }
+    T.clear();
+    bool same = (int64_t)kept.size() == out.off[v + 1] - out.off[v];
+    for (size_t k = 0; same && k < kept.size(); ++k) {
+      const int64_t j = out.off[v] + (int64_t)k;
+      same = std::get<0>(kept[k]) == out.hub[j] && std::get<1>(kept[k]) == out.hop[j] &&
+             std::get<2>(kept[k]) == out.dist[j];
+    }
+    bad[i] = same ? 0 : 1;
+  });
+  for (auto *t : tabs) delete t;
+}
+
 extern "C" {
 
 typedef struct Labels orc_labels;
@@ -809,6 +1040,12 @@ orc_labels *orc_kround_grouped(int32_t n, const int64_t *off, const int32_t *col
                                const int32_t *w, int K, int group, int threads) {
   Graph g = make_graph(n, off, col, w);
   return kround(g, K, threads, 0, group);
+}
+
+orc_labels *orc_kround_gid(int32_t n, const int64_t *off, const int32_t *col,
+                           const int32_t *w, int K, const int32_t *gid, int threads) {
+  Graph g = make_graph(n, off, col, w);
+  return kround(g, K, threads, 0, 0, gid);
 }
 
 orc_labels *orc_2023www(int32_t n, const int64_t *off, const int32_t *col,
@@ -959,5 +1196,56 @@ void orc_graph_export(const orc_graph *g, int64_t *off, int32_t *col, int32_t *w
   std::memcpy(w, g->w.data(), sizeof(int32_t) * g->off[g->n]);
 }
 void orc_graph_free(orc_graph *g) { delete g; }
+
+// Per block of `block` vertices: label count and the wrapping sum of
+// mix(mix(mix(v << 32 | i) ^ (hub | hop << 32)) + dist) over its labels
+// (i = position in L(v), mix = the splitmix64 finaliser); the same digest
+// as tests/label_digest.py, compiled for label sets of billions.
+static inline uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+void orc_block_digests(int32_t n, const int64_t *off, const uint32_t *hub, const uint8_t *hop,
+                       const int32_t *dist, int32_t block, int threads, uint64_t *counts,
+                       uint64_t *dig) {
+  const int64_t nb = ((int64_t)n + block - 1) / block;
+  parallel_for(nb, threads, [&](int, int64_t b) {
+    const int64_t v0 = b * block, v1 = std::min<int64_t>(n, v0 + block);
+    uint64_t acc = 0;
+    for (int64_t v = v0; v < v1; ++v)
+      for (int64_t j = off[v]; j < off[v + 1]; ++j) {
+        const uint64_t a = mix64(((uint64_t)v << 32) | (uint64_t)(j - off[v]));
+        const uint64_t key = (uint64_t)hub[j] | ((uint64_t)hop[j] << 32);
+        acc += mix64(mix64(a ^ key) + (uint64_t)(uint32_t)dist[j]);
+      }
+    counts[b] = (uint64_t)(off[v1] - off[v0]);
+    dig[b] = acc;
+  });
+}
+
+void orc_dijkstra_timed(int32_t n, const int64_t *off, const int32_t *col, const int32_t *w,
+                        int K, int threads, double seconds, int64_t *hubs_done, int64_t *labels) {
+  Graph g = make_graph(n, off, col, w);
+  dijkstra_timed(g, K, threads, seconds, hubs_done, labels);
+}
+
+void orc_replay_hubs(int32_t n, const int64_t *off, const int32_t *col, const int32_t *w, int K,
+                     const int64_t *loff, const uint32_t *lhub, const uint8_t *lhop,
+                     const int32_t *ldist, int64_t nh, const int32_t *hubs, int threads,
+                     int32_t *bad) {
+  Graph g = make_graph(n, off, col, w);
+  replay_hubs(g, K, FlatLabels{loff, lhub, lhop, ldist}, nh, hubs, threads, bad);
+}
+
+void orc_clean_check(int32_t n, const int64_t *ioff, const uint32_t *ihub, const uint8_t *ihop,
+                     const int32_t *idist, const int64_t *ooff, const uint32_t *ohub,
+                     const uint8_t *ohop, const int32_t *odist, int64_t nv, const int32_t *verts,
+                     int threads, int32_t *bad) {
+  clean_check(n, FlatLabels{ioff, ihub, ihop, idist}, FlatLabels{ooff, ohub, ohop, odist}, nv,
+              verts, threads, bad);
+}
 
 }  // extern "C"

@@ -33,6 +33,22 @@ def lib():
     L.orc_kround.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.orc_kround_grouped.restype = vp
     L.orc_kround_grouped.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.orc_kround_gid.restype = vp
+    L.orc_kround_gid.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, _i32p, ctypes.c_int]
+    L.orc_block_digests.restype = None
+    L.orc_block_digests.argtypes = [ctypes.c_int32, _i64p, _u32p, _u8p, _i32p, ctypes.c_int32,
+                                    ctypes.c_int, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS"),
+                                    np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+    L.orc_dijkstra_timed.restype = None
+    L.orc_dijkstra_timed.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64)]
+    L.orc_replay_hubs.restype = None
+    L.orc_replay_hubs.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, _i64p, _u32p,
+                                  _u8p, _i32p, ctypes.c_int64, _i32p, ctypes.c_int, _i32p]
+    L.orc_clean_check.restype = None
+    L.orc_clean_check.argtypes = [ctypes.c_int32, _i64p, _u32p, _u8p, _i32p, _i64p, _u32p, _u8p,
+                                  _i32p, ctypes.c_int64, _i32p, ctypes.c_int, _i32p]
     L.orc_2023www.restype = vp
     L.orc_2023www.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int]
     L.orc_dijkstra.restype = vp
@@ -159,6 +175,13 @@ def kround_grouped(g, K, group, threads=0):
     return OracleLabels(lib().orc_kround_grouped(g.n, g.off, g.col, g.w, K, group, threads))
 
 
+def kround_gid(g, K, gid, threads=0):
+    """K-round labels with pruning restricted to hubs of the same group id
+    (gid: one id per vertex)."""
+    gid = np.ascontiguousarray(gid, np.int32)
+    return OracleLabels(lib().orc_kround_gid(g.n, g.off, g.col, g.w, K, gid, threads))
+
+
 def www2023(g, K, mutation=0):
     """The 2023WWW per-hub level search, single-threaded in hub order."""
     return OracleLabels(lib().orc_2023www(g.n, g.off, g.col, g.w, K, mutation))
@@ -250,3 +273,55 @@ def random_graph(rng, n, p, wmax):
     g = CSR.from_edges(n, u, v, w)
     g2, _ = reorder(g)
     return g2
+
+
+def _flat(labels):
+    off, hub, hop, dist = labels
+    return (np.ascontiguousarray(off, np.int64), np.ascontiguousarray(hub, np.uint32),
+            np.ascontiguousarray(hop, np.uint8), np.ascontiguousarray(dist, np.int32))
+
+
+def replay_hubs(g, K, labels, hubs, threads=0):
+    """Per sampled hub: 0 when its K rounds, recomputed from the finished
+    label set `labels` (off, hub, hop, dist arrays), give exactly the set's
+    labels of that hub, else the first round that differs."""
+    off, hub, hop, dist = _flat(labels)
+    hubs = np.ascontiguousarray(hubs, np.int32)
+    bad = np.zeros(len(hubs), np.int32)
+    lib().orc_replay_hubs(g.n, g.off, g.col, g.w, K, off, hub, hop, dist, len(hubs), hubs,
+                          threads, bad)
+    return bad
+
+
+def clean_check(uncleaned, cleaned, verts, threads=0):
+    """Per sampled vertex: 1 when the normative clean of its list in
+    `uncleaned` differs from its list in `cleaned` (both label arrays)."""
+    a = _flat(uncleaned)
+    b = _flat(cleaned)
+    verts = np.ascontiguousarray(verts, np.int32)
+    bad = np.zeros(len(verts), np.int32)
+    lib().orc_clean_check(len(a[0]) - 1, *a, *b, len(verts), verts, threads, bad)
+    return bad
+
+
+def block_digests(labels, block=1024, threads=0):
+    """tests/label_digest.block_digests computed in C (large label sets)."""
+    off, hub, hop, dist = _flat(labels)
+    n = len(off) - 1
+    nb = (n + block - 1) // block
+    counts = np.zeros(nb, np.uint64)
+    dig = np.zeros(nb, np.uint64)
+    lib().orc_block_digests(n, off, hub, hop, dist, block, threads, counts, dig)
+    return counts, dig
+
+
+def dijkstra_timed(g, K, threads=0, seconds=0.0):
+    """The reference's threaded CPU_HopHL generation, stopped after `seconds`
+    (0: to the end): (hubs finished, labels made, wall seconds)."""
+    import time
+    hubs = ctypes.c_int64()
+    labels = ctypes.c_int64()
+    t0 = time.perf_counter()
+    lib().orc_dijkstra_timed(g.n, g.off, g.col, g.w, K, threads, seconds, ctypes.byref(hubs),
+                             ctypes.byref(labels))
+    return hubs.value, labels.value, time.perf_counter() - t0

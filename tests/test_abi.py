@@ -161,6 +161,38 @@ def test_compat_driver_is_built_and_links():
     assert out.returncode == 3 and "no usable GPU" in out.stdout
 
 
+def test_hybrid_compat_driver_is_built_and_links():
+    """The hybrid-flow C++ driver (label_gen per vertex group, gpu_clean)
+    links libhsdl.so and, without a GPU, stops at toCSR's upload with
+    HSDL_ENODEV (no CPU fallback)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "compat_hybrid_driver")
+    assert os.path.exists(exe), "run make -C gpu4hophl_code_amd/csrc"
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("GPU present: the GPU tests run the driver")
+    except ImportError:
+        pass
+    out = subprocess.run([exe, os.path.join(ROOT, "tests", "golden", "euroroad2.txt"), "3", "300",
+                          "5", "/dev/null"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 3 and "no usable GPU" in out.stdout
+
+
+def test_dijkstra_golden_equals_kround_golden(golden_dir):
+    """The committed CPU_HopHL goldens (Dijkstra + clean) equal the K-round
+    goldens (K-round + clean) on every graph and K they share: the two
+    reference generators clean to the same labels."""
+    z = np.load(os.path.join(golden_dir, "dijkstra_k.npz"))
+    f = np.load(os.path.join(golden_dir, "fixtures_k.npz"))
+    c1 = np.load(os.path.join(golden_dir, "c1_rmat12_k4.npz"))
+    names = {"off": "lab_off", "hub": "hub", "hop": "hop", "dist": "dist"}
+    assert len(z.files) == 4 * 19
+    for k in z.files:
+        want = c1[names[k.split("|")[-1]]] if k.startswith("rmat12|") else f[k]
+        assert np.array_equal(z[k].astype(np.int64), want.astype(np.int64)), k
+
+
 def test_generation_needs_gpu_no_cpu_fallback():
     import ctypes
     L = H.lib()

@@ -1,0 +1,98 @@
+"""The reference's drivers on the C++ shim (include/hsdl_compat.hpp), and
+the GPU's cleaned labels against the CPU_HopHL path (Dijkstra generation +
+canonical clean) restated by the oracle.  Exact comparisons."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gpu4hophl_code_amd as H
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def read_dump(path):
+    """The reference's label dump (Clean_pc/main.cu:11-57) as arrays."""
+    raw = open(path, "rb").read()
+    pos = 0
+    n = int(np.frombuffer(raw, np.uint64, 1, pos)[0])
+    pos += 8
+    off = [0]
+    recs = []
+    for _ in range(n):
+        c = int(np.frombuffer(raw, np.uint64, 1, pos)[0])
+        pos += 8
+        r = np.frombuffer(raw, np.int32, 4 * c, pos).reshape(c, 4)
+        pos += 16 * c
+        recs.append(r)
+        off.append(off[-1] + c)
+    assert pos == len(raw)
+    r = np.concatenate(recs) if recs else np.zeros((0, 4), np.int32)
+    return np.array(off, np.int64), r[:, 0].astype(np.uint32), r[:, 2].astype(np.uint8), r[:, 3]
+
+
+@pytest.mark.parametrize("name,K,gmax", [("simple_iterative_tests.txt", 4, 50),
+                                         ("simple_iterative_tests.txt", 5, 200),
+                                         ("euroroad2.txt", 5, 300)])
+def test_hybrid_driver_groups_match_the_oracle(tmp_path, golden_dir, name, K, gmax):
+    """Hybrid_Generation_Clean's GPU flow (toCSR, case_info_v2, label_gen per
+    vertex group on four GPU cores, gpu_clean_init + gpu_clean per group)
+    gives the oracle's group-restricted K-round labels before the clean and
+    their canonical clean after it, label for label."""
+    exe = os.path.join(ROOT, "tests", "compat_hybrid_driver")
+    out_bin = tmp_path / "labels.bin"
+    out = subprocess.run([exe, os.path.join(golden_dir, name), str(K), str(gmax), "300",
+                          str(out_bin)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "check pass" in out.stdout
+    c, _ = O.reorder(O.load_txt(os.path.join(golden_dir, name)))
+    groups = (c.n + gmax - 1) // gmax
+    per = (c.n - 1) // groups + 1
+    gid = (np.arange(c.n) // per).astype(np.int32)
+    red = O.kround_gid(c, K, gid)
+    cl = O.clean(red)
+    line = [l for l in out.stdout.splitlines() if l.startswith("groups")][0]
+    assert f"labels before clean {red.total()}, after clean {cl.total()}" in line, line
+    off, hub, hop, dist = read_dump(str(out_bin))
+    woff, whub, whop, wdist = cl.arrays()
+    assert np.array_equal(off, woff) and np.array_equal(hub, whub)
+    assert np.array_equal(hop, whop) and np.array_equal(dist, wdist)
+
+
+@pytest.fixture(scope="module")
+def dijkstra_golden(golden_dir):
+    return np.load(os.path.join(golden_dir, "dijkstra_k.npz"))
+
+
+def dijkstra_cases():
+    cases = [("rmat12", 4)]
+    for nm in ("simple_iterative_tests_2023www.txt", "simple_iterative_tests.txt", "euroroad2.txt"):
+        cases += [(nm, K) for K in (1, 2, 3, 4, 5, 8)]
+    return cases
+
+
+@pytest.mark.parametrize("name,K", dijkstra_cases())
+@pytest.mark.parametrize("mode", ["batched", "plain"])
+def test_gpu_clean_labels_equal_cpu_hophl(golden_dir, dijkstra_golden, name, K, mode):
+    """The north-star parity target: the GPU's generate + clean equals the
+    CPU_HopHL path's output (Dijkstra generation + hop_constrained_clean_L,
+    CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:
+    45-224,422-511, restated by the oracle and committed as golden labels) on
+    the R-MAT scale-12 workload and the reference's three fixture graphs."""
+    if name == "rmat12":
+        g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    else:
+        g = H.Graph.load_txt(os.path.join(golden_dir, name))
+        g.reorder_by_degree()
+    m = H.MODE_PLAIN if mode == "plain" else H.MODE_BATCHED
+    idx = H.hop_constrained_two_hop_labels_generation(g, K, mode=m, clean=True)
+    off, hub, hop, dist = idx.labels()
+    key = f"{name}|{K}"
+    assert np.array_equal(off.astype(np.int64), dijkstra_golden[key + "|off"].astype(np.int64))
+    assert np.array_equal(hub, dijkstra_golden[key + "|hub"])
+    assert np.array_equal(hop, dijkstra_golden[key + "|hop"])
+    assert np.array_equal(dist, dijkstra_golden[key + "|dist"])

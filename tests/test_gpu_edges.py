@@ -248,3 +248,59 @@ def test_label_dump_byte_layout(tmp_path, c1_index):
     q.write_bytes(_dump_bytes(off, rh, rp, rd))
     back = H.Index.load(str(q), 4)
     same_labels(back, (off, hub, hop, dist))
+
+
+def _union_arrays(n, parts):
+    """Per-vertex union of several label exports, sorted by (hub, hop, dist)."""
+    vs, hs, ps, ds = [], [], [], []
+    for off, hub, hop, dist in parts:
+        off = off.astype(np.int64)
+        vs.append(np.repeat(np.arange(n, dtype=np.int64), np.diff(off)))
+        hs.append(hub)
+        ps.append(hop)
+        ds.append(dist)
+    v, hub, hop, dist = (np.concatenate(x) for x in (vs, hs, ps, ds))
+    order = np.lexsort((dist, hop, hub, v))
+    off = np.zeros(n + 1, np.int64)
+    off[1:] = np.cumsum(np.bincount(v, minlength=n))
+    return off, hub[order], hop[order], dist[order]
+
+
+@pytest.mark.parametrize("grouping", ["contiguous", "random"])
+def test_group_builds_match_the_oracle_grouping(monkeypatch, grouping):
+    """Per-group builds (hsdl_generate_group, the reference's label_gen per
+    vertex group): each group's labels and the union over the groups equal
+    the oracle's group-restricted K-round set (contiguous groups of 1000
+    hubs as the reference's driver cuts them, and random groups), through
+    the batched, small-batch, overflow-redo and plain forms; the GPU clean of
+    the union gives the oracle's clean of it."""
+    g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    c = csr_of(g)
+    K, n = 4, c.n
+    rng = np.random.default_rng(12)
+    gid = (np.arange(n) // 1000 if grouping == "contiguous" else rng.integers(0, 5, n)).astype(np.int32)
+    want = O.kround_gid(c, K, gid)
+    parts = []
+    for i, grp in enumerate(np.unique(gid)):
+        hubs = np.flatnonzero(gid == grp).astype(np.uint32)
+        kw = [dict(), dict(hub_batch=7), dict(pool_init_labels=n * 16 + 1, arena_init_words=1),
+              dict(mode=H.MODE_PLAIN)][i % 4]
+        idx = H.hop_constrained_two_hop_labels_generation(g, K, hubs=hubs, **kw)
+        off, hub, hop, dist = idx.labels()
+        assert np.isin(hub, hubs).all()
+        parts.append((off, hub, hop, dist))
+    uoff, uhub, uhop, udist = _union_arrays(n, parts)
+    woff, whub, whop, wdist = want.arrays()
+    assert np.array_equal(uoff, woff)
+    assert np.array_equal(uhub, whub) and np.array_equal(uhop, whop) and np.array_equal(udist, wdist)
+    idx = H.Index.from_labels(n, K, uoff.astype(np.uint64), uhub, uhop, udist)
+    idx.clean()
+    same_labels(idx, O.clean(want).arrays())
+
+
+def test_group_argument_errors():
+    g = H.Graph.rmat(8, 4, 1)
+    for bad in ([3, 2], [1, 1], [0, 256], []):
+        with pytest.raises(H.HSDLError) as e:
+            H.hop_constrained_two_hop_labels_generation(g, 3, hubs=np.array(bad, np.uint32))
+        assert e.value.code == -1, bad

@@ -412,3 +412,144 @@ def test_2023www_labels_are_first_hop_exact_distances_and_clean_to_canonical():
         if caught:
             break
     assert caught
+
+
+def test_group_id_generation_pins():
+    """The oracle's vertex-group generation with arbitrary group ids
+    (orc_kround_gid; the reference's per-group label_gen, whose pruning sees
+    only labels of the group's hubs):
+
+    * one group for all vertices is the plain K-round set;
+    * groups v // G are the contiguous groups of orc_kround_grouped;
+    * singleton groups leave each hub only its own labels as witnesses, so
+      the set is the closed form (s, h, D_h(y)) for every y >= s with
+      D_h(y) < D_{h-1}(y), D_h the shortest <= h-edge walk from s through
+      vertices >= s (numpy min-plus on that subgraph);
+    * any grouping cleans to the brute-force canonical labels."""
+    rng = np.random.default_rng(77)
+    for i, (g, K) in enumerate(tiny_graphs(31, 80)):
+        n = g.n
+        plain = O.kround(g, K).as_lists()
+        assert O.kround_gid(g, K, np.zeros(n, np.int32)).as_lists() == plain
+        G = 1 + i % 4
+        assert (O.kround_gid(g, K, np.arange(n, dtype=np.int32) // G).as_lists()
+                == O.kround_grouped(g, K, G).as_lists())
+        single = O.kround_gid(g, K, np.arange(n, dtype=np.int32)).as_lists()
+        W = np.full((n, n), np.inf)
+        for v in range(n):
+            for e in range(g.off[v], g.off[v + 1]):
+                W[v, g.col[e]] = min(W[v, g.col[e]], g.w[e])
+        want = [[] for _ in range(n)]
+        for s in range(n):
+            sub = W[s:, s:]
+            D = np.full(n - s, np.inf)
+            D[0] = 0.0
+            want[s].append((s, 0, 0))
+            for h in range(1, K + 1):
+                nd = np.minimum(D, np.min(D[:, None] + sub, axis=0))
+                for j in np.flatnonzero(nd < D):
+                    if j > 0:
+                        want[s + j].append((s, h, int(nd[j])))
+                D = nd
+        want = [sorted(x) for x in want]
+        assert single == want, i
+        gid = rng.integers(0, 3, n).astype(np.int32)
+        canon = O.canonical(g, K).as_lists()
+        assert O.clean(O.kround_gid(g, K, gid)).as_lists() == canon
+
+
+def test_replay_and_clean_checkers_have_teeth():
+    """The full-size checkers (orc_replay_hubs: a hub's K rounds recomputed
+    from a finished label set; orc_clean_check: the normative clean of
+    sampled vertices) accept the oracle's own sets and catch planted
+    mistakes: a changed distance, a dropped label, an extra label, and an
+    uncleaned list passed off as cleaned."""
+    rng = np.random.default_rng(99)
+    caught = {"dist": 0, "drop": 0, "extra": 0, "clean": 0}
+    for g, K in tiny_graphs(41, 60, nmax=40):
+        lab = O.kround(g, K)
+        off, hub, hop, dist = lab.arrays()
+        hubs = np.arange(g.n, dtype=np.int32)
+        assert not O.replay_hubs(g, K, (off, hub, hop, dist), hubs).any()
+        nonself = np.flatnonzero(hop > 0)
+        if len(nonself):
+            j = int(rng.choice(nonself))
+            d2 = dist.copy()
+            d2[j] += 1
+            bad = O.replay_hubs(g, K, (off, hub, hop, d2), hubs)
+            caught["dist"] += int(bad[hub[j]] != 0)
+            keep = np.ones(len(hub), bool)
+            keep[j] = False
+            v = np.searchsorted(off, j, side="right") - 1
+            off2 = off.copy()
+            off2[v + 1:] -= 1
+            bad = O.replay_hubs(g, K, (off2, hub[keep], hop[keep], dist[keep]), hubs)
+            caught["drop"] += int(bad[hub[j]] != 0)
+        # an extra label (hub 0, hop K) appended at the end of the last vertex
+        # when it has no hub-0 hop-K label
+        v = g.n - 1
+        if v > 0 and not ((hub[off[v]:off[v + 1]] == 0) & (hop[off[v]:off[v + 1]] == K)).any():
+            ins = off[v + 1]
+            pos = off[v] + np.searchsorted(hub[off[v]:off[v + 1]], 1)  # after hub 0's labels
+            hub3 = np.insert(hub, pos, 0)
+            hop3 = np.insert(hop, pos, K)
+            dist3 = np.insert(dist, pos, 1)
+            off3 = off.copy()
+            off3[v + 1:] += 1
+            bad = O.replay_hubs(g, K, (off3, hub3, hop3, dist3), hubs)
+            caught["extra"] += int(bad[0] != 0)
+            assert ins >= pos
+        red = O.kround_grouped(g, K, 2)
+        cl = O.clean(red)
+        verts = np.arange(g.n, dtype=np.int32)
+        assert not O.clean_check(red.arrays(), cl.arrays(), verts).any()
+        if red.total() > cl.total():
+            caught["clean"] += int(O.clean_check(red.arrays(), red.arrays(), verts).any())
+    assert all(c > 10 for c in caught.values()), caught
+
+
+def test_label_digest_python_and_c_agree():
+    """The label-set digest of the full-size fixtures has two
+    implementations (numpy in tests/label_digest.py, C in the oracle library
+    for billions of labels); they agree, and a changed, moved or missing
+    label changes the digest of its block only."""
+    import label_digest as D
+    rng = np.random.default_rng(5)
+    for g, K in tiny_graphs(51, 20, nmax=60):
+        off, hub, hop, dist = O.kround(g, K).arrays()
+        for block in (1, 7, 1024):
+            c1, d1 = D.block_digests(off, hub, hop, dist, block=block, chunk=97)
+            c2, d2 = O.block_digests((off, hub, hop, dist), block=block)
+            assert np.array_equal(c1, c2) and np.array_equal(d1, d2)
+        j = int(rng.integers(0, len(hub)))
+        v = np.searchsorted(off, j, side="right") - 1
+        d3 = dist.copy()
+        d3[j] ^= 1
+        _, dd = O.block_digests((off, hub, hop, d3), block=1)
+        _, d0 = O.block_digests((off, hub, hop, dist), block=1)
+        assert (dd != d0).sum() == 1 and dd[v] != d0[v]
+        if off[v + 1] - off[v] >= 2:  # two labels of a vertex swapped
+            a, b = off[v], off[v] + 1
+            h4 = hub.copy(); h4[[a, b]] = h4[[b, a]]
+            p4 = hop.copy(); p4[[a, b]] = p4[[b, a]]
+            x4 = dist.copy(); x4[[a, b]] = x4[[b, a]]
+            _, d4 = O.block_digests((off, h4, p4, x4), block=1)
+            assert d4[v] != d0[v]
+
+
+def test_threaded_dijkstra_timer_counts():
+    """The time-boxed, threaded CPU_HopHL restatement (the CPU baseline's
+    timer) finishes every hub without a limit, and on one thread makes
+    exactly the sequential Dijkstra generator's labels; with a limit it
+    stops early with fewer hubs."""
+    for g, K in tiny_graphs(61, 30, nmax=40):
+        hubs, labels, _ = O.dijkstra_timed(g, K, threads=1)
+        assert hubs == g.n and labels == O.dijkstra(g, K).total()
+        hubs, labels, _ = O.dijkstra_timed(g, K, threads=4)
+        assert hubs == g.n and labels >= O.canonical(g, K).total()
+    import gpu4hophl_code_amd as H
+    r = H.Graph.rmat(14, 16, 3, 1, 100)
+    off, col, w = r.csr()
+    c = O.CSR(len(off) - 1, off.astype(np.int64), col.astype(np.int32), w)
+    hubs, labels, sec = O.dijkstra_timed(c, 5, threads=2, seconds=0.05)
+    assert 0 < hubs < c.n and labels > 0
