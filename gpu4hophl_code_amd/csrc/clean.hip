@@ -359,35 +359,68 @@ __global__ void __launch_bounds__(256) k_index_maxima(uint32_t n, const unsigned
 // minima m[x] (x = h1 + h2).  A wave min-reduction and a prefix-min over x
 // give the best pair for every hop budget; the group's labels are then
 // decided in hop order, adding the pairs through hub u itself.
+// kLong = false: lists of up to kSeqBitsPerWave labels, keep bits in LDS;
+// longer lists are left to the kLong = true launch, whose keep bits live in
+// the wave's slice of `gbits` (gwords words): read and written with
+// agent-scope atomics, so every lane sees lane 0's updates without relying
+// on the vector L1.  A hub group of more than 32 labels (impossible with one
+// label per (hub, hop) and K <= 15, which import and generation guarantee)
+// sets bad bit 2 instead of overflowing the group's bit mask.
+template <bool kLong>
 __global__ void __launch_bounds__(256) k_clean_seq(uint32_t K, const unsigned long long *off,
                                                    const unsigned long long *lab, uint8_t *keep,
                                                    uint32_t *new_len, const uint32_t *list,
-                                                   const uint32_t *list_cnt, uint32_t *bad) {
-  __shared__ uint32_t bits_all[kCleanWaves * kSeqWordsPerWave];
+                                                   const uint32_t *list_cnt, uint32_t *bad,
+                                                   uint32_t *gbits, uint32_t gwords) {
+  __shared__ uint32_t bits_all[kLong ? 1 : kCleanWaves * kSeqWordsPerWave];
   const int lane = lane_id();
   const uint32_t wv = threadIdx.x / kWave;
-  uint32_t *bits = bits_all + wv * kSeqWordsPerWave;
   const uint32_t K1 = K + 1;
   const uint32_t wave = blockIdx.x * kCleanWaves + wv;
   const uint32_t nwaves = gridDim.x * kCleanWaves;
+  uint32_t *const gb_slice = kLong ? gbits + (uint64_t)wave * gwords : nullptr;
+  auto bit_word = [&](uint32_t i) -> uint32_t {
+    if constexpr (kLong)
+      return __hip_atomic_load(gb_slice + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      return bits_all[wv * kSeqWordsPerWave + i];
+  };
+  auto bits_sync = [&]() {
+    if constexpr (kLong) {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      wave_lds_sync();
+    }
+  };
   const uint32_t cnt = *list_cnt;
   for (uint32_t li = wave; li < cnt; li += nwaves) {
     const uint32_t v = list[li];
     const unsigned long long vb = off[v];
     const uint32_t len = (uint32_t)(off[v + 1] - vb);
     const unsigned long long *Lv = lab + vb;
-    if (len > kSeqBitsPerWave) {
+    if (kLong ? (len <= kSeqBitsPerWave) : (len > kSeqBitsPerWave)) continue;  // the other launch's
+    if (kLong && (len + 31) / 32 > gwords) {
       if (lane == 0) atomicOr(bad, 1u);
       continue;
     }
-    for (uint32_t i = lane; i < (len + 31) / 32; i += kWave) bits[i] = 0;
-    wave_lds_sync();
+    for (uint32_t i = lane; i < (len + 31) / 32; i += kWave) {
+      if constexpr (kLong)
+        __hip_atomic_store(gb_slice + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      else
+        bits_all[wv * kSeqWordsPerWave + i] = 0;
+    }
+    bits_sync();
     uint32_t kept_total = 0;
     uint32_t gb = 0;
     while (gb < len) {
       const uint32_t u = label_hub(Lv[gb]);
       uint32_t ge = gb + 1;
       while (ge < len && label_hub(Lv[ge]) == u) ++ge;  // <= K+1 labels
+      if (ge - gb > 32) {  // see above: cannot happen on valid input
+        if (lane == 0) atomicOr(bad, 2u);
+        break;
+      }
       uint32_t m[kMaxK + 1];
 #pragma unroll
       for (uint32_t x = 0; x <= kMaxK; ++x) m[x] = kInf32;
@@ -403,7 +436,7 @@ __global__ void __launch_bounds__(256) k_clean_seq(uint32_t K, const unsigned lo
           for (uint32_t p = lb_hub(Lv, 0, gb, w); p < gb; ++p) {
             const unsigned long long q = Lv[p];
             if (label_hub(q) != w) break;
-            if (!((bits[p >> 5] >> (p & 31)) & 1u)) continue;
+            if (!((bit_word(p >> 5) >> (p & 31)) & 1u)) continue;
             const uint32_t x = h1 + label_hop(q);
             if (x <= K) {
               const uint32_t s = d1 + label_dist(q);
@@ -446,15 +479,20 @@ __global__ void __launch_bounds__(256) k_clean_seq(uint32_t K, const unsigned lo
       }
       if (lane == 0) {
         for (uint32_t i = gb; i < ge; ++i)
-          if ((newbits >> (i - gb)) & 1u) atomicOr(&bits[i >> 5], 1u << (i & 31));
+          if ((newbits >> (i - gb)) & 1u) {
+            if constexpr (kLong)
+              atomicOr(gb_slice + (i >> 5), 1u << (i & 31));
+            else
+              atomicOr(&bits_all[wv * kSeqWordsPerWave + (i >> 5)], 1u << (i & 31));
+          }
       }
       kept_total += __popc(newbits);
-      wave_lds_sync();
+      bits_sync();
       gb = ge;
     }
-    for (uint32_t i = lane; i < len; i += kWave) keep[vb + i] = (bits[i >> 5] >> (i & 31)) & 1u;
+    for (uint32_t i = lane; i < len; i += kWave) keep[vb + i] = (bit_word(i >> 5) >> (i & 31)) & 1u;
     if (lane == 0) new_len[v] = kept_total;
-    wave_lds_sync();
+    bits_sync();
   }
 }
 
@@ -573,15 +611,30 @@ int clean_index(hsdl_index *idx) {
     else launch(k_clean_tile<uint32_t, Stream64>, s64);
   }
   HSDL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_clean_seq, dim3(grid), dim3(kCleanWaves * kWave), 0, st, (uint32_t)idx->K,
-                     idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
-                     keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
-                     cnt + 1);
+  hipLaunchKernelGGL(k_clean_seq<false>, dim3(grid), dim3(kCleanWaves * kWave), 0, st,
+                     (uint32_t)idx->K, idx->off.as<unsigned long long>(),
+                     idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
+                     new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, cnt + 1, nullptr, 0u);
   HSDL_HIP(hipGetLastError());
+  DevBuf long_bits;
+  if (mx[0] > kSeqBitsPerWave) {
+    // vertices with more labels than an LDS bit set holds: keep bits in a
+    // global slice per wave (few waves: such vertices are rare)
+    const uint32_t lgrid = 64;
+    const uint32_t words = (mx[0] + 31) / 32;
+    if ((rc = long_bits.alloc((uint64_t)lgrid * kCleanWaves * words * 4))) return rc;
+    hipLaunchKernelGGL(k_clean_seq<true>, dim3(lgrid), dim3(kCleanWaves * kWave), 0, st,
+                       (uint32_t)idx->K, idx->off.as<unsigned long long>(),
+                       idx->lab.as<unsigned long long>(), keep.as<uint8_t>(),
+                       new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt, cnt + 1,
+                       long_bits.as<uint32_t>(), words);
+    HSDL_HIP(hipGetLastError());
+  }
   uint32_t hc[6] = {0, 0, 0, 0, 0, 0};
   HSDL_HIP(hipMemcpyAsync(hc, ctr.p, 24, hipMemcpyDeviceToHost, st));
   HSDL_HIP(hipStreamSynchronize(st));
-  if (hc[1]) return fail(HSDL_ERANGE, "a vertex holds more than 65536 labels; clean not supported");
+  if (hc[1] & 2u) return fail(HSDL_EINVAL, "a vertex holds more than 32 labels of one hub");
+  if (hc[1]) return fail(HSDL_EDEVICE, "clean: keep-bit scratch smaller than a list");
   if ((rc = scan_counts_u32(new_len.as<uint32_t>(), noff.as<uint64_t>(), n, st))) return rc;
   uint64_t total = 0;
   HSDL_HIP(hipMemcpy(&total, noff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost));

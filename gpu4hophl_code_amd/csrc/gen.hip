@@ -135,6 +135,13 @@ struct GenArgs {
   unsigned long long *seg_start;
   uint32_t *seg_len;
   uint32_t *seg_cap;
+  // two-zone lists: a vertex's labels [0, sorted_len) are sorted by hop
+  // (stably) with hcnt[y * hcnt_stride + x - 1] of them of hop <= x
+  // (x = 1..K-1); labels appended since follow in append order.  nullptr:
+  // hop zones off (every list is read whole)
+  uint32_t *sorted_len;
+  uint32_t *hcnt;
+  uint32_t hcnt_stride;  // K - 1
   uint32_t *add_cnt;
   uint32_t *add_cur;
   uint32_t *touched;
@@ -163,6 +170,41 @@ __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return whic
 // vertex id of hub slot b of the running batch
 __device__ inline uint32_t hub_of(const GenArgs &a, uint32_t b) {
   return a.hubs ? a.hubs[a.s0 + b] : a.s0 + b;
+}
+
+// The positions of a label list a round-h query can use, as one sequence
+// j = 0..len-1: the sorted zone's run [a0, a0 + na), then the append zone
+// from b0 on.  Every position lies below the list's length.
+struct ListView {
+  uint32_t a0, na, b0, len;
+  __device__ uint32_t at(uint32_t j) const { return j < na ? a0 + j : b0 + (j - na); }
+};
+
+// labels of hop <= x in y's sorted zone (the self label is the only hop-0
+// label and sits at position 0)
+__device__ inline uint32_t hop_prefix(const GenArgs &a, uint32_t y, uint32_t x) {
+  return x == 0 ? 1u : a.hcnt[(uint64_t)y * a.hcnt_stride + (x - 1)];
+}
+
+// L(y) for candidate (s, y) of round h.  A witness pairs (a, ha, .) in L(y)
+// with (a, hb, .) in L(s), ha + hb <= h: a = s pairs only s's own labels of
+// earlier rounds of this batch (append zone) with s's self label; a = y would
+// need a label of hub y > s in L(s); any other a has ha, hb >= 1.  So the
+// query needs y's labels of hop 1..h-1 and its append zone -- and none at
+// h = 1.
+__device__ inline ListView cand_view(const GenArgs &a, uint32_t len, uint32_t sorted,
+                                     uint32_t pre) {
+  if (a.h == 1) return ListView{0, 0, 0, 0};
+  if (!a.hcnt) return ListView{0, len, len, len};
+  return ListView{1, pre - 1, sorted, (pre - 1) + (len - sorted)};
+}
+// L(s) for the hub tile of round h: labels of hop <= h - 1 (the L(y) side
+// brings at least one hop) with the self label, and the append zone.
+__device__ inline ListView tile_view(const GenArgs &a, uint32_t s) {
+  const uint32_t len = a.seg_len[s];
+  if (!a.hcnt) return ListView{0, len, len, len};
+  const uint32_t sorted = a.sorted_len[s], pre = hop_prefix(a, s, a.h - 1);
+  return ListView{0, pre, sorted, pre + (len - sorted)};
 }
 
 // 32-bit labels of the pool copy: hub << (hop_bits + dist_bits) | hop <<
@@ -229,6 +271,10 @@ __global__ void k_init_labels(GenArgs a, uint32_t c0) {
     a.add_cur[v] = 0;
     a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
     if (a.pool32) a.pool32[v * c0] = pack32(a, (uint32_t)v, 0, 0);
+    if (a.hcnt) {
+      a.sorted_len[v] = 1;
+      for (uint32_t x = 0; x < a.hcnt_stride; ++x) a.hcnt[v * a.hcnt_stride + x] = 1;
+    }
   }
 }
 
@@ -619,15 +665,15 @@ __device__ inline void atomic_min_v(uint16_t *p, uint32_t v) {
 __device__ inline void atomic_min_v(uint32_t *p, uint32_t v) { atomicMin(p, v); }
 
 template <class V>
-__device__ void build_tile(const GenArgs &a, uint32_t s, uint32_t *keys, V *vals, uint32_t cap) {
+__device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, uint32_t *keys,
+                           V *vals, uint32_t cap) {
   const uint32_t K1 = a.K + 1;
   for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) keys[i] = kInf32;
   for (uint32_t i = threadIdx.x; i < cap * K1; i += blockDim.x) vals[i] = (V)TileVal<V>::kEmpty;
   __syncthreads();
   const unsigned long long st = a.seg_start[s];
-  const uint32_t len = a.seg_len[s];
-  for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) {
-    const unsigned long long l = a.pool[st + i];
+  for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
+    const unsigned long long l = a.pool[st + view.at(i)];
     const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
     uint32_t slot = hash_slot(hub, cap);
     for (;;) {
@@ -658,14 +704,13 @@ __device__ inline bool probe_label(const GenArgs &a, LT l, bool valid, uint32_t 
   return v != TileVal<V>::kEmpty && da + v <= dy;
 }
 
-// Label `lane` of a list of `len` labels, clamped to the list (a lane past
-// the end reloads the last label and is masked in the probe); an empty list
-// reads its first slot, which lies in the pool.
+// Label `lane` of a list view, clamped to it (a lane past the end reloads
+// the last label and is masked in the probe); an empty view reads the list's
+// position 0, its self label.
 template <class LT>
-__device__ inline LT first_step(const LT *L, uint32_t len) {
+__device__ inline LT first_step(const LT *L, const ListView &v) {
   const uint32_t lane = (uint32_t)lane_id();
-  const uint32_t last = len ? len - 1 : 0;
-  return L[min(lane, last)];
+  return L[v.len ? v.at(min(lane, v.len - 1)) : 0u];
 }
 
 // Witness search over L(y) = L[0, len): true when some common hub gives
@@ -679,16 +724,17 @@ __device__ inline LT first_step(const LT *L, uint32_t len) {
 // sits behind a branch and the step buffers alternate (a, b) instead of
 // being copied, so a wait leaves the step in flight alone.
 template <class V, class KP, class VP, class LT>
-__device__ bool witness_search(const GenArgs &a, const LT *L, uint32_t len, LT f0, uint32_t dy,
-                               KP keys, VP vals, uint32_t cap, uint32_t &reads) {
+__device__ bool witness_search(const GenArgs &a, const LT *L, const ListView &view, LT f0,
+                               uint32_t dy, KP keys, VP vals, uint32_t cap, uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
   const uint32_t K1 = a.K + 1;
+  const uint32_t len = view.len;
   if (len == 0) {
     reads = 0;
     return false;
   }
   const uint32_t last = len - 1;
-  auto ld = [&](uint32_t j) { return L[min(j, last)]; };
+  auto ld = [&](uint32_t j) { return L[view.at(min(j, last))]; };
   auto probe2 = [&](LT l0, LT l1, uint32_t j) {
     const bool h0 = probe_label<V>(a, l0, j + lane < len, dy, keys, vals, cap, K1);
     const bool h1 = probe_label<V>(a, l1, j + kWave + lane < len, dy, keys, vals, cap, K1);
@@ -806,29 +852,37 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
     if (c >= c_end) break;
     const uint32_t cn = min(k, c_end - c);
-    uint32_t y = 0, dy = 0, len = 0;
+    uint32_t y = 0, dy = 0, len = 0, sorted = 0, pre = 1;
     unsigned long long st = 0;
     if (lane < cn) {
       y = cands[c + lane];
       dy = slot_dist(a, a.cand_d[(uint64_t)b * a.n + y]);
       st = a.seg_start[y];
       len = a.seg_len[y];
+      if (a.hcnt && a.h > 1) {
+        sorted = a.sorted_len[y];
+        pre = hop_prefix(a, y, a.h - 1);
+      }
     }
     auto start_of = [&](uint32_t i) {
       const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)st, i);
       const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(st >> 32), i);
       return stream_pool<LT>(a) + (((unsigned long long)hi << 32) | lo);
     };
-    LT f0 = first_step(start_of(0), __builtin_amdgcn_readlane(len, 0));
+    auto view_of = [&](uint32_t i) {
+      return cand_view(a, __builtin_amdgcn_readlane(len, i), __builtin_amdgcn_readlane(sorted, i),
+                       __builtin_amdgcn_readlane(pre, i));
+    };
+    LT f0 = first_step(start_of(0), view_of(0));
     for (uint32_t i = 0; i < cn; ++i) {
       // next candidate's first step (the last one reloads its own, so the
       // loads are unconditional)
       const uint32_t nx = min(i + 1, cn - 1);
-      const LT g0 = first_step(start_of(nx), __builtin_amdgcn_readlane(len, nx));
+      const LT g0 = first_step(start_of(nx), view_of(nx));
       const uint32_t yi = __builtin_amdgcn_readlane(y, i), di = __builtin_amdgcn_readlane(dy, i);
       uint32_t reads = 0;
-      const bool pruned = witness_search<V>(a, start_of(i), __builtin_amdgcn_readlane(len, i), f0,
-                                            di, keys, vals, cap, reads);
+      const bool pruned = witness_search<V>(a, start_of(i), view_of(i), f0, di, keys, vals, cap,
+                                            reads);
       wave_reads += reads;
       if (lane == 0) {
         a.cand_d[(uint64_t)b * a.n + yi] = slot_stale(a, di);  // later rounds must undercut di
@@ -908,13 +962,14 @@ k_prune(GenArgs a) {
     if (sh_acc_n + (c_end - c_beg) > kAccSlots)
       publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
     const uint32_t *cands = a.cgroup + a.hub_off[b];
-    const uint32_t cap = tile_cap_for(a.seg_len[s]);
+    const ListView sv = tile_view(a, s);
+    const uint32_t cap = tile_cap_for(sv.len);
     if (cap <= a.lds_slots) {
-      const uint32_t lcap = lds_tile_cap(a.seg_len[s], a.lds_slots);
+      const uint32_t lcap = lds_tile_cap(sv.len, a.lds_slots);
       if (tile_b != b) {
-        build_tile<V>(a, s, lds, reinterpret_cast<V *>(lds + lcap), lcap);
+        build_tile<V>(a, s, sv, lds, reinterpret_cast<V *>(lds + lcap), lcap);
         tile_b = b;
-        tile_reads += a.seg_len[s];
+        tile_reads += sv.len;
       }
       prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds,
                           (lds_ptr<const V>)(lds + lcap), lcap, sh_next, sh_acc_n, sh_acc,
@@ -927,12 +982,12 @@ k_prune(GenArgs a) {
         __syncthreads();
         continue;
       }
-      build_tile<V>(a, s, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
+      build_tile<V>(a, s, sv, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
       // the tile was written through L2 (atomics); drop stale L1 lines of
       // this workgroup's arena slice before plain loads read it
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tile_b = kInf32;
-      tile_reads += a.seg_len[s];
+      tile_reads += sv.len;
       prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (glb_ptr<const uint32_t>)gkeys,
                           (glb_ptr<const V>)(gkeys + cap), cap, sh_next, sh_acc_n, sh_acc,
                           wave_reads);
@@ -1009,9 +1064,10 @@ k_prune_rows(GenArgs a) {
     if (b == kInf32) break;
     const uint32_t k = sh_k;
     const uint32_t s = hub_of(a, b);
-    const uint32_t cap = tile_cap_for(a.seg_len[s]);
+    const ListView sv = tile_view(a, s);
+    const uint32_t cap = tile_cap_for(sv.len);
     const bool in_lds = cap <= a.lds_slots;
-    const uint32_t lcap = in_lds ? lds_tile_cap(a.seg_len[s], a.lds_slots) : cap;
+    const uint32_t lcap = in_lds ? lds_tile_cap(sv.len, a.lds_slots) : cap;
     if (!in_lds && tile_words<V>(cap, K1) > gwords) {
       // arena too small: the host rolls the batch back, grows the arena and
       // runs the batch again
@@ -1045,16 +1101,16 @@ k_prune_rows(GenArgs a) {
       if (!built) {
         if (in_lds) {
           if (tile_b != b) {
-            build_tile<V>(a, s, lds, reinterpret_cast<V *>(lds + lcap), lcap);
+            build_tile<V>(a, s, sv, lds, reinterpret_cast<V *>(lds + lcap), lcap);
             tile_b = b;
-            tile_reads += a.seg_len[s];
+            tile_reads += sv.len;
           }
         } else {
-          build_tile<V>(a, s, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
+          build_tile<V>(a, s, sv, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
           // written through L2 (atomics): drop stale L1 lines of the slice
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
           tile_b = kInf32;
-          tile_reads += a.seg_len[s];
+          tile_reads += sv.len;
         }
         built = true;
       }
@@ -1185,6 +1241,104 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
   }
 }
 
+// Hop zones: after a batch, every list whose append zone has grown past
+// 1/alpha_div of its sorted zone is sorted again, stably by hop, and its
+// per-hop prefix counts rewritten -- so a later round's query skips the
+// labels of hops it cannot use.  A wave per list: it counts the hops of the
+// list (hop 0 is the self label), scatters the labels stably by hop into its
+// slice of `tmp` (and the 32-bit copies into tmp32), then copies them back
+// over the segment.  Lists longer than a slice (`per_wave` labels) stay as
+// they are: the query reads their append zone whole, which is always
+// correct.  Runs between batches (never while a batch can still be rolled
+// back, whose labels must stay at the tail of every list).
+__global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long *tmp,
+                                                  uint32_t *tmp32, uint32_t per_wave,
+                                                  uint32_t alpha_div) {
+  const uint32_t lane = (uint32_t)lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  unsigned long long *t64 = tmp + (uint64_t)wave * per_wave;
+  uint32_t *t32 = tmp32 ? tmp32 + (uint64_t)wave * per_wave : nullptr;
+  const uint32_t K1 = a.K + 1;
+  for (uint64_t base = (uint64_t)wave * kWave; base < a.n; base += (uint64_t)nwaves * kWave) {
+    const uint32_t v = (uint32_t)base + lane;
+    bool need = false;
+    if (v < a.n) {
+      const uint32_t len = a.seg_len[v], sorted = a.sorted_len[v];
+      need = len > sorted && (uint64_t)(len - sorted) * alpha_div > sorted && len <= per_wave;
+    }
+    unsigned long long todo = __ballot(need);
+    while (todo) {
+      const uint32_t i = (uint32_t)(__ffsll((long long)todo) - 1);
+      todo &= todo - 1;
+      const uint32_t y = (uint32_t)base + i;
+      const unsigned long long st = a.seg_start[y];
+      const uint32_t len = a.seg_len[y];
+      // 1. labels per hop (wave-uniform counters)
+      uint32_t cnt[kMaxK + 1];
+#pragma unroll
+      for (uint32_t x = 0; x <= kMaxK; ++x) cnt[x] = 0;
+      for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
+        const uint32_t j = j0 + lane;
+        const uint32_t hop = j < len ? label_hop(a.pool[st + j]) : kInf32;
+#pragma unroll
+        for (uint32_t x = 0; x <= kMaxK; ++x)
+          if (x < K1) cnt[x] += (uint32_t)__popcll(__ballot(hop == x));
+      }
+      // 2. stable scatter by hop into the wave's slice
+      uint32_t pos[kMaxK + 1];
+      uint32_t run = 0;
+#pragma unroll
+      for (uint32_t x = 0; x <= kMaxK; ++x) {
+        pos[x] = run;
+        run += x < K1 ? cnt[x] : 0u;
+      }
+      for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
+        const uint32_t j = j0 + lane;
+        unsigned long long l = 0;
+        uint32_t l32 = 0, hop = kInf32;
+        if (j < len) {
+          l = a.pool[st + j];
+          if (t32) l32 = a.pool32[st + j];
+          hop = label_hop(l);
+        }
+        uint32_t dst = 0;
+#pragma unroll
+        for (uint32_t x = 0; x <= kMaxK; ++x) {
+          if (x >= K1) continue;
+          const unsigned long long m = __ballot(hop == x);
+          if (hop == x) dst = pos[x] + (uint32_t)__popcll(m & lt_mask);
+          pos[x] += (uint32_t)__popcll(m);
+        }
+        if (j < len) {
+          t64[dst] = l;
+          if (t32) t32[dst] = l32;
+        }
+      }
+      // the slice was written by other lanes of this wave
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      // 3. back over the segment, then the zone bounds
+      for (uint32_t j = lane; j < len; j += kWave) {
+        a.pool[st + j] = t64[j];
+        if (t32) a.pool32[st + j] = t32[j];
+      }
+      if (lane == 0) {
+        uint32_t c = cnt[0];
+        for (uint32_t x = 1; x < K1 - 1; ++x) {
+          c += cnt[x];
+          a.hcnt[(uint64_t)y * a.hcnt_stride + (x - 1)] = c;
+        }
+        a.sorted_len[y] = len;
+      }
+      // the next list reuses the slice
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+  }
+}
+
 // A group build keeps only labels of its own hubs: the self label (v,0,0) at
 // position 0 of a vertex outside the group is dropped from its segment
 // before the export (in_group: one byte per vertex).
@@ -1239,6 +1393,9 @@ static int upload_graph(hsdl_graph *g, int dev) {
 struct Gen {
   GenArgs a{};
   DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
+  DevBuf sorted_len, hcnt, sort_tmp, sort_tmp32;  // hop zones
+  uint32_t sort_per_wave = 0;                     // labels per wave slice of sort_tmp
+  int sort_grid = 0;
   DevBuf cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
@@ -1256,7 +1413,7 @@ struct Gen {
   double h_grow = 0, h_enqueue = 0, h_wait = 0;  // host-side phase timers (verbose)
   uint64_t held() const {
     return fixed + pool.bytes + pool32.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes +
-           gtile.bytes;
+           gtile.bytes + sort_tmp.bytes + sort_tmp32.bytes;
   }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -1393,6 +1550,42 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t n
   return HSDL_OK;
 }
 
+// Re-sorts the lists whose append zones have grown (k_hop_sort), after a
+// batch that completed.  The wave slices of the scratch hold the longest
+// list so far (grown here); a list that outgrows them before the next growth
+// is left unsorted, which only costs reads.
+static int sort_zones(Gen &g, uint32_t maxlen, uint32_t resort_div) {
+  if (!g.sort_grid) g.sort_grid = std::max(1, g.grid / 4);  // 256 threads: 4 waves each
+  const uint64_t waves = (uint64_t)g.sort_grid * 4;
+  // tuning knob HSDL_SORT_SLICE caps the slice (longer lists stay unsorted)
+  const uint32_t slice_cap = env_u32("HSDL_SORT_SLICE", 1u << 30, 1, 1u << 30);
+  if (maxlen > g.sort_per_wave && g.sort_per_wave < slice_cap) {
+    uint32_t per = std::max<uint32_t>(1024, g.sort_per_wave);
+    while (per < maxlen) per *= 2;
+    per = std::min(per, slice_cap);
+    const bool p32 = g.a.pool32 != nullptr;
+    const uint64_t bytes = waves * per * (p32 ? 12ull : 8ull);
+    if (g.held() - g.sort_tmp.bytes - g.sort_tmp32.bytes + bytes > g.budget) {
+      // no room for bigger slices: the longer lists stay unsorted
+      if (!g.sort_per_wave) return HSDL_OK;
+    } else {
+      HSDL_HIP(hipStreamSynchronize(g.st));
+      g.sort_tmp.release();
+      g.sort_tmp32.release();
+      int rc = g.sort_tmp.alloc(waves * per * 8);
+      if (!rc && p32) rc = g.sort_tmp32.alloc(waves * per * 4);
+      if (rc) return rc;
+      g.sort_per_wave = per;
+    }
+  }
+  hipLaunchKernelGGL(k_hop_sort, dim3(g.sort_grid), dim3(256), 0, g.st, g.a,
+                     g.sort_tmp.as<unsigned long long>(),
+                     g.a.pool32 ? g.sort_tmp32.as<uint32_t>() : nullptr, g.sort_per_wave,
+                     resort_div);
+  HSDL_HIP(hipGetLastError());
+  return HSDL_OK;
+}
+
 // Chooses the next hub batch from the last one's duration (the autotuner of
 // the batched form; the plain form pins B = 1).
 struct BatchTuner {
@@ -1468,7 +1661,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       "HSDL_LIST_INIT",
       (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30), 64,
       1u << 30);
-  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5);
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5 + 4 * (uint64_t)K);
   // dense dedupe slots: at most a quarter of the budget, <= 65536 hubs
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
   uint32_t Bmax =
@@ -1548,6 +1741,17 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
     }
   }
 
+  // hop zones (tuning knob HSDL_HOP_ZONES=0 reads every list whole; they
+  // need K >= 2 and pay only when batches share lists, so not in the plain
+  // form): sorted-zone lengths and per-hop prefix counts
+  const bool zones = env_u32("HSDL_HOP_ZONES", 1, 0, 1) && K >= 2 && !plain;
+  // a list is sorted again once its append zone passes 1/resort_div of its
+  // sorted zone (HSDL_RESORT_DIV)
+  const uint32_t resort_div = env_u32("HSDL_RESORT_DIV", 8, 1, 1u << 20);
+  if (zones && ((rc = g.sorted_len.alloc(n * 4ull)) ||
+                (rc = g.hcnt.alloc((uint64_t)n * (K - 1) * 4))))
+    return rc;
+
   // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
   const uint32_t K1 = K + 1;
   g.small_vals = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1) < 0xFFFFull;
@@ -1592,6 +1796,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.seg_start = g.seg_start.as<unsigned long long>();
   a.seg_len = g.seg_len.as<uint32_t>();
   a.seg_cap = g.seg_cap.as<uint32_t>();
+  a.sorted_len = zones ? g.sorted_len.as<uint32_t>() : nullptr;
+  a.hcnt = zones ? g.hcnt.as<uint32_t>() : nullptr;
+  a.hcnt_stride = K - 1;
   a.add_cnt = g.add_cnt.as<uint32_t>();
   a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
@@ -1713,6 +1920,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       g.h_grow += (now_ns() - tg) * 1e-9;
     }
     c_ok = c;
+    if (zones && B > 1 && (rc = sort_zones(g, c.maxlen, resort_div))) return rc;
     s0 += B;
     ++batches;
     const double dt = (now_ns() - tb) * 1e-9;

@@ -257,9 +257,12 @@ static int graph_load_txt_body(const char *path, hsdl_graph **out) {
         return fail(HSDL_EIO, "bad Edge line: " + line);
       if (n < 0 || a < 0 || b < 0 || a >= n || b >= n)
         return fail(HSDL_EIO, "Edge endpoint outside |V|: " + line);
+      // std::stod then an int conversion truncates; outside int32 the
+      // conversion is undefined, so such weights are refused
+      if (!(wd > -2147483649.0 && wd < 2147483648.0)) return fail(HSDL_EIO, "weight outside int32: " + line);
       us.push_back((uint32_t)a);
       vs.push_back((uint32_t)b);
-      ws.push_back((int32_t)wd);  // std::stod then int conversion truncates
+      ws.push_back((int32_t)wd);
     }
   }
   if (n < 0) return fail(HSDL_EIO, "missing |V|= line");
@@ -304,9 +307,13 @@ static int graph_load_edgelist_body(const char *path, uint32_t flags, hsdl_graph
     pos = line.find("|V| =");
     skip = 5;
   }
-  if (pos != std::string::npos) {
+  // like the reference's reader (GPU4HSDL_EXP/CPU_HopHL/include/graph_v_of_v/
+  // graph_v_of_v.h:243-271), a header without a readable vertex count is
+  // refused; so is one outside 0..2^32-1
+  if (pos == std::string::npos) return fail(HSDL_EIO, "header has no |V|= count");
+  {
     const char *p = line.c_str() + pos + skip;
-    if (!parse_int(p, n)) n = -1;
+    if (!parse_int(p, n) || n < 0 || n > 0xFFFFFFFFll) return fail(HSDL_EIO, "bad |V|= count");
   }
   std::vector<uint32_t> us, vs;
   std::vector<int32_t> ws;
@@ -319,12 +326,12 @@ static int graph_load_edgelist_body(const char *path, uint32_t flags, hsdl_graph
     if (!parse_int(p, b) || !parse_double(p, wd)) return fail(HSDL_EIO, "bad edge line: " + line);
     if (a < 0 || b < 0 || a > 0xFFFFFFFEll || b > 0xFFFFFFFEll)
       return fail(HSDL_EIO, "vertex id out of range: " + line);
+    if (!(wd > -2147483649.0 && wd < 2147483648.0)) return fail(HSDL_EIO, "weight outside int32: " + line);
     maxid = std::max(maxid, std::max(a, b));
     us.push_back((uint32_t)a);
     vs.push_back((uint32_t)b);
     ws.push_back((int32_t)wd);
   }
-  if (n < 0) n = maxid + 1;
   if (maxid >= n) return fail(HSDL_EIO, "edge endpoint outside |V|");
   return hsdl_graph_from_edges((uint32_t)n, us.size(), us.data(), vs.data(), ws.data(), flags, out);
 }

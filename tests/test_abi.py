@@ -373,3 +373,18 @@ def test_reorder_tie_rule_is_degree_desc_then_id():
         # a planted mistake (ties by id descending) differs on these graphs
         if trial == 0:
             assert not np.array_equal(np.lexsort((-np.arange(n), -deg)), want) or n < 3
+
+
+@pytest.mark.parametrize("text", ["|V|= 99999999999999\n", "|V|= -4\n0 1 1\n", "garbage\n0 1 1\n",
+                                  "x |V|= 3\n0 1 1e30\n", "x |V|= 3\n0 1 nan\n",
+                                  "x |V|= 3\n0 5 2\n"])
+def test_edgelist_header_and_weight_errors(tmp_path, text):
+    """The .e loader refuses a header without a readable vertex count in
+    0..2^32-1 (the reference's reader exits on it), weights outside int32
+    (whose int conversion would be undefined) and endpoints outside |V|,
+    with HSDL_EIO and without sizing anything from the bad values."""
+    p = tmp_path / "bad.e"
+    p.write_text(text)
+    with pytest.raises(H.HSDLError) as e:
+        H.Graph.load_edgelist(str(p))
+    assert e.value.code == -3

@@ -304,3 +304,85 @@ def test_group_argument_errors():
         with pytest.raises(H.HSDLError) as e:
             H.hop_constrained_two_hop_labels_generation(g, 3, hubs=np.array(bad, np.uint32))
         assert e.value.code == -1, bad
+
+
+@pytest.mark.parametrize("zones,div,slice_", [("0", None, None), ("1", "1", None), ("1", "8", None),
+                                              ("1", "1048576", None), ("1", "8", "64")])
+@pytest.mark.parametrize("rows,hub_batch", [("1", 0), ("2", 7), ("0", 100)])
+def test_hop_zones(monkeypatch, golden_dir, heavy13, ragged_grid, zones, div, slice_, rows, hub_batch):
+    """Two-zone label lists (a hop-sorted zone with per-hop counts, then the
+    labels appended since; a round's query reads hops below its own and the
+    append zone): re-sorted after every batch (divisor 1), at the default 1/8
+    threshold, almost never, with lists longer than a 64-label slice left
+    unsorted, and switched off -- the labels equal the golden and oracle
+    sets in every case."""
+    import os
+    monkeypatch.setenv("HSDL_HOP_ZONES", zones)
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    if div:
+        monkeypatch.setenv("HSDL_RESORT_DIV", div)
+    if slice_:
+        monkeypatch.setenv("HSDL_SORT_SLICE", slice_)
+    z = np.load(os.path.join(golden_dir, "c1_rmat12_k4.npz"))
+    g = H.Graph.rmat(12, 8, 0x5EED0001, 1, 100)
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=hub_batch),
+                (z["lab_off"], z["hub"], z["hop"], z["dist"]))
+    g, want = heavy13
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=hub_batch), want)
+    gg, gwant = ragged_grid
+    same_labels(H.hop_constrained_two_hop_labels_generation(gg, 6, hub_batch=hub_batch), gwant)
+
+
+def test_hop_zones_with_overflow_redo_and_32bit(monkeypatch, wide32):
+    """Hop zones through the pool and arena overflow path (a batch rolled
+    back and run again after earlier batches were re-sorted) and at the
+    exact 32-bit packing."""
+    monkeypatch.setenv("HSDL_RESORT_DIV", "1")
+    g, c, K, kr, want = wide32
+    for kw in (dict(hub_batch=50, pool_init_labels=4096 * 16 + 1, arena_init_words=1),
+               dict(hub_batch=13), dict()):
+        same_labels(H.hop_constrained_two_hop_labels_generation(g, K, **kw), want)
+
+
+def test_clean_of_a_vertex_with_more_than_65536_labels():
+    """A vertex whose list outgrows the LDS keep bits of the sequential clean
+    pass (70,010 labels: hop-1 labels of hubs 0..9 and hop-2 labels of 70,000
+    further hubs, some dominated through the first ten and some by the same
+    hub at a lower hop) is cleaned through the global keep-bit variant, equal
+    to the oracle's clean; so is the rest of the index."""
+    rng = np.random.default_rng(70)
+    W, U = 10, 70000
+    n = W + U + 1
+    v = n - 1
+    offs, hubs, hops, dists = [0], [], [], []
+    for x in range(n - 1):
+        hs, ps, ds = [], [], []
+        if x >= W:  # L(u): hop-1 labels of the ten top hubs, then the self label
+            for w in range(W):
+                hs.append(w); ps.append(1); ds.append(int(rng.integers(1, 50)))
+        hs.append(x); ps.append(0); ds.append(0)
+        hubs += hs; hops += ps; dists += ds
+        offs.append(offs[-1] + len(hs))
+    # L(v): hop-1 labels of the top hubs, hop-2 labels of every other hub
+    # (some dominated through a top hub), hop-3 repeats of some (dominated by
+    # the same hub's hop-2 label), and the self label
+    for w in range(W):
+        hubs.append(w); hops.append(1); dists.append(int(rng.integers(1, 50)))
+    for u in range(W, W + U):
+        d = int(rng.integers(1, 120))
+        hubs.append(u); hops.append(2); dists.append(d)
+        if u % 7 == 0:
+            hubs.append(u); hops.append(3); dists.append(d + int(rng.integers(0, 5)))
+    hubs.append(v); hops.append(0); dists.append(0)
+    offs.append(len(hubs))
+    off = np.array(offs, np.uint64)
+    hub = np.array(hubs, np.uint32)
+    hop = np.array(hops, np.uint8)
+    dist = np.array(dists, np.int32)
+    assert off[-1] - off[-2] > 65536
+    want = O.clean(O.labels_from_arrays(n, off.astype(np.int64), hub, hop, dist))
+    assert want.total() < len(hub)
+    idx = H.Index.from_labels(n, 3, off, hub, hop, dist)
+    idx.clean()
+    assert idx.stats()["clean_passes"] == 2
+    same_labels(idx, want.arrays())
