@@ -168,6 +168,13 @@ def cpu_baseline(g, K, n, seconds=20.0):
                      f"({100.0 * hubs / n:.2f}%, DNF), {labels} uncleaned labels in {sec:.2f} s",
            "hubs_done": hubs, "hubs_total": n, "completed": hubs == n,
            "nproc": os.cpu_count(), "cpu_model": cpu_model()}
+    # the reference's other CPU generation, 2023WWW (per-hub level search
+    # without rank pruning), on the same graph and the same time box
+    wh, wl, ws = O.www2023_timed(c, K, threads=threads, seconds=seconds / 2)
+    out["www2023"] = {"labels_per_s": wl / ws, "hubs_done": wh, "hubs_total": n,
+                      "labels": wl, "seconds": ws, "threads": threads,
+                      "sample": f"2023WWW threaded generation on the bench graph, stopped at "
+                                f"{seconds / 2:.0f} s: {wh} of {n} hubs ({100.0 * wh / n:.2f}%)"}
     # the K-round twin (the GPU's own algorithm on the CPU) on a graph of the
     # same family small enough to finish
     g15 = __import__("gpu4hophl_code_amd").Graph.rmat(15, 16, 0x5EED0002, 1, 100)
@@ -442,6 +449,9 @@ def main():
                          "build) after the headline workload")
     ap.add_argument("--s20-plain-deadline", type=float, default=60.0)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--reference-bytes", default="on", choices=["on", "off"],
+                    help="count the reference clean's bytes on an extra untimed build "
+                         "(off: one build only, e.g. under a profiler)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: this "
@@ -478,9 +488,10 @@ def main():
     labels = stats["labels_cleaned"]
     value = labels / (ms * 1e-3)
 
-    roofline, roofline_other = rooflines(H, stats, args.config, K, ref_counters(args.config),
-                                         clean_reference_bytes(H, g, K) if args.config in
-                                         ALG_FIXTURES else None)
+    roofline, roofline_other = rooflines(
+        H, stats, args.config, K, ref_counters(args.config),
+        clean_reference_bytes(H, g, K)
+        if args.config in ALG_FIXTURES and args.reference_bytes == "on" else None)
 
     # queries on the device-resident cleaned index
     lens = label_lengths(idx)

@@ -128,6 +128,7 @@ _SIGNATURES = [
     ("hsdl_device_copy_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(ctypes.c_double)]),
     ("hsdl_device_gather_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
     ("hsdl_device_atomic_bench", ctypes.c_int, [ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]),
+    ("hsdl_build_id", ctypes.c_char_p, []),
     ("hsdl_strerror", ctypes.c_char_p, [ctypes.c_int]),
     ("hsdl_last_error", ctypes.c_char_p, []),
     ("hsdl_now_ns", ctypes.c_uint64, []),
@@ -169,6 +170,27 @@ def _check(rc):
 
 def now_ns():
     return lib().hsdl_now_ns()
+
+
+# the sources libhsdl.so is built from, in the order csrc/Makefile hashes them
+HASHED_SOURCES = ["index.hip", "gen.hip", "clean.hip", "query.hip", "server.hip", "util.hip",
+                  "parents.hip", "graph.cpp", "../../include/hsdl.h", "common.hpp", "device.hpp"]
+
+
+def source_hash():
+    """SHA-256 of the library's sources in this tree (hex)."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+    for name in HASHED_SOURCES:
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def build_id():
+    """SHA-256 of the sources the loaded libhsdl.so was built from."""
+    return lib().hsdl_build_id().decode()
 
 
 class Graph:

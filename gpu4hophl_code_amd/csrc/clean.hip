@@ -88,17 +88,30 @@ struct Stream64 {
   using T = unsigned long long;
   const unsigned long long *lab;
   __device__ const T *list(unsigned long long b) const { return lab + b; }
+  // elements of L(u) a group of max hop `maxh` can use: the whole list
+  __device__ uint32_t usable(uint32_t, uint32_t, uint32_t len) const { return len; }
   __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
     w = label_hub(l);
     h = label_hop(l);
     d = label_dist(l);
   }
 };
+// The 32-bit copy keeps each list stably sorted by hop (hub order within a
+// hop), with hcnt[u * K + x] = elements of L(u) of hop <= x (x < K), when
+// hcnt is set: an element pairs with the tile through hop h1 + 1 or more
+// (w != v), so a group whose deepest label has hop maxh can use only L(u)'s
+// elements of hop <= maxh - 1 -- a prefix of the sorted list, which the
+// stream stops at (the self label, hop 0, is its first element).
 struct Stream32 {
   using T = uint32_t;
   const uint32_t *lab;
   uint32_t hop_bits, dist_bits;
+  const uint32_t *hcnt;
+  uint32_t K;
   __device__ const T *list(unsigned long long b) const { return lab + b; }
+  __device__ uint32_t usable(uint32_t u, uint32_t maxh, uint32_t len) const {
+    return hcnt ? hcnt[(uint64_t)u * K + (maxh - 1)] : len;
+  }
   __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
     w = l >> (hop_bits + dist_bits);
     h = (l >> dist_bits) & ((1u << hop_bits) - 1u);
@@ -106,12 +119,70 @@ struct Stream32 {
   }
 };
 
+__device__ inline uint32_t pack32(unsigned long long l, uint32_t hop_bits, uint32_t dist_bits) {
+  return (label_hub(l) << (hop_bits + dist_bits)) | (label_hop(l) << dist_bits) | label_dist(l);
+}
+
 __global__ void k_pack32(uint64_t total, const unsigned long long *lab, uint32_t *out,
                          uint32_t hop_bits, uint32_t dist_bits) {
   for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
-       i += (uint64_t)gridDim.x * blockDim.x) {
-    const unsigned long long l = lab[i];
-    out[i] = (label_hub(l) << (hop_bits + dist_bits)) | (label_hop(l) << dist_bits) | label_dist(l);
+       i += (uint64_t)gridDim.x * blockDim.x)
+    out[i] = pack32(lab[i], hop_bits, dist_bits);
+}
+
+// The 32-bit copy with every list stably sorted by hop, and the per-hop
+// prefix counts (Stream32): a wave per vertex counts its labels' hops, then
+// scatters them in list order into their hop's run (positions by ballot).
+__global__ void __launch_bounds__(256) k_pack32_hop(uint32_t n, uint32_t K,
+                                                    const unsigned long long *off,
+                                                    const unsigned long long *lab, uint32_t *out,
+                                                    uint32_t *hcnt, uint32_t hop_bits,
+                                                    uint32_t dist_bits) {
+  const uint32_t lane = (uint32_t)lane_id();
+  const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  const uint32_t K1 = K + 1;
+  for (uint32_t v = wave; v < n; v += nwaves) {
+    const unsigned long long b = off[v];
+    const uint32_t len = (uint32_t)(off[v + 1] - b);
+    uint32_t cnt[kMaxK + 1];
+#pragma unroll
+    for (uint32_t x = 0; x <= kMaxK; ++x) cnt[x] = 0;
+    for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      const uint32_t hop = j < len ? label_hop(lab[b + j]) : kInf32;
+#pragma unroll
+      for (uint32_t x = 0; x <= kMaxK; ++x)
+        if (x < K1) cnt[x] += (uint32_t)__popcll(__ballot(hop == x));
+    }
+    uint32_t pos[kMaxK + 1];
+    uint32_t run = 0;
+#pragma unroll
+    for (uint32_t x = 0; x <= kMaxK; ++x) {
+      pos[x] = run;
+      run += x < K1 ? cnt[x] : 0u;
+      // hcnt[v * K + x]: labels of hop <= x, for x = 0..K-1
+      if (x < K && lane == 0) hcnt[(uint64_t)v * K + x] = run;
+    }
+    for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
+      const uint32_t j = j0 + lane;
+      unsigned long long l = 0;
+      uint32_t hop = kInf32;
+      if (j < len) {
+        l = lab[b + j];
+        hop = label_hop(l);
+      }
+      uint32_t dst = 0;
+#pragma unroll
+      for (uint32_t x = 0; x <= kMaxK; ++x) {
+        if (x >= K1) continue;
+        const unsigned long long m = __ballot(hop == x);
+        if (hop == x) dst = pos[x] + (uint32_t)__popcll(m & lt_mask);
+        pos[x] += (uint32_t)__popcll(m);
+      }
+      if (j < len) out[b + dst] = pack32(l, hop_bits, dist_bits);
+    }
   }
 }
 
@@ -142,7 +213,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       ++glen;
     }
     const unsigned long long ub = off[u];
-    const uint32_t ulen = (uint32_t)(off[u + 1] - ub);
+    const uint32_t ulen = str.usable(u, maxh, (uint32_t)(off[u + 1] - ub));
     if (ulen == 0) continue;
     const typename S::T *Lu = str.list(ub);
     // the group's hops and distances, decoded once per head into scalars for
@@ -578,17 +649,27 @@ int clean_index(hsdl_index *idx) {
   while ((1ull << hub_bits) <= (uint64_t)(n ? n - 1 : 0)) ++hub_bits;
   while ((1ull << hop_bits) <= (uint64_t)idx->K) ++hop_bits;
   while ((1ull << dist_bits) <= (uint64_t)mx[1]) ++dist_bits;
-  DevBuf lab32;
+  DevBuf lab32, hcnt;
   bool use32 = env_u32("HSDL_STREAM32", 1, 0, 1) && idx->total > 0 &&
                hub_bits + hop_bits + dist_bits <= 32;
+  // tuning knob HSDL_CLEAN_HOP_SORT=0 keeps the 32-bit copy in the index's
+  // (hub, hop, dist) order and streams whole lists
+  const bool hop_sort = use32 && env_u32("HSDL_CLEAN_HOP_SORT", 1, 0, 1);
   if (use32) {
     void *p = nullptr;
     if (hipMalloc(&p, idx->total * 4) == hipSuccess) {
       lab32.p = p;
       lab32.bytes = idx->total * 4;
-      hipLaunchKernelGGL(k_pack32, dim3(cus * 8), dim3(256), 0, st, (uint64_t)idx->total,
-                         idx->lab.as<unsigned long long>(), lab32.as<uint32_t>(), hop_bits,
-                         dist_bits);
+      if (hop_sort) {
+        if ((rc = hcnt.alloc((uint64_t)n * idx->K * 4))) return rc;
+        hipLaunchKernelGGL(k_pack32_hop, dim3(cus * 8), dim3(256), 0, st, n, (uint32_t)idx->K,
+                           idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
+                           lab32.as<uint32_t>(), hcnt.as<uint32_t>(), hop_bits, dist_bits);
+      } else {
+        hipLaunchKernelGGL(k_pack32, dim3(cus * 8), dim3(256), 0, st, (uint64_t)idx->total,
+                           idx->lab.as<unsigned long long>(), lab32.as<uint32_t>(), hop_bits,
+                           dist_bits);
+      }
       HSDL_HIP(hipGetLastError());
     } else {
       (void)hipGetLastError();
@@ -596,7 +677,8 @@ int clean_index(hsdl_index *idx) {
     }
   }
   const Stream64 s64{idx->lab.as<unsigned long long>()};
-  const Stream32 s32{lab32.as<uint32_t>(), hop_bits, dist_bits};
+  const Stream32 s32{lab32.as<uint32_t>(), hop_bits, dist_bits,
+                     use32 && hop_sort ? hcnt.as<uint32_t>() : nullptr, (uint32_t)idx->K};
   auto launch = [&](auto kern, auto str) {
     hipLaunchKernelGGL(kern, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n, (uint32_t)idx->K,
                        idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(), str,

@@ -115,7 +115,8 @@ enum { HSDL_MODE_BATCHED = 0, HSDL_MODE_PLAIN = 1 };
 typedef struct hsdl_opts {
   uint32_t mode;          /* HSDL_MODE_BATCHED (default) or HSDL_MODE_PLAIN  */
   uint32_t hub_batch;     /* hubs per batch; 0 = autotune                    */
-  uint32_t tile;          /* candidates per prune work item; 0 = autotune    */
+  uint32_t tile;          /* candidates per prune work item, at most 256;
+                             0 = 256                                       */
   uint64_t hbm_budget;    /* bytes the build may hold on the device; 0 = 80%
                              of free HBM at the call                          */
   double deadline_s;      /* wall-clock limit, 0 = none (HSDL_ETIMEDOUT)     */
@@ -348,6 +349,10 @@ int hsdl_device_atomic_bench(uint64_t bytes, uint64_t ops, double *gops);
  * The known-traffic kernel for calibrating PMC byte counters on random
  * access, where each load moves a whole memory sector. */
 int hsdl_device_gather_bench(uint64_t bytes, uint64_t reads, double *gbps);
+
+/* SHA-256 (hex) of the library's sources as they were built: which
+ * sources a loaded libhsdl.so comes from (no counterpart in the reference). */
+const char *hsdl_build_id(void);
 
 const char *hsdl_strerror(int code);
 const char *hsdl_last_error(void);

@@ -564,6 +564,96 @@ void dijkstra_timed(const Graph &g, int K, int threads, double seconds, int64_t 
 }
 
 // ---------------------------------------------------------------------------
+// The threaded 2023WWW generation as the reference runs it, for timing: the
+// same thread pool over hubs in id order
+// (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:
+// 560-575) running _2023WWW_thread_function (:226-370) -- per hub a
+// level-synchronous search without rank pruning, one distance-so-far per
+// vertex, a vertex labelled when it beats the pair query against the hub's
+// list taken before the search -- with a mutex per vertex around its list.
+// Stops taking hubs after `seconds` (0: no limit); returns the hubs finished
+// and the labels made (schedule-dependent, as the reference's).
+// ---------------------------------------------------------------------------
+void www2023_timed(const Graph &g, int K, int threads, double seconds, int64_t *hubs_done,
+                   int64_t *labels) {
+  const int32_t n = g.n;
+  LabelSet L(n);
+  std::vector<std::mutex> mtx(n);
+  threads = nthreads_or_default(threads);
+  std::atomic<int64_t> next{0}, done{0}, made{0};
+  std::atomic<bool> stop{false};
+  const auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&]() {
+    HubTable T(n);
+    std::vector<int32_t> dist(n, INF32), stamp(n, -1), changed, cur, nxt;
+    std::vector<std::pair<int32_t, int32_t>> updates;
+    for (;;) {
+      if (seconds > 0 && std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                                 .count() > seconds)
+        stop = true;
+      if (stop) break;
+      const int64_t vki = next.fetch_add(1);
+      if (vki >= n) break;
+      const int32_t vk = (int32_t)vki;
+      {
+        std::lock_guard<std::mutex> lk(mtx[vk]);
+        for (const Label &x : L[vk]) T.add(x.hub, x.dist, x.hop);
+      }
+      int64_t mine = 0;
+      cur.assign(1, vk);
+      dist[vk] = 0;
+      changed.push_back(vk);
+      for (int h = 0; h <= K; ++h) {
+        for (const auto &u : updates)
+          if (u.second < dist[u.first]) {
+            if (dist[u.first] == INF32) changed.push_back(u.first);
+            dist[u.first] = u.second;
+          }
+        updates.clear();
+        nxt.clear();
+        for (int32_t u : cur) {
+          const int32_t Pu = dist[u];
+          bool add;
+          {
+            std::lock_guard<std::mutex> lk(mtx[u]);
+            add = Pu < pair_min(L[u], T, h, 0);
+            if (add) {
+              L[u].push_back({vk, h, Pu, 0});
+              ++mine;
+            }
+          }
+          if (!add) continue;
+          for (int64_t e = g.off[u]; e < g.off[u + 1]; ++e) {
+            const int32_t x = g.col[e];
+            const int64_t nd = (int64_t)Pu + g.w[e];
+            if (nd < dist[x]) {
+              if (stamp[x] != h + 1) {
+                stamp[x] = h + 1;
+                nxt.push_back(x);
+              }
+              updates.push_back({x, (int32_t)nd});
+            }
+          }
+        }
+        std::swap(cur, nxt);
+        for (int32_t x : cur) stamp[x] = -1;
+      }
+      updates.clear();
+      for (int32_t x : changed) dist[x] = INF32;
+      changed.clear();
+      T.clear();
+      done += 1;
+      made += mine;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t) pool.emplace_back(worker);
+  for (auto &th : pool) th.join();
+  *hubs_done = done;
+  *labels = made;
+}
+
+// ---------------------------------------------------------------------------
 // Canonical repair, hop_constrained_clean_L:
 // CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:422-511.
 // Walk L(v) in (hub,hop,dist) order with kept := {}; keep (u,h,d) iff
@@ -1230,6 +1320,12 @@ void orc_dijkstra_timed(int32_t n, const int64_t *off, const int32_t *col, const
                         int K, int threads, double seconds, int64_t *hubs_done, int64_t *labels) {
   Graph g = make_graph(n, off, col, w);
   dijkstra_timed(g, K, threads, seconds, hubs_done, labels);
+}
+
+void orc_2023www_timed(int32_t n, const int64_t *off, const int32_t *col, const int32_t *w,
+                       int K, int threads, double seconds, int64_t *hubs_done, int64_t *labels) {
+  Graph g = make_graph(n, off, col, w);
+  www2023_timed(g, K, threads, seconds, hubs_done, labels);
 }
 
 void orc_replay_hubs(int32_t n, const int64_t *off, const int32_t *col, const int32_t *w, int K,

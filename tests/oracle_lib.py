@@ -43,6 +43,8 @@ def lib():
     L.orc_dijkstra_timed.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_double, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(ctypes.c_int64)]
+    L.orc_2023www_timed.restype = None
+    L.orc_2023www_timed.argtypes = L.orc_dijkstra_timed.argtypes
     L.orc_replay_hubs.restype = None
     L.orc_replay_hubs.argtypes = [ctypes.c_int32, _i64p, _i32p, _i32p, ctypes.c_int, _i64p, _u32p,
                                   _u8p, _i32p, ctypes.c_int64, _i32p, ctypes.c_int, _i32p]
@@ -324,4 +326,16 @@ def dijkstra_timed(g, K, threads=0, seconds=0.0):
     t0 = time.perf_counter()
     lib().orc_dijkstra_timed(g.n, g.off, g.col, g.w, K, threads, seconds, ctypes.byref(hubs),
                              ctypes.byref(labels))
+    return hubs.value, labels.value, time.perf_counter() - t0
+
+
+def www2023_timed(g, K, threads=0, seconds=0.0):
+    """The reference's threaded 2023WWW generation, stopped after `seconds`
+    (0: to the end): (hubs finished, labels made, wall seconds)."""
+    import time
+    hubs = ctypes.c_int64()
+    labels = ctypes.c_int64()
+    t0 = time.perf_counter()
+    lib().orc_2023www_timed(g.n, g.off, g.col, g.w, K, threads, seconds, ctypes.byref(hubs),
+                            ctypes.byref(labels))
     return hubs.value, labels.value, time.perf_counter() - t0

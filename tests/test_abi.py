@@ -25,6 +25,15 @@ def test_library_exports_every_declared_symbol():
     assert sorted(H.EXPORTED) == decl
 
 
+def test_library_is_built_from_the_tree_sources():
+    mk = open(os.path.join(ROOT, "gpu4hophl_code_amd", "csrc", "Makefile")).read()
+    lists = {k: re.search(rf"^{k} := (.*)$", mk, re.M).group(1).split()
+             for k in ("HIP_SRCS", "CPP_SRCS", "DEPS")}
+    assert H.HASHED_SOURCES == lists["HIP_SRCS"] + lists["CPP_SRCS"] + lists["DEPS"]
+    assert re.fullmatch(r"[0-9a-f]{64}", H.build_id())
+    assert H.build_id() == H.source_hash(), "libhsdl.so is older than its sources"
+
+
 def test_strerror_codes():
     L = H.lib()
     assert L.hsdl_strerror(0) == b"ok"

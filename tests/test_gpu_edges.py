@@ -386,3 +386,26 @@ def test_clean_of_a_vertex_with_more_than_65536_labels():
     idx.clean()
     assert idx.stats()["clean_passes"] == 2
     same_labels(idx, want.arrays())
+
+
+@pytest.mark.parametrize("hop_sort", ["0", "1"])
+def test_clean_hop_sorted_stream(monkeypatch, wide32, hop_sort):
+    """The clean's first pass streams each head's list from a 32-bit copy
+    sorted by hop, only up to the hops the group can use
+    (HSDL_CLEAN_HOP_SORT=1, the default), or whole in index order (0): on
+    redundant labels (vertex groups of 64 hubs) and on a grid at K=8 the
+    cleaned labels equal the oracle's."""
+    monkeypatch.setenv("HSDL_CLEAN_HOP_SORT", hop_sort)
+    g, c, K, kr, want = wide32
+    red = O.kround_grouped(c, K, 64)
+    off, hub, hop, dist = red.arrays()
+    idx = H.Index.from_labels(c.n, K, off.astype(np.uint64), hub, hop, dist)
+    idx.clean()
+    same_labels(idx, O.clean(red).arrays())
+    gg = H.Graph.grid(30, 41, 0x5EED0099, 1, 1000)
+    gc = csr_of(gg)
+    gred = O.kround_grouped(gc, 8, 100)
+    off, hub, hop, dist = gred.arrays()
+    idx = H.Index.from_labels(gc.n, 8, off.astype(np.uint64), hub, hop, dist)
+    idx.clean()
+    same_labels(idx, O.clean(gred).arrays())

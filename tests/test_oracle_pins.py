@@ -538,18 +538,23 @@ def test_label_digest_python_and_c_agree():
 
 
 def test_threaded_dijkstra_timer_counts():
-    """The time-boxed, threaded CPU_HopHL restatement (the CPU baseline's
-    timer) finishes every hub without a limit, and on one thread makes
-    exactly the sequential Dijkstra generator's labels; with a limit it
-    stops early with fewer hubs."""
+    """The time-boxed, threaded CPU_HopHL and 2023WWW restatements (the CPU
+    baselines' timers) finish every hub without a limit, and on one thread
+    make exactly the sequential generators' labels; with a limit they stop
+    early with fewer hubs."""
     for g, K in tiny_graphs(61, 30, nmax=40):
         hubs, labels, _ = O.dijkstra_timed(g, K, threads=1)
         assert hubs == g.n and labels == O.dijkstra(g, K).total()
         hubs, labels, _ = O.dijkstra_timed(g, K, threads=4)
         assert hubs == g.n and labels >= O.canonical(g, K).total()
+    for g, K in tiny_graphs(62, 30, nmax=40):
+        hubs, labels, _ = O.www2023_timed(g, K, threads=1)
+        assert hubs == g.n and labels == O.www2023(g, K).total()
     import gpu4hophl_code_amd as H
     r = H.Graph.rmat(14, 16, 3, 1, 100)
     off, col, w = r.csr()
     c = O.CSR(len(off) - 1, off.astype(np.int64), col.astype(np.int32), w)
     hubs, labels, sec = O.dijkstra_timed(c, 5, threads=2, seconds=0.05)
+    assert 0 < hubs < c.n and labels > 0
+    hubs, labels, sec = O.www2023_timed(c, 5, threads=2, seconds=0.05)
     assert 0 < hubs < c.n and labels > 0
