@@ -22,17 +22,20 @@
 //
 // Per round, all on one stream, no host synchronisation:
 //   round_prep  -> zero per-hub counters
-//   expand      -> wave-cooperative CSR scan of the frontier: each wave takes
-//                  64 frontier items, binary-searches each item's id-sorted
-//                  adjacency for the first neighbour above the hub (rank
-//                  pruning without reading the pruned edges), prefix-sums the
-//                  remaining degrees in registers and walks the concatenated
-//                  edges 64 at a time (coalesced 8 B (col,w) loads);
+//   expand      -> the frontier is grouped by vertex (the labels vertex x
+//                  gained in the last round, of many hubs); a wave takes 64
+//                  groups, binary-searches each x's id-sorted adjacency for
+//                  the first neighbour above the batch's first hub and walks
+//                  the (edge, label) pairs edge-major, 64 at a time, so the
+//                  lanes of one edge reach the neighbour's slots of nearby
+//                  hub slots (slot windows, slot_at) in a few sectors;
 //                  candidates are deduplicated by atomicMin on a dense
 //                  per-(hub,vertex) slot that keeps the batch's earlier
 //                  rounds' minima (tagged per batch attempt); the first
 //                  toucher that undercuts them appends (hub,vertex) to a flat
-//                  list and the per-hub counts are tallied per wave
+//                  list
+//   count       -> per-hub candidate counts of the flat list (LDS hash per
+//                  2048-entry chunk: one global atomic per chunk and hub)
 //   chunk_scan  -> per-hub offsets and work items of `tile` candidates
 //   group       -> flat candidates scattered into per-hub runs (hubs tallied
 //                  per 2048-entry chunk in an LDS hash: one global
@@ -44,8 +47,9 @@
 //                  with the next step in flight and stop at the first witness
 //   reserve     -> grows label segments that the round's accepted labels
 //                  overflow (bump allocation in a device pool; no locks)
-//   append      -> writes accepted labels; they also form the next frontier
-//   finalize    -> publishes segment lengths
+//   append      -> writes accepted labels
+//   finalize    -> publishes segment lengths; each vertex's new labels
+//                  become its frontier group of the next round
 // Capacities (label pool, tile arena, candidate and frontier lists) start
 // small; a batch that overflows one raises a flag, the later kernels of the
 // batch stand down, and the host rolls the batch back, grows the buffer and
@@ -63,10 +67,19 @@
 
 namespace hsdl {
 
-struct Item {  // frontier entry / accepted label: hub slot, vertex, distance
+struct Item {  // accepted label of a round: hub slot, vertex, distance
   uint32_t b;
   uint32_t y;
   uint32_t d;
+};
+
+// The next round's frontier, grouped by vertex: the labels vertex y gained
+// in a round are the cnt labels of its list at pool offset off (appended
+// together, so contiguous), and they are expanded together over y's edges.
+struct FGroup {
+  uint32_t y;
+  uint32_t cnt;
+  unsigned long long off;
 };
 
 enum : uint32_t {
@@ -90,21 +103,22 @@ struct Ctrl {
   uint32_t total_chunks;
   uint32_t overflow;
   uint32_t maxlen;
-  uint32_t expand_ctr;  // next 64-item chunk of the frontier (k_expand)
+  uint32_t expand_ctr;  // next chunk of 64 frontier groups (k_expand)
+  uint32_t gcnt;        // frontier groups of the running round
+  uint32_t cand_round;  // candidates the running round's expansion opened
   uint32_t heavy_cnt;   // pieces pushed to the heavy list this round
   uint32_t heavy_ctr;   // next heavy piece (k_expand_heavy)
   uint32_t row_mode;    // this round's prune takes row items (k_prune_rows)
   uint32_t row_ctr[8];  // next row item of each XCD's queue
 };
 
-// A piece of a high-degree frontier item: up to kHeavyPiece consecutive
-// edges, relaxed by a whole workgroup.
+// A piece of a large frontier group: its cnt labels (at pool offset off)
+// over ecnt consecutive edges from beg, relaxed by a whole workgroup.
 struct HeavyItem {
-  uint32_t b;
-  uint32_t d;
+  unsigned long long off;
   unsigned long long beg;
   uint32_t cnt;
-  uint32_t pad;
+  uint32_t ecnt;
 };
 
 struct GenArgs {
@@ -119,6 +133,8 @@ struct GenArgs {
   uint32_t row_density;  // "dense": candidates >= batch slots / row_density
   uint32_t row_run;      // y-blocks per row item (one tile build per item)
   uint32_t row_block;    // vertices per y-block (a multiple of 1024)
+  uint32_t heavy_pairs;  // frontier groups with more pairs go to heavy pieces
+  uint32_t piece_pairs;  // pairs per heavy piece (whole edges)
   uint32_t lds_slots;  // LDS tile capacity (slots)
   // hub ids of a group build, ascending (hub slot b of the batch is
   // hubs[s0 + b]); nullptr: every vertex is a hub (slot b is hub s0 + b)
@@ -145,7 +161,12 @@ struct GenArgs {
   uint32_t *add_cnt;
   uint32_t *add_cur;
   uint32_t *touched;
-  uint32_t *cand_d;           // dense slots [Bmax][n], tagged (slot_fresh / slot_stale)
+  FGroup *fgrp;               // frontier groups (n entries)
+  uint32_t *hub_slot;         // group builds: hub slot of each hub of the batch
+  // dense slots, tagged (slot_fresh / slot_stale), in hub windows of
+  // 2^slot_lw slots: [Bmax >> slot_lw][n][2^slot_lw] (slot_at)
+  uint32_t *cand_d;
+  uint32_t slot_lw;
   uint32_t tag;               // epoch of this batch attempt (high bits of a slot)
   uint32_t slot_shift;        // tag << slot_shift | dist << 1 | stale
   unsigned long long *cflat;  // (b << 32 | y), ccap entries
@@ -166,6 +187,15 @@ struct GenArgs {
 };
 
 __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
+
+// Dense slot of (hub slot b, vertex y): the slots of a window of 2^slot_lw
+// consecutive hub slots lie side by side for every vertex, so the expansion
+// of a frontier group (one vertex's new labels of many hubs) reaches a
+// neighbour's slots in a few memory sectors instead of one sector per label.
+__device__ inline uint64_t slot_at(const GenArgs &a, uint32_t b, uint32_t y) {
+  return ((((uint64_t)(b >> a.slot_lw)) * a.n + y) << a.slot_lw) |
+         (b & ((1u << a.slot_lw) - 1u));
+}
 
 // vertex id of hub slot b of the running batch
 __device__ inline uint32_t hub_of(const GenArgs &a, uint32_t b) {
@@ -284,11 +314,18 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t val) {
     p[i] = val;
 }
 
+// The first frontier: every hub of the batch with its self label (position
+// 0 of its list), one group each.
 __global__ void k_batch_begin(GenArgs a) {
-  Item *F = frontier(a, a.cur);
-  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x)
-    F[b] = Item{b, hub_of(a, b), 0u};
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->fcnt[a.cur] = a.B;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x) {
+    const uint32_t s = hub_of(a, b);
+    a.fgrp[b] = FGroup{s, 1u, a.seg_start[s]};
+    if (a.hubs) a.hub_slot[s] = b;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.ctrl->fcnt[a.cur] = a.B;
+    a.ctrl->gcnt = a.B;
+  }
 }
 
 __global__ void k_round_prep(GenArgs a) {
@@ -304,10 +341,15 @@ __global__ void k_round_prep(GenArgs a) {
     a.ctrl->expand_ctr = 0;
     a.ctrl->heavy_cnt = 0;
     a.ctrl->heavy_ctr = 0;
+    a.ctrl->cand_round = 0;
   }
 }
 
 // ---------------------------------------------------------------- expand
+// XCDs of an MI355X (each has its own L2); workgroups are dealt to them
+// round-robin by id.
+constexpr uint32_t kXcds = 8;
+
 // first index in adj[beg, end) whose neighbour id exceeds s
 __device__ inline unsigned long long first_above(const uint2 *adj, unsigned long long beg,
                                                  unsigned long long end, uint32_t s) {
@@ -318,22 +360,20 @@ __device__ inline unsigned long long first_above(const uint2 *adj, unsigned long
   return beg;
 }
 
-// Frontier items with more than kHeavyDeg edges above the hub are cut into
-// pieces of kHeavyPiece edges for k_expand_heavy, so no wave is left alone
-// with a hub-sized adjacency list while the rest of the GPU idles.
-constexpr uint32_t kHeavyDeg = 256;
-constexpr uint32_t kHeavyPiece = 1024;
+// Frontier groups whose (label, edge) pairs above the batch's first hub
+// exceed a.heavy_pairs are cut into pieces of about a.piece_pairs pairs
+// (whole edges) for k_expand_heavy, so no wave is left alone with a large
+// group while the rest of the GPU idles.
 
 // Flat-list space is reserved per wave in blocks of kListBlock entries (one
 // atomic on the list counter per block instead of one per step); unused
-// tails are filled with empty entries (hub slot kInf32) that the grouping
-// skips.
+// tails are filled with empty entries that the counting and grouping skip.
 constexpr uint32_t kListBlock = 512;
 constexpr unsigned long long kEmptyEntry = ~0ull;
 
 struct ListRes {
   uint32_t base = 0, used = kListBlock;  // starts exhausted
-  uint32_t hub = kInf32, hub_cnt = 0;    // per-hub count not yet added to hub_ccnt
+  uint32_t cands = 0;                    // candidates opened, not yet added to cand_round
 };
 
 // Writes empty entries over [base+used, base+kListBlock) (whole wave).
@@ -342,33 +382,58 @@ __device__ inline void list_pad(const GenArgs &a, const ListRes &r) {
     if (r.base + i < a.ccap) a.cflat[r.base + i] = kEmptyEntry;
 }
 
-// End of a kernel: pad the open block and add the pending hub count.
+// End of a kernel: pad the open block and add the wave's candidate count.
 __device__ inline void list_close(const GenArgs &a, const ListRes &r) {
   list_pad(a, r);
-  if (lane_id() == 0 && r.hub_cnt) atomicAdd(&a.hub_ccnt[r.hub], r.hub_cnt);
+  if (lane_id() == 0 && r.cands) atomicAdd(&a.ctrl->cand_round, r.cands);
 }
 
-// Relaxes edge e of hub slot kb at distance kd (when valid): atomicMin on the
-// tagged slot; first touches that undercut earlier rounds go to the flat
-// list (the wave's reserved block) and are counted per hub.  Called by all
-// lanes of a wave.
-__device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uint32_t kd,
-                                  unsigned long long e, unsigned long long lt_mask, ListRes &res) {
+// A frontier label at pool offset `at`: its hub slot, hub and distance.
+// `ok` is false for a hub outside the running batch, which a list never
+// holds at these offsets; the check keeps every slot access in bounds.
+__device__ inline void frontier_label(const GenArgs &a, unsigned long long at, uint32_t &b,
+                                      uint32_t &hub, uint32_t &d, bool &ok) {
+  uint32_t hop;
+  if (a.pool32)
+    unpack(a, a.pool32[at], hub, hop, d);
+  else
+    unpack(a, a.pool[at], hub, hop, d);
+  if (a.hubs) {
+    b = hub < a.n ? a.hub_slot[hub] : kInf32;
+    ok = b < a.B && a.hubs[a.s0 + b] == hub;
+  } else {
+    b = hub - a.s0;
+    ok = hub >= a.s0 && b < a.B;
+  }
+}
+
+// Relaxes edge ed = (vertex, weight) for hub slot kb (hub kh) at distance
+// kd, when valid and the edge's vertex ranks below the hub: atomicMin on the
+// tagged slot; first touches that undercut earlier rounds go to the flat list
+// (the wave's reserved block).  Called by all lanes of a wave; returns
+// whether this lane relaxed an edge.  (The callers load the edge and the
+// frontier label independently, so the two loads are in flight together.)
+__device__ inline bool relax_edge(const GenArgs &a, bool valid, uint32_t kb, uint32_t kh,
+                                  uint32_t kd, uint2 ed, unsigned long long lt_mask,
+                                  ListRes &res) {
   const int lane = lane_id();
-  bool first = false;
+  bool first = false, relaxed = false;
   uint32_t y = 0;
   if (valid) {
-    const uint2 ed = a.adj[e];
     y = ed.x;
-    const uint32_t nv = slot_fresh(a, kd + ed.y);
-    uint32_t *slot = &a.cand_d[(uint64_t)kb * a.n + y];
-    // A slot only falls while the expansion runs (atomicMin; clears happen
-    // between batches), so a read at or below nv, even a stale one, means the
-    // atomic would neither change the slot nor open it: most relaxations end
-    // with this read, which costs less than a returning atomic.
-    if (nv < *slot) {
-      const uint32_t old = atomicMin(slot, nv);
-      first = nv < old && slot_opens(a, old);
+    if (y > kh) {  // rank pruning: hub kh labels only vertices ranked below it
+      relaxed = true;
+      const uint32_t nv = slot_fresh(a, kd + ed.y);
+      uint32_t *slot = &a.cand_d[slot_at(a, kb, y)];
+      // A slot only falls while the expansion runs (atomicMin; clears happen
+      // between batches), so a read at or below nv, even a stale one, means
+      // the atomic would neither change the slot nor open it: most
+      // relaxations end with this read, which costs less than a returning
+      // atomic.
+      if (nv < *slot) {
+        const uint32_t old = atomicMin(slot, nv);
+        first = nv < old && slot_opens(a, old);
+      }
     }
   }
   const unsigned long long firsts = __ballot(first);
@@ -389,27 +454,23 @@ __device__ inline void relax_edge(const GenArgs &a, bool valid, uint32_t kb, uin
         atomicOr(&a.ctrl->overflow, kOvfCand);
     }
     res.used += cnt;
-    // per-hub counts: a wave's steps mostly share one hub, so the count for
-    // the current hub stays in a register until the hub changes
-    unsigned long long pend = firsts;
-    while (pend) {
-      const int leader = __ffsll((long long)pend) - 1;
-      const uint32_t bl = __shfl(kb, leader, kWave);
-      const unsigned long long grp = __ballot(first && kb == bl);
-      if (bl != res.hub) {
-        if (lane == 0 && res.hub_cnt) atomicAdd(&a.hub_ccnt[res.hub], res.hub_cnt);
-        res.hub = bl;
-        res.hub_cnt = 0;
-      }
-      res.hub_cnt += (uint32_t)__popcll(grp);
-      pend &= ~grp;
-    }
+    res.cands += cnt;
   }
+  return relaxed;
 }
 
+// Round h's expansion.  A wave takes 64 frontier groups (vertex x and the
+// labels x gained in round h-1, of many hubs) and walks the pairs (edge of x
+// above the batch's first hub, label of the group) edge-major: consecutive
+// lanes take the labels of one group on one edge, so they share the edge's
+// load and reach the neighbour's slots of nearby hub slots, side by side in
+// the slot windows.  Lanes whose label's hub ranks above the neighbour do no
+// work (rank pruning).
 __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
-  const Item *F = frontier(a, a.cur);
-  const uint32_t fcnt = min(a.ctrl->fcnt[a.cur], a.fcap);
+  if (a.ctrl->overflow) return;  // the batch is rolled back and run again
+  const FGroup *G = a.fgrp;
+  const uint32_t gcnt = min(a.ctrl->gcnt, a.n);
+  const uint32_t lo_hub = hub_of(a, 0);
   const int lane = lane_id();
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   uint64_t edges_seen = 0;
@@ -417,61 +478,101 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   const uint32_t wave_id = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
   for (bool first = true;; first = false) {
-    // 64-item chunks of the frontier: a wave's first chunk is its own id,
-    // later ones come from a counter (waves with nothing to do never touch it)
+    // chunks of 64 groups: a wave's first chunk is its own id and later ones
+    // come from a counter (waves with nothing to do never touch it; dealing
+    // the chunks to XCDs in contiguous ranges instead cost every wave its
+    // claims even in small rounds: the 1024x1024 grid's expansion took
+    // 0.258 s that way against 0.042 s)
     uint32_t chunk = wave_id;
     if (!first) {
       if (lane == 0) chunk = nwaves + atomicAdd(&a.ctrl->expand_ctr, 1u);
       chunk = __shfl(chunk, 0, kWave);
     }
     const uint64_t base = (uint64_t)chunk * kWave;
-    if (base >= fcnt) break;
+    if (base >= gcnt) break;
     const uint64_t i = base + lane;
-    uint32_t ib = 0, id = 0;
-    unsigned long long beg = 0;
-    uint32_t deg = 0;
-    if (i < fcnt) {
-      const Item it = F[i];
-      ib = it.b;
-      id = it.d;
-      const unsigned long long e0 = a.off[it.y], e1 = a.off[it.y + 1];
-      beg = first_above(a.adj, e0, e1, hub_of(a, it.b));  // rank pruning y > s
-      deg = (uint32_t)(e1 - beg);
-      if (deg > kHeavyDeg) {
-        const uint32_t pieces = (deg + kHeavyPiece - 1) / kHeavyPiece;
-        const uint32_t pos = atomicAdd(&a.ctrl->heavy_cnt, pieces);
-        if (pos + pieces <= a.heavy_cap) {
-          for (uint32_t j = 0; j < pieces; ++j)
-            a.heavy[pos + j] = HeavyItem{ib, id, beg + (unsigned long long)j * kHeavyPiece,
-                                         min(kHeavyPiece, deg - j * kHeavyPiece), 0u};
-          edges_seen += deg;
-          deg = 0;
+    uint32_t gc = 0, pairs = 0;
+    unsigned long long goff = 0, beg = 0, deg = 0;
+    bool kept = false;  // a large group this wave expands itself (heavy list full)
+    if (i < gcnt) {
+      const FGroup gr = G[i];
+      gc = gr.cnt;
+      goff = gr.off;
+      const unsigned long long e0 = a.off[gr.y], e1 = a.off[gr.y + 1];
+      beg = first_above(a.adj, e0, e1, lo_hub);
+      deg = gc ? e1 - beg : 0;
+      const unsigned long long pr = (unsigned long long)gc * deg;
+      if (pr > a.heavy_pairs) {
+        const uint32_t ep = max(1u, a.piece_pairs / gc);  // edges per piece
+        const unsigned long long np = (deg + ep - 1) / ep;
+        uint32_t pos = kInf32;
+        if (np < a.heavy_cap) pos = atomicAdd(&a.ctrl->heavy_cnt, (uint32_t)np);
+        if (pos != kInf32 && pos + np <= a.heavy_cap) {
+          for (uint32_t j = 0; j < (uint32_t)np; ++j) {
+            const unsigned long long eb = (unsigned long long)j * ep;
+            a.heavy[pos + j] =
+                HeavyItem{goff, beg + eb, gc, (uint32_t)min((unsigned long long)ep, deg - eb)};
+          }
         } else {
-          // no room: this wave keeps the item; reserved slots below the cap
-          // are written as empty pieces
-          for (uint32_t j = 0; j < pieces && pos + j < a.heavy_cap; ++j)
-            a.heavy[pos + j] = HeavyItem{0u, 0u, 0ull, 0u, 0u};
+          // no room: this wave keeps the group; reserved slots below the
+          // cap are written as empty pieces
+          if (pos != kInf32)
+            for (uint32_t j = 0; pos + j < a.heavy_cap && j < (uint32_t)np; ++j)
+              a.heavy[pos + j] = HeavyItem{0ull, 0ull, 0u, 0u};
+          kept = true;
         }
+      } else {
+        pairs = (uint32_t)pr;
       }
     }
-    const uint32_t incl = wave_incl_scan_u32(deg);
-    const uint32_t excl = incl - deg;
+    const uint32_t incl = wave_incl_scan_u32(pairs);
+    const uint32_t excl = incl - pairs;
     const uint32_t total = __shfl(incl, kWave - 1, kWave);
-    if (lane == 0) edges_seen += total;
     for (uint32_t jb = 0; jb < total; jb += kWave) {
       const uint32_t j = jb + lane;
-      // largest k with excl_k <= j (items with no edges are skipped over)
+      // largest k with excl_k <= j (groups without pairs are skipped over)
       int k = 0;
 #pragma unroll
       for (int step = 32; step > 0; step >>= 1) {
         const uint32_t e = __shfl(excl, k + step, kWave);
         if (e <= j) k += step;
       }
-      const uint32_t kb = __shfl(ib, k, kWave);
-      const uint32_t kd = __shfl(id, k, kWave);
+      const uint32_t kc = max(1u, (uint32_t)__shfl(gc, k, kWave));
       const uint32_t kex = __shfl(excl, k, kWave);
       const unsigned long long kbeg = __shfl(beg, k, kWave);
-      relax_edge(a, j < total, kb, kd, kbeg + (j - kex), lt_mask, res);
+      const unsigned long long koff = __shfl(goff, k, kWave);
+      const uint32_t p = j - kex;
+      const uint32_t q = p / kc, r = p - q * kc;  // edge q, label r of group k
+      uint32_t b = 0, hub = 0, d = 0;
+      bool ok = false;
+      uint2 ed = make_uint2(0u, 0u);
+      if (j < total) {
+        ed = a.adj[kbeg + q];
+        frontier_label(a, koff + r, b, hub, d, ok);
+      }
+      edges_seen += relax_edge(a, j < total && ok, b, hub, d, ed, lt_mask, res);
+    }
+    // large groups the heavy list had no room for, one at a time
+    unsigned long long keep = __ballot(kept);
+    while (keep) {
+      const int l = __ffsll((long long)keep) - 1;
+      keep &= keep - 1;
+      const uint32_t kc = __shfl(gc, l, kWave);
+      const unsigned long long kbeg = __shfl(beg, l, kWave);
+      const unsigned long long koff = __shfl(goff, l, kWave);
+      const unsigned long long kpairs = (unsigned long long)kc * __shfl(deg, l, kWave);
+      for (unsigned long long pb = 0; pb < kpairs; pb += kWave) {
+        const unsigned long long p = pb + lane;
+        const unsigned long long q = p / kc;
+        uint32_t b = 0, hub = 0, d = 0;
+        bool ok = false;
+        uint2 ed = make_uint2(0u, 0u);
+        if (p < kpairs) {
+          ed = a.adj[kbeg + q];
+          frontier_label(a, koff + (p - q * kc), b, hub, d, ok);
+        }
+        edges_seen += relax_edge(a, p < kpairs && ok, b, hub, d, ed, lt_mask, res);
+      }
     }
   }
   list_close(a, res);
@@ -479,31 +580,48 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
   if (lane == 0 && edges_seen) atomicAdd(&a.ctrl->edge_total, (unsigned long long)edges_seen);
 }
 
-// Heavy pieces: a workgroup takes one piece at a time and spreads its edges
-// over its 256 lanes.
+// Heavy pieces: a workgroup takes one piece at a time and spreads its pairs
+// over its 256 lanes, edge-major like k_expand.
 __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
   __shared__ uint32_t sh_h;
+  if (a.ctrl->overflow) return;
   const int lane = lane_id();
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint32_t wv = threadIdx.x / kWave;
   const uint32_t cnt = min(a.ctrl->heavy_cnt, a.heavy_cap);
   if (blockIdx.x >= cnt) return;  // uniform: most launches find no pieces
   ListRes res;
+  uint64_t edges_seen = 0;
   for (bool first = true;; first = false) {
-    // a workgroup's first piece is its own id, later ones come from a counter
-    if (threadIdx.x == 0) sh_h = first ? blockIdx.x : gridDim.x + atomicAdd(&a.ctrl->heavy_ctr, 1u);
+    // a workgroup's first piece is its own id and later ones come from a
+    // counter
+    if (threadIdx.x == 0)
+      sh_h = first ? blockIdx.x : gridDim.x + atomicAdd(&a.ctrl->heavy_ctr, 1u);
     __syncthreads();
     const uint32_t h = sh_h;
     __syncthreads();
     if (h >= cnt) break;
     const HeavyItem it = a.heavy[h];
-    // whole waves iterate together so the ballots see every lane
-    for (uint32_t eb = wv * kWave; eb < it.cnt; eb += 256) {
-      const uint32_t e = eb + (uint32_t)lane;
-      relax_edge(a, e < it.cnt, it.b, it.d, it.beg + e, lt_mask, res);
+    // pairs <= max(piece_pairs, cnt) < 2^32; whole waves iterate together so
+    // the ballots see every lane
+    const uint32_t kc = max(1u, it.cnt);
+    const uint32_t pairs = it.cnt * it.ecnt;
+    for (uint32_t pb = wv * kWave; pb < pairs; pb += 256) {
+      const uint32_t p = pb + (uint32_t)lane;
+      const uint32_t q = p / kc;
+      uint32_t b = 0, hub = 0, d = 0;
+      bool ok = false;
+      uint2 ed = make_uint2(0u, 0u);
+      if (p < pairs) {
+        ed = a.adj[it.beg + q];
+        frontier_label(a, it.off + (p - q * kc), b, hub, d, ok);
+      }
+      edges_seen += relax_edge(a, p < pairs && ok, b, hub, d, ed, lt_mask, res);
     }
   }
   list_close(a, res);
+  edges_seen = wave_sum_u64(edges_seen);
+  if (lane == 0 && edges_seen) atomicAdd(&a.ctrl->edge_total, (unsigned long long)edges_seen);
 }
 
 // ------------------------------------------------------------ chunk scan
@@ -511,9 +629,68 @@ __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
 // least 1/row_density of the batch's dense slots (default 32: the row scan
 // then costs at most 128 B per candidate, against about 1.5 KB of list reads).
 // Row items: XCDs of an MI355X (each has its own L2).
-constexpr uint32_t kXcds = 8;
 // (vertices per row block: a.row_block, a multiple of the 1024 a workgroup
 // scans at a time)
+
+// Row items when the round's candidates fill at least 1/row_density of the
+// batch's slot rows (or always / never by the policy), and only while an XCD
+// queue's item count fits its 32-bit counter.  A batch of one hub has no list
+// reads to share: the plain form never takes row items unless forced.
+__device__ inline bool rows_chosen(const GenArgs &a) {
+  const uint32_t cands = a.ctrl->cand_round;
+  const uint64_t queue_items =
+      (uint64_t)a.B * (((uint64_t)a.n + (uint64_t)a.row_block * kXcds - 1) /
+                       ((uint64_t)a.row_block * kXcds));
+  return cands && queue_items < (1ull << 31) &&
+         (a.row_policy == 2 || (a.row_policy == 1 && a.B > 1 &&
+                                (uint64_t)cands * a.row_density >= (uint64_t)a.B * a.n));
+}
+
+// Per-hub candidate counts of a round that takes list items: a workgroup
+// takes 2048 consecutive entries of the flat list, counts their hubs in an
+// LDS hash and adds each hub's count with one global atomic.
+constexpr uint32_t kGroupThreads = 256;
+constexpr uint32_t kGroupPer = 8;                                 // entries per thread
+constexpr uint32_t kGroupChunk = kGroupThreads * kGroupPer;       // 2048
+constexpr uint32_t kGroupSlots = 2 * kGroupChunk;                 // load factor <= 1/2
+static_assert(kGroupSlots == 4096, "hubs hash to 12 bits");
+
+// LDS hash slot of hub slot b (inserted when absent)
+__device__ inline uint32_t group_slot(uint32_t *hkey, uint32_t b) {
+  uint32_t sl = (b * 0x9E3779B1u) >> 20;  // 12 bits: kGroupSlots = 4096
+  for (;;) {
+    const uint32_t prev = atomicCAS(&hkey[sl], kInf32, b);
+    if (prev == kInf32 || prev == b) return sl;
+    sl = (sl + 1) & (kGroupSlots - 1);
+  }
+}
+
+__global__ void __launch_bounds__(kGroupThreads) k_count(GenArgs a) {
+  __shared__ uint32_t hkey[kGroupSlots];
+  __shared__ uint32_t hcnt[kGroupSlots];
+  if (a.ctrl->overflow || rows_chosen(a)) return;
+  // reserved blocks may reach past the list's end; only empty entries lie there
+  const uint32_t cnt = min(a.ctrl->ccnt, a.ccap);
+  const uint32_t tid = threadIdx.x;
+  for (uint64_t c0 = (uint64_t)blockIdx.x * kGroupChunk; c0 < cnt;
+       c0 += (uint64_t)gridDim.x * kGroupChunk) {
+    for (uint32_t i = tid; i < kGroupSlots; i += kGroupThreads) {
+      hkey[i] = kInf32;
+      hcnt[i] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < kGroupPer; ++r) {
+      const uint64_t i = c0 + r * kGroupThreads + tid;
+      const unsigned long long e = i < cnt ? a.cflat[i] : kEmptyEntry;
+      if (e != kEmptyEntry) atomicAdd(&hcnt[group_slot(hkey, (uint32_t)(e >> 32))], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < kGroupSlots; i += kGroupThreads)
+      if (hcnt[i]) atomicAdd(&a.hub_ccnt[hkey[i]], hcnt[i]);
+    __syncthreads();
+  }
+}
 
 // One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
 // sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
@@ -549,24 +726,13 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
   if (t < 8) a.ctrl->row_ctr[t] = 0;
   if (t == blockDim.x - 1) {
     const bool dead = a.ctrl->overflow != 0;
-    // row items when the round's candidates fill at least 1/row_density of
-    // the batch's slot rows (or always / never by the policy)
-    // (and only while an XCD queue's item count fits its 32-bit counter)
-    const uint64_t queue_items =
-        (uint64_t)a.B * (((uint64_t)a.n + (uint64_t)a.row_block * kXcds - 1) /
-                         ((uint64_t)a.row_block * kXcds));
-    // (a batch of one hub has no list reads to share: the plain form never
-    // takes row items unless forced)
-    const bool rows = !dead && partc[t] && queue_items < (1ull << 31) &&
-                      (a.row_policy == 2 ||
-                       (a.row_policy == 1 && a.B > 1 &&
-                        (uint64_t)partc[t] * a.row_density >= (uint64_t)a.B * a.n));
+    const bool rows = !dead && rows_chosen(a);
     a.chunk_off[a.B] = part[t];
     a.hub_off[a.B] = partc[t];
     a.ctrl->row_mode = rows ? 1u : 0u;
     a.ctrl->total_chunks = (dead || rows) ? 0u : part[t];
     a.ctrl->work_ctr = 0;
-    a.ctrl->cand_total += partc[t];
+    a.ctrl->cand_total += a.ctrl->cand_round;
     a.ctrl->item_total += min(a.ctrl->fcnt[a.cur], a.fcap);
   }
 }
@@ -576,11 +742,6 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
 // space with one global atomic and writes the entries: one global atomic per
 // (chunk, hub) instead of one per (wave, hub), so hubs with millions of
 // candidates do not serialize the scatter on their counters.
-constexpr uint32_t kGroupThreads = 256;
-constexpr uint32_t kGroupPer = 8;                                 // entries per thread
-constexpr uint32_t kGroupChunk = kGroupThreads * kGroupPer;       // 2048
-constexpr uint32_t kGroupSlots = 2 * kGroupChunk;                 // load factor <= 1/2
-static_assert(kGroupSlots == 4096, "k_group hashes hubs to 12 bits");
 
 __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
   __shared__ uint32_t hkey[kGroupSlots];
@@ -605,14 +766,8 @@ __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
       yv[r] = 0;
       const unsigned long long e = i < cnt ? a.cflat[i] : kEmptyEntry;
       if (e != kEmptyEntry) {
-        const uint32_t b = (uint32_t)(e >> 32);
         yv[r] = (uint32_t)e;
-        uint32_t sl = (b * 0x9E3779B1u) >> 20;  // 12 bits: kGroupSlots = 4096
-        for (;;) {
-          const uint32_t prev = atomicCAS(&hkey[sl], kInf32, b);
-          if (prev == kInf32 || prev == b) break;
-          sl = (sl + 1) & (kGroupSlots - 1);
-        }
+        const uint32_t sl = group_slot(hkey, (uint32_t)(e >> 32));
         slot[r] = sl;
         atomicAdd(&hcnt[sl], 1u);
       }
@@ -856,7 +1011,7 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
     unsigned long long st = 0;
     if (lane < cn) {
       y = cands[c + lane];
-      dy = slot_dist(a, a.cand_d[(uint64_t)b * a.n + y]);
+      dy = slot_dist(a, a.cand_d[slot_at(a, b, y)]);
       st = a.seg_start[y];
       len = a.seg_len[y];
       if (a.hcnt && a.h > 1) {
@@ -885,7 +1040,7 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
                                             reads);
       wave_reads += reads;
       if (lane == 0) {
-        a.cand_d[(uint64_t)b * a.n + yi] = slot_stale(a, di);  // later rounds must undercut di
+        a.cand_d[slot_at(a, b, yi)] = slot_stale(a, di);  // later rounds must undercut di
         if (!pruned) {
           const uint32_t p = atomicAdd(&sh_acc_n, 1u);
           sh_acc[p] = Item{b, yi, di};
@@ -1088,7 +1243,7 @@ k_prune_rows(GenArgs a) {
       if (threadIdx.x == 0) sh_cnt = 0;
       __syncthreads();
       const uint32_t y = y0 + threadIdx.x;
-      const uint32_t v = y < y_end ? a.cand_d[(uint64_t)b * a.n + y] : kInf32;
+      const uint32_t v = y < y_end ? a.cand_d[slot_at(a, b, y)] : kInf32;
       const bool cand = y < y_end && (v >> a.slot_shift) == a.tag && !(v & 1u);
       const unsigned long long m = __ballot(cand);
       uint32_t base = 0;
@@ -1197,14 +1352,18 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
     const uint32_t y = a.touched[t];
     if (keep) {
       // capped: when the pool ran out some appends were dropped
-      const uint32_t nl = min(a.seg_len[y] + a.add_cnt[y], a.seg_cap[y]);
+      const uint32_t len = a.seg_len[y];
+      const uint32_t nl = min(len + a.add_cnt[y], a.seg_cap[y]);
       a.seg_len[y] = nl;
       mx = max(mx, nl);
+      // the next round's frontier group of y: the labels it just gained
+      a.fgrp[t] = FGroup{y, nl - len, a.seg_start[y] + len};
     }
     a.add_cnt[y] = 0;
     a.add_cur[y] = 0;
   }
   if (mx) atomicMax(&a.ctrl->maxlen, mx);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->gcnt = keep ? min(cnt, a.n) : 0u;
 }
 
 // Removes the labels of the batch's hubs (the batch being rolled back): they
@@ -1392,7 +1551,7 @@ static int upload_graph(hsdl_graph *g, int dev) {
 
 struct Gen {
   GenArgs a{};
-  DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched;
+  DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched, fgrp, hub_slot;
   DevBuf sorted_len, hcnt, sort_tmp, sort_tmp32;  // hop zones
   uint32_t sort_per_wave = 0;                     // labels per wave slice of sort_tmp
   int sort_grid = 0;
@@ -1519,6 +1678,7 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t n
     hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_expand_heavy, dim3(g.grid), dim3(256), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 1], g.st));
+    hipLaunchKernelGGL(k_count, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
     if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
@@ -1586,13 +1746,16 @@ static int sort_zones(Gen &g, uint32_t maxlen, uint32_t resort_div) {
   return HSDL_OK;
 }
 
-// Chooses the next hub batch from the last one's duration (the autotuner of
-// the batched form; the plain form pins B = 1).
+// Chooses the next hub batch from the last one's measure (the autotuner of
+// the batched form; the plain form pins B = 1): its work -- candidates plus
+// an eighth of the scanned edges, both fixed by the graph and the schedule,
+// so the schedule repeats from run to run -- or, with HSDL_TUNER=time, its
+// wall-clock duration.
 struct BatchTuner {
   uint32_t B, Bmax;
-  double target_s;
-  uint32_t next(double last_s) {
-    double f = last_s > 0 ? target_s / last_s : 2.0;
+  double target;
+  uint32_t next(double last) {
+    double f = last > 0 ? target / last : 2.0;
     f = std::min(2.0, std::max(0.5, f));
     uint64_t nb = (uint64_t)std::llround(B * f);
     nb = std::max<uint64_t>(1, std::min<uint64_t>(nb, Bmax));
@@ -1661,7 +1824,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       "HSDL_LIST_INIT",
       (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30), 64,
       1u << 30);
-  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5 + 4 * (uint64_t)K);
+  // per vertex: segment start, length, capacity, add counters, touched
+  // entry, frontier group (16 B), hub slot, hop zone counts
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5 + 16 + 4 + 4 * (uint64_t)K);
   // dense dedupe slots: at most a quarter of the budget, <= 65536 hubs
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
   uint32_t Bmax =
@@ -1669,6 +1834,12 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   if (plain) Bmax = 1;
   if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
   if (Bmax == 0) return fail(HSDL_ENOMEM, "HBM budget below one hub's dedupe slots");
+  // slot windows of 2^slot_lw hub slots (tuning knob HSDL_SLOT_WINDOW, log2,
+  // default 4: 16 slots, one 64-byte sector), no wider than the batch cap,
+  // whose slots then fill whole windows
+  uint32_t slot_lw = env_u32("HSDL_SLOT_WINDOW", 4, 0, 8);
+  while (slot_lw && (1u << slot_lw) > Bmax) --slot_lw;
+  Bmax &= ~((1u << slot_lw) - 1u);
   list0 = std::max(list0, Bmax);  // a batch's first frontier (its hubs) must fit
   // heavy pieces per round (24 MB); when a round has more, the waves keep
   // the items that did not fit (HSDL_HEAVY_CAP lowers the cap)
@@ -1718,7 +1889,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   if ((!g.pool.p && (rc = g.pool.alloc(pool0 * 8))) || (rc = g.seg_start.alloc(n * 8ull)) ||
       (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
       (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
-      (rc = g.touched.alloc(n * 4ull)) ||
+      (rc = g.touched.alloc(n * 4ull)) || (rc = g.fgrp.alloc(n * 16ull)) ||
+      (group && (rc = g.hub_slot.alloc(n * 4ull))) ||
       (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
       (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))) ||
@@ -1775,6 +1947,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.row_run = env_u32("HSDL_ROW_RUN", 4, 1, 64);
   // tuning knob HSDL_ROW_BLOCK: vertices per y-block, rounded to 1024
   a.row_block = ((env_u32("HSDL_ROW_BLOCK", 2048, 1024, 1u << 20) + 1023) / 1024) * 1024;
+  // tuning knobs HSDL_HEAVY_PAIRS, HSDL_PIECE_PAIRS: (label, edge) pairs of a
+  // frontier group above which it is cut into heavy pieces, and per piece
+  // (a wave's light groups then hold at most 64 * heavy_pairs pairs).  Swept
+  // on R-MAT scale 18 / 20, K=5, expansion seconds: 256 / 2048 0.093 / 0.91,
+  // 256 / 8192 0.083 / 0.62, 512 / 16384 0.089 / 0.60, 1024 / 16384 0.099 /
+  // 0.66, 128 / 1024 0.13 / 1.44 (every 64 groups of a wave wait for its
+  // largest, so big light groups idle lanes; small pieces cost a claim each)
+  a.heavy_pairs = env_u32("HSDL_HEAVY_PAIRS", 256, 1, 1u << 16);
+  a.piece_pairs = env_u32("HSDL_PIECE_PAIRS", 8192, 1, 1u << 16);
   a.lds_slots = lds_slots;
   DevBuf hubs_buf, in_group;
   if (group) {
@@ -1802,7 +1983,10 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.add_cnt = g.add_cnt.as<uint32_t>();
   a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
+  a.fgrp = g.fgrp.as<FGroup>();
+  a.hub_slot = group ? g.hub_slot.as<uint32_t>() : nullptr;
   a.cand_d = static_cast<uint32_t *>(mg->d_slots);
+  a.slot_lw = slot_lw;
   {
     // slot layout: candidate distances need bits for K * max weight; the
     // rest of the word holds the attempt tag
@@ -1837,7 +2021,16 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // at 12.5 ms; scale 18 takes the same at 25 and 50 ms, longer at 100 ms);
   // HSDL_HUB_BATCH_MAX: the largest batch it may choose (default: the slots')
   const uint32_t tune_cap = env_u32("HSDL_HUB_BATCH_MAX", Bmax, 1, Bmax);
-  const double tune_target = env_u32("HSDL_BATCH_TARGET_US", 50000, 100, 10000000) * 1e-6;
+  // HSDL_TUNER=time tunes on batch durations (HSDL_BATCH_TARGET_US); the
+  // default tunes on work, HSDL_BATCH_TARGET_WORK units (candidates plus an
+  // eighth of the scanned edges) per batch, so the schedule repeats from run
+  // to run.  At 2e8 R-MAT scale 20 K=5 generates in 3.87 s against 4.12 s at
+  // 1e8 and 4.61 s at 5e7; the 1024x1024 grid is the same at all three.
+  const char *tm = getenv("HSDL_TUNER");
+  const bool tune_time = tm && !strcmp(tm, "time");
+  const double tune_target =
+      tune_time ? env_u32("HSDL_BATCH_TARGET_US", 50000, 100, 10000000) * 1e-6
+                : (double)env_u32("HSDL_BATCH_TARGET_WORK", 200000000u, 1000, 4000000000u);
   BatchTuner tuner{plain ? 1u : std::min<uint32_t>(tune_cap, 64u), tune_cap, tune_target};
   if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
   uint64_t batches = 0, redos = 0;
@@ -1853,7 +2046,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
     const uint64_t tb = now_ns();
     if (tag_next > tag_top) {  // wrapped below zero
       hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, g.a.cand_d,
-                         (uint64_t)rows_used * n, kInf32);
+                         (((uint64_t)rows_used + (1u << slot_lw) - 1) >> slot_lw << slot_lw) * n,
+                         kInf32);
       HSDL_HIP(hipGetLastError());
       tag_next = tag_top;
       rows_used = 0;
@@ -1919,6 +2113,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       if ((rc = grow_pool(g, c.pool_top, 0))) return rc;
       g.h_grow += (now_ns() - tg) * 1e-9;
     }
+    const double work = (double)(c.cand_total - c_ok.cand_total) +
+                        (double)(c.edge_total - c_ok.edge_total) / 8.0;
     c_ok = c;
     if (zones && B > 1 && (rc = sort_zones(g, c.maxlen, resort_div))) return rc;
     s0 += B;
@@ -1929,7 +2125,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
               (unsigned long long)batches, s0 - B, B, dt * 1e3,
               (unsigned long long)c.cand_total, (unsigned long long)c.pool_top,
               (unsigned long long)g.a.pool_cap);
-    if (!plain && !opts->hub_batch) tuner.next(dt);
+    if (!plain && !opts->hub_batch) tuner.next(tune_time ? dt : work);
     if (progress_s > 0) {
       const uint64_t now = now_ns();
       if (!next_progress) next_progress = t_csr + (uint64_t)(progress_s * 1e9);
