@@ -102,6 +102,9 @@ _SIGNATURES = [
     ("hsdl_generate", ctypes.c_int, [_vp, ctypes.c_uint8, ctypes.POINTER(hsdl_opts), ctypes.POINTER(_vp)]),
     ("hsdl_generate_group", ctypes.c_int, [_vp, ctypes.c_uint8, ctypes.POINTER(hsdl_opts), _u32p,
                                            ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    ("hsdl_graph_groups", ctypes.c_int, [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_uint64, ctypes.c_int32, _u32p,
+                                          ctypes.POINTER(ctypes.c_uint32)]),
     ("hsdl_clean", ctypes.c_int, [_vp]),
     ("hsdl_index_stats", ctypes.c_int, [_vp, ctypes.POINTER(hsdl_stats)]),
     ("hsdl_query", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
@@ -174,7 +177,7 @@ def now_ns():
 
 # the sources libhsdl.so is built from, in the order csrc/Makefile hashes them
 HASHED_SOURCES = ["index.hip", "gen.hip", "clean.hip", "query.hip", "server.hip", "util.hip",
-                  "parents.hip", "graph.cpp", "../../include/hsdl.h", "common.hpp", "device.hpp"]
+                  "parents.hip", "groups.hip", "graph.cpp", "../../include/hsdl.h", "common.hpp", "device.hpp"]
 
 
 def source_hash():
@@ -264,6 +267,17 @@ class Graph:
     @property
     def n(self):
         return self.info()[0]
+
+    GROUP_METHODS = {"contiguous": 0, "random": 1, "cdlp": 2}
+
+    def groups(self, max_size, method="cdlp", rounds=20, seed=0, device=-1):
+        """Vertex groups of at most max_size vertices (hsdl_graph_groups):
+        the group of every vertex, numbered in build order, and their count."""
+        out = np.zeros(self.n, np.uint32)
+        cnt = ctypes.c_uint32(0)
+        _check(lib().hsdl_graph_groups(self.h, self.GROUP_METHODS[method], max_size, rounds, seed,
+                                       device, out, ctypes.byref(cnt)))
+        return out, cnt.value
 
     def csr(self):
         n, nnz = self.info()

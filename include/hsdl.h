@@ -163,6 +163,29 @@ int hsdl_generate(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts, hsdl_in
 int hsdl_generate_group(const hsdl_graph *g, uint8_t K, const hsdl_opts *opts,
                         const uint32_t *hubs, uint64_t nhubs, hsdl_index **out);
 
+/* Vertex groups for group builds, each of at most max_size vertices:
+ * group_of[v] (n entries) receives vertex v's group, numbered in the order a
+ * driver takes them, and *ngroups their count.  Replaces the reference's
+ * vgroup generators (Hybrid_Generation_Clean/include/vgroup/):
+ *   HSDL_GROUP_CONTIGUOUS  ids in order, max_size per group;
+ *   HSDL_GROUP_RANDOM      a shuffle seeded by `seed`, cut every max_size
+ *                          (Random_group.cuh:10-33, whose seed is random);
+ *   HSDL_GROUP_CDLP        `rounds` rounds of label propagation on the device
+ *                          with community sizes capped at max_size, then
+ *                          communities cut to max_size, merged greedily and
+ *                          ordered by total degree, largest first
+ *                          (CDLP_group.cuh:20-107; CDLP/GPU_Community_
+ *                          Detection.cuh:104-210).  Deterministic: ties go
+ *                          to the smallest label and the size cap reads the
+ *                          previous round's sizes, where the reference's
+ *                          threads race on them.
+ * `device` is used by CDLP only (-1: the current device). */
+#define HSDL_GROUP_CONTIGUOUS 0u
+#define HSDL_GROUP_RANDOM 1u
+#define HSDL_GROUP_CDLP 2u
+int hsdl_graph_groups(const hsdl_graph *g, uint32_t method, uint32_t max_size, uint32_t rounds,
+                      uint64_t seed, int32_t device, uint32_t *group_of, uint32_t *ngroups);
+
 /* Canonical repair in place, on the GPU.  Replaces hop_constrained_clean_L
  * (CPU_HopHL/include/HBPLL/hop_constrained_two_hop_labels_generation.h:422-511)
  * and gpu_clean (Hybrid_Generation_Clean/src/label_clean.cu:244-329;

@@ -25,6 +25,8 @@
 //   label_gen(csr, info, L, nid_vec, nid_vec_id)
 //       (Hybrid_Generation_Clean/src/label_gen.cu:603-837)
 //   Graph_pool<int> (Hybrid_Generation_Clean/include/memoryManagement/graph_pool.hpp)
+//   generate_Group_CDLP, generate_Group_Random
+//       (Hybrid_Generation_Clean/include/vgroup/CDLP_group.cuh, Random_group.cuh)
 //   gpu_clean_init / gpu_clean(graph, info, res, thread_num, group)
 //       (Hybrid_Generation_Clean/src/label_clean.cu:166-329)
 //   double gpu_clean(graph, input_L, res, tc, K)   (Clean_pc/src/gpu_clean.cu:140)
@@ -47,6 +49,7 @@
 #include <chrono>
 #include <climits>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -606,16 +609,62 @@ CSR_graph<weight_type> toCSR(LDBC<weight_type> &graph) {
   return a;
 }
 
-// Vertex groups of the hybrid driver
-// (Hybrid_Generation_Clean/include/memoryManagement/graph_pool.hpp).
+// Vertex groups of the hybrid driver and the queue its executive cores take
+// them from (Hybrid_Generation_Clean/include/memoryManagement/
+// graph_pool.hpp:13-63): get_next_graph hands out group ids in order, -1
+// when none is left; size() is the count not yet handed out.
 template <typename T>
 class Graph_pool {
  public:
   std::vector<std::vector<T>> graph_group, graph_group_bfs;
+  int next_graph = 0;
+  std::mutex mtx;
   Graph_pool() = default;
   explicit Graph_pool(int groups) : graph_group(groups) {}
-  int size() { return (int)graph_group.size(); }
+  int get_next_graph() {
+    std::lock_guard<std::mutex> lock(mtx);
+    return next_graph < (int)graph_group.size() ? next_graph++ : -1;
+  }
+  int size() {
+    std::lock_guard<std::mutex> lock(mtx);
+    return (int)graph_group.size() - next_graph;
+  }
 };
+
+// Groups from hsdl_graph_groups, as member lists in group order.
+template <typename weight_type>
+void graph_groups(graph_v_of_v<weight_type> &instance_graph, uint32_t method, int max_group_size,
+                  uint32_t rounds, uint64_t seed, std::vector<std::vector<int>> &groups) {
+  if (max_group_size < 1) throw error(HSDL_EINVAL, "the group size must be at least 1");
+  hsdl_graph *g = instance_graph.to_handle();
+  std::vector<uint32_t> group_of(instance_graph.size());
+  uint32_t ng = 0;
+  const int rc = hsdl_graph_groups(g, method, (uint32_t)max_group_size, rounds, seed, -1,
+                                   group_of.data(), &ng);
+  hsdl_free_graph(g);
+  check(rc, "hsdl_graph_groups");
+  groups.assign(ng, {});
+  for (size_t v = 0; v < group_of.size(); ++v) groups[group_of[v]].push_back((int)v);
+}
+
+// generate_Group_CDLP (Hybrid_Generation_Clean/include/vgroup/
+// CDLP_group.cuh:20-107): capped label propagation on the GPU (at most 100
+// rounds; it stops once a round changes no label), communities cut and
+// merged to MAX_GROUP_SIZE, ordered by total degree, largest first.  Members
+// of a group come in ascending id order.
+inline void generate_Group_CDLP(graph_v_of_v<int> &instance_graph,
+                                std::vector<std::vector<int>> &groups, int MAX_GROUP_SIZE) {
+  graph_groups(instance_graph, HSDL_GROUP_CDLP, MAX_GROUP_SIZE, 100, 0, groups);
+}
+
+// generate_Group_Random (Hybrid_Generation_Clean/include/vgroup/
+// Random_group.cuh:10-33), whose group size is a global constant and whose
+// shuffle is seeded from std::random_device: both are arguments here.
+inline void generate_Group_Random(graph_v_of_v<int> &instance_graph, int /*hop_cst*/,
+                                  std::vector<std::vector<int>> &groups, int max_group_size,
+                                  uint64_t seed = 0) {
+  graph_groups(instance_graph, HSDL_GROUP_RANDOM, max_group_size, 0, seed, groups);
+}
 
 // Device state of the GPU path (Hybrid_Generation_Clean/include/label/
 // global_labels_v2.cuh:14-200): the reference's pools, hash tables and

@@ -4,7 +4,8 @@
 // Follows Hybrid_Generation_Clean/src/test.cu:219-478 with every core a GPU
 // core: read a |V|=/Edge graph, rename vertices by degree, convert it to LDBC
 // and CSR (toCSR), cut the vertices into groups of G_max consecutive ids
-// (use_cd = 0, :310-320), set up hop_constrained_case_info_v2 (init,
+// (use_cd = 0, :310-320), or take CDLP groups (use_cd = 1, :321-325) or
+// shuffled ones, set up hop_constrained_case_info_v2 (init,
 // set_nid), hand the groups to the least-loaded of four GPU cores
 // (Executive_Core queue, :361-383) that each run label_gen, release the device
 // state (destroy_L_cuda, destroy_csr_graph), sort every L(v), then clean per
@@ -23,11 +24,13 @@
 // "check pass".
 //
 // usage: compat_hybrid_driver <graph.txt> <K> <G_max> <pairs> <out.bin>
+//                             [contiguous|cdlp|random]
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <queue>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "hsdl_compat.hpp"
@@ -84,9 +87,11 @@ static bool same_lists(const std::vector<std::vector<hop_constrained_two_hop_lab
 
 int main(int argc, char **argv) {
   if (argc < 6) {
-    fprintf(stderr, "usage: %s graph.txt K G_max pairs out.bin\n", argv[0]);
+    fprintf(stderr, "usage: %s graph.txt K G_max pairs out.bin [contiguous|cdlp|random]\n",
+            argv[0]);
     return 2;
   }
+  const std::string grouping = argc > 6 ? argv[6] : "contiguous";
   const int hop_cst = atoi(argv[2]);
   int G_max = atoi(argv[3]);
   const int pairs = atoi(argv[4]);
@@ -96,19 +101,30 @@ int main(int argc, char **argv) {
     g0.txt_read(argv[1]);
     graph_v_of_v<int> instance_graph = graph_v_of_v_update_vertexIDs_by_degrees_large_to_small(g0);
     const int V = instance_graph.size();
-    const int Distributed_Graph_Num = (V + G_max - 1) / G_max;
+    int Distributed_Graph_Num = (V + G_max - 1) / G_max;
 
     LDBC<int> graph(V);
     graph_v_of_v_to_LDBC(graph, instance_graph);
     CSR_graph<int> csr_graph = toCSR(graph);
 
+    // groups as test.cu:309-326 makes them (use_cd 0: contiguous; 1: CDLP),
+    // or shuffled (Random_group.cuh)
     Graph_pool<int> graph_pool;
-    graph_pool.graph_group.resize(Distributed_Graph_Num);
-    const int Nodes_Per_Graph = (V - 1) / Distributed_Graph_Num + 1;
-    for (int i = 0; i < Distributed_Graph_Num; ++i)
-      for (int j = Nodes_Per_Graph * i; j < Nodes_Per_Graph * (i + 1) && j < V; ++j)
-        graph_pool.graph_group[i].push_back(j);
-    G_max = V / Distributed_Graph_Num + 1;
+    if (grouping == "cdlp") {
+      generate_Group_CDLP(instance_graph, graph_pool.graph_group, G_max);
+      Distributed_Graph_Num = (int)graph_pool.graph_group.size();
+    } else if (grouping == "random") {
+      generate_Group_Random(instance_graph, hop_cst, graph_pool.graph_group, G_max, 7);
+      Distributed_Graph_Num = (int)graph_pool.graph_group.size();
+    } else {
+      graph_pool.graph_group.resize(Distributed_Graph_Num);
+      const int Nodes_Per_Graph = (V - 1) / Distributed_Graph_Num + 1;
+      for (int i = 0; i < Distributed_Graph_Num; ++i)
+        for (int j = Nodes_Per_Graph * i; j < Nodes_Per_Graph * (i + 1) && j < V; ++j)
+          graph_pool.graph_group[i].push_back(j);
+      G_max = V / Distributed_Graph_Num + 1;
+    }
+    graph_pool.graph_group_bfs.resize(Distributed_Graph_Num);
 
     hop_constrained_case_info_v2 *info_gpu = new hop_constrained_case_info_v2();
     info_gpu->init(V, hop_cst, G_max, thread_num, graph_pool.graph_group);

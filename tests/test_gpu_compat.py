@@ -63,6 +63,42 @@ def test_hybrid_driver_groups_match_the_oracle(tmp_path, golden_dir, name, K, gm
     assert np.array_equal(hop, whop) and np.array_equal(dist, wdist)
 
 
+@pytest.mark.parametrize("grouping", ["cdlp", "random"])
+@pytest.mark.parametrize("name,K,gmax", [("simple_iterative_tests.txt", 4, 60),
+                                         ("euroroad2.txt", 5, 250)])
+def test_hybrid_driver_with_cdlp_and_random_groups(tmp_path, golden_dir, grouping, name, K,
+                                                   gmax):
+    """The hybrid driver with use_cd = 1 (generate_Group_CDLP,
+    Hybrid_Generation_Clean/src/test.cu:321-325) or shuffled groups
+    (Random_group.cuh): the groups are hsdl_graph_groups' on the renamed graph,
+    the uncleaned labels the oracle's group-restricted generation over them,
+    and the cleaned labels the canonical set, label for label."""
+    exe = os.path.join(ROOT, "tests", "compat_hybrid_driver")
+    out_bin = tmp_path / "labels.bin"
+    out = subprocess.run([exe, os.path.join(golden_dir, name), str(K), str(gmax), "300",
+                          str(out_bin), grouping], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "check pass" in out.stdout
+    g = H.Graph.load_txt(os.path.join(golden_dir, name))
+    g.reorder_by_degree()
+    if grouping == "cdlp":
+        gid, ng = g.groups(gmax, "cdlp", rounds=100, device=0)
+    else:
+        gid, ng = g.groups(gmax, "random", seed=7)
+    assert ng >= 2 and np.bincount(gid).max() <= gmax
+    c, _ = O.reorder(O.load_txt(os.path.join(golden_dir, name)))
+    red = O.kround_gid(c, K, gid.astype(np.int32))
+    cl = O.clean(red)
+    assert cl.total() == O.kround(c, K).total()
+    line = [l for l in out.stdout.splitlines() if l.startswith("groups")][0]
+    assert line.startswith(f"groups {ng},"), line
+    assert f"labels before clean {red.total()}, after clean {cl.total()}" in line, line
+    off, hub, hop, dist = read_dump(str(out_bin))
+    woff, whub, whop, wdist = cl.arrays()
+    assert np.array_equal(off, woff) and np.array_equal(hub, whub)
+    assert np.array_equal(hop, whop) and np.array_equal(dist, wdist)
+
+
 @pytest.fixture(scope="module")
 def dijkstra_golden(golden_dir):
     return np.load(os.path.join(golden_dir, "dijkstra_k.npz"))
