@@ -110,6 +110,9 @@ struct Ctrl {
   uint32_t heavy_ctr;   // next heavy piece (k_expand_heavy)
   uint32_t row_mode;    // this round's prune takes row items (k_prune_rows)
   uint32_t row_ctr[8];  // next row item of each XCD's queue
+  // the running batch attempt (k_set_batch): the kernels take these instead
+  // of their arguments' s0 / B / tag, so one captured batch replays for any
+  uint32_t bs0, bB, btag;
 };
 
 // A piece of a large frontier group: its cnt labels (at pool offset off)
@@ -163,12 +166,13 @@ struct GenArgs {
   uint32_t *touched;
   FGroup *fgrp;               // frontier groups (n entries)
   uint32_t *hub_slot;         // group builds: hub slot of each hub of the batch
-  // dense slots, tagged (slot_fresh / slot_stale), in hub windows of
+  // dense slots, tagged (slot_fresh), in hub windows of
   // 2^slot_lw slots: [Bmax >> slot_lw][n][2^slot_lw] (slot_at)
   uint32_t *cand_d;
   uint32_t slot_lw;
   uint32_t tag;               // epoch of this batch attempt (high bits of a slot)
-  uint32_t slot_shift;        // tag << slot_shift | dist << 1 | stale
+  uint32_t slot_shift;        // tag << slot_shift | dist << slot_rbits | round
+  uint32_t slot_rbits;        // bits of the round field (rounds 1..K)
   unsigned long long *cflat;  // (b << 32 | y), ccap entries
   uint32_t *cgroup;           // y grouped by hub, ccap entries
   uint32_t ccap;
@@ -187,6 +191,20 @@ struct GenArgs {
 };
 
 __device__ inline Item *frontier(const GenArgs &a, uint32_t which) { return which ? a.F1 : a.F0; }
+
+// The batch attempt's first hub slot, size and slot tag, from the control
+// block (wave-uniform).
+__device__ inline void with_batch(GenArgs &a) {
+  a.s0 = __builtin_amdgcn_readfirstlane(a.ctrl->bs0);
+  a.B = __builtin_amdgcn_readfirstlane(a.ctrl->bB);
+  a.tag = __builtin_amdgcn_readfirstlane(a.ctrl->btag);
+}
+
+__global__ void k_set_batch(Ctrl *c, uint32_t s0, uint32_t B, uint32_t tag) {
+  c->bs0 = s0;
+  c->bB = B;
+  c->btag = tag;
+}
 
 // Dense slot of (hub slot b, vertex y): the slots of a window of 2^slot_lw
 // consecutive hub slots lie side by side for every vertex, so the expansion
@@ -265,30 +283,30 @@ template <> __device__ inline const unsigned long long *stream_pool<unsigned lon
 }
 
 // Dense dedupe slots keep, per (hub slot, vertex), the smallest candidate
-// distance of every round of the current batch attempt:
-//   fresh  tag << shift | d << 1        a candidate of the running round
-//   stale  tag << shift | (2d - 1)      left by an earlier round (d >= 1)
+// distance of every round of the current batch attempt, with the round that
+// set it:  tag << shift | d << rbits | h.
 // Older attempts carry larger tags (tags count down; a cleared slot is all
 // ones), so any value of the running attempt is below them.  A round-h
 // candidate whose distance is not below an earlier round's candidate for the
 // same (s, y) is always pruned -- the earlier one was either accepted (then
 // (s,0,0) in L(s) with that label is a witness) or pruned by a witness that
-// still applies with h >= its hop count -- so the expansion drops it: with
-// stale = 2d - 1, a fresh value undercuts a stale one exactly when its
-// distance is smaller.
+// still applies with h >= its hop count -- so the expansion drops it: a
+// round-h value undercuts an earlier round's exactly when its distance is
+// smaller (at equal distance the later round compares larger), and a slot
+// opens for round h when its old value is from another attempt or round.
+// The prune leaves the slot as it is.
 __device__ inline uint32_t slot_fresh(const GenArgs &a, uint32_t d) {
-  return (a.tag << a.slot_shift) | (d << 1);
-}
-__device__ inline uint32_t slot_stale(const GenArgs &a, uint32_t d) {
-  return (a.tag << a.slot_shift) | (d ? 2 * d - 1 : 1u);
+  return (a.tag << a.slot_shift) | (d << a.slot_rbits) | a.h;
 }
 __device__ inline uint32_t slot_dist(const GenArgs &a, uint32_t v) {
-  return (v & ((1u << a.slot_shift) - 1u)) >> 1;
+  return (v & ((1u << a.slot_shift) - 1u)) >> a.slot_rbits;
+}
+// a candidate of the running round (attempt and round match)
+__device__ inline bool slot_current(const GenArgs &a, uint32_t v) {
+  return (v >> a.slot_shift) == a.tag && (v & ((1u << a.slot_rbits) - 1u)) == a.h;
 }
 // first candidate of the running round at a slot whose previous value was `old`
-__device__ inline bool slot_opens(const GenArgs &a, uint32_t old) {
-  return (old & 1u) || (old >> a.slot_shift) != a.tag;
-}
+__device__ inline bool slot_opens(const GenArgs &a, uint32_t old) { return !slot_current(a, old); }
 
 // ------------------------------------------------------------------ init
 __global__ void k_init_labels(GenArgs a, uint32_t c0) {
@@ -317,6 +335,7 @@ __global__ void k_fill_u32(uint32_t *p, uint64_t n, uint32_t val) {
 // The first frontier: every hub of the batch with its self label (position
 // 0 of its list), one group each.
 __global__ void k_batch_begin(GenArgs a) {
+  with_batch(a);
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x) {
     const uint32_t s = hub_of(a, b);
     a.fgrp[b] = FGroup{s, 1u, a.seg_start[s]};
@@ -329,6 +348,7 @@ __global__ void k_batch_begin(GenArgs a) {
 }
 
 __global__ void k_round_prep(GenArgs a) {
+  with_batch(a);
   for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.B; b += gridDim.x * blockDim.x) {
     a.hub_ccnt[b] = 0;
     a.hub_cur[b] = 0;
@@ -467,6 +487,7 @@ __device__ inline bool relax_edge(const GenArgs &a, bool valid, uint32_t kb, uin
 // the slot windows.  Lanes whose label's hub ranks above the neighbour do no
 // work (rank pruning).
 __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
+  with_batch(a);
   if (a.ctrl->overflow) return;  // the batch is rolled back and run again
   const FGroup *G = a.fgrp;
   const uint32_t gcnt = min(a.ctrl->gcnt, a.n);
@@ -583,6 +604,7 @@ __global__ void __launch_bounds__(256) k_expand(GenArgs a) {
 // Heavy pieces: a workgroup takes one piece at a time and spreads its pairs
 // over its 256 lanes, edge-major like k_expand.
 __global__ void __launch_bounds__(256) k_expand_heavy(GenArgs a) {
+  with_batch(a);
   __shared__ uint32_t sh_h;
   if (a.ctrl->overflow) return;
   const int lane = lane_id();
@@ -666,6 +688,7 @@ __device__ inline uint32_t group_slot(uint32_t *hkey, uint32_t b) {
 }
 
 __global__ void __launch_bounds__(kGroupThreads) k_count(GenArgs a) {
+  with_batch(a);
   __shared__ uint32_t hkey[kGroupSlots];
   __shared__ uint32_t hcnt[kGroupSlots];
   if (a.ctrl->overflow || rows_chosen(a)) return;
@@ -695,6 +718,7 @@ __global__ void __launch_bounds__(kGroupThreads) k_count(GenArgs a) {
 // One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
 // sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
 __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
+  with_batch(a);
   __shared__ uint32_t part[1024];
   __shared__ uint32_t partc[1024];
   const uint32_t t = threadIdx.x;
@@ -744,6 +768,7 @@ __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
 // candidates do not serialize the scatter on their counters.
 
 __global__ void __launch_bounds__(kGroupThreads) k_group(GenArgs a) {
+  with_batch(a);
   __shared__ uint32_t hkey[kGroupSlots];
   __shared__ uint32_t hcnt[kGroupSlots];
   __shared__ uint32_t hbase[kGroupSlots];
@@ -1040,7 +1065,6 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
                                             reads);
       wave_reads += reads;
       if (lane == 0) {
-        a.cand_d[slot_at(a, b, yi)] = slot_stale(a, di);  // later rounds must undercut di
         if (!pruned) {
           const uint32_t p = atomicAdd(&sh_acc_n, 1u);
           sh_acc[p] = Item{b, yi, di};
@@ -1055,6 +1079,7 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
 template <class V, class LT>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune(GenArgs a) {
+  with_batch(a);
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next, sh_touch_n, sh_touch_base;
   __shared__ uint32_t sh_item, sh_item_end;
@@ -1170,6 +1195,7 @@ k_prune(GenArgs a) {
 template <class V, class LT>
 __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per_eu(8)))
 k_prune_rows(GenArgs a) {
+  with_batch(a);
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_b, sh_k, sh_kn, sh_q, sh_next, sh_cnt;
   __shared__ uint32_t sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base;
@@ -1244,7 +1270,7 @@ k_prune_rows(GenArgs a) {
       __syncthreads();
       const uint32_t y = y0 + threadIdx.x;
       const uint32_t v = y < y_end ? a.cand_d[slot_at(a, b, y)] : kInf32;
-      const bool cand = y < y_end && (v >> a.slot_shift) == a.tag && !(v & 1u);
+      const bool cand = y < y_end && slot_current(a, v);
       const unsigned long long m = __ballot(cand);
       uint32_t base = 0;
       if (lane == 0 && m) base = atomicAdd(&sh_cnt, (uint32_t)__popcll(m));
@@ -1296,6 +1322,7 @@ k_prune_rows(GenArgs a) {
 
 // -------------------------------------------------------- reserve / append
 __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
+  with_batch(a);
   const int lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
@@ -1326,6 +1353,7 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
 }
 
 __global__ void __launch_bounds__(256) k_append(GenArgs a) {
+  with_batch(a);
   const Item *Fn = frontier(a, a.cur ^ 1);
   const uint32_t cnt = min(a.ctrl->fcnt[a.cur ^ 1], a.fcap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -1343,6 +1371,7 @@ __global__ void __launch_bounds__(256) k_append(GenArgs a) {
 }
 
 __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
+  with_batch(a);
   const uint32_t cnt = a.ctrl->touched_cnt;
   // a lost frontier item leaves a counted label unwritten: publish nothing
   // (the host rolls the batch back)
@@ -1372,6 +1401,7 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
 // position 0 stays.  The redo runs under a new slot tag, so the batch's dense
 // slots need no refill.
 __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
+  with_batch(a);
   const uint32_t lo = hub_of(a, 0), hi = hub_of(a, a.B - 1);
   for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.n; v += gridDim.x * blockDim.x) {
     uint32_t len = a.seg_len[v];
@@ -1562,6 +1592,13 @@ struct Gen {
   int prune_grid = 0;  // resident k_prune workgroups (two per CU)
   size_t lds_bytes = 0;
   bool small_vals = false;  // 16-bit tile values
+  uint32_t bmax = 1;        // largest batch (slot rows)
+  // captured batch sequences (without / with the row-item prune) and the
+  // buffers each was captured with
+  bool use_graph = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  GenArgs gkey[2]{};
+  uint64_t graph_captures = 0;
   hipStream_t st = nullptr;
   bool verbose = false;
   // kernel timing (events around every expand and prune launch of a batch)
@@ -1576,6 +1613,8 @@ struct Gen {
   }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+    for (hipGraphExec_t x : gexec)
+      if (x) (void)hipGraphExecDestroy(x);
   }
 };
 
@@ -1661,27 +1700,25 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
 }
 
 // Enqueues one batch (K rounds) of hubs [s0, s0+B).
-static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t nhubs) {
-  // k_batch_begin writes the B hubs into the first frontier unchecked
-  if (B == 0 || B > g.a.fcap || s0 + (uint64_t)B > nhubs)
-    return fail(HSDL_EINVAL, "hub batch does not fit the frontier list or the hub sequence");
-  GenArgs a = g.a;
-  a.s0 = s0;
-  a.B = B;
+// The launches of one batch attempt (its K rounds) on g.st.  The kernels
+// read the batch's s0 / B / tag from the control block (k_set_batch), so the
+// sequence depends only on the buffers, K and whether row items may run.
+static void launch_rounds(Gen &g, GenArgs a, uint32_t K, bool rows) {
   a.cur = 0;
-  const int small_grid = (int)std::min<uint32_t>((B + 255) / 256, 1024);
+  // grid-stride loops over the batch's hubs: sized for the largest batch
+  const int small_grid = (int)std::min<uint32_t>((g.bmax + 255) / 256, 1024);
   hipLaunchKernelGGL(k_batch_begin, dim3(small_grid), dim3(256), 0, g.st, a);
   for (uint32_t h = 1; h <= K; ++h) {
     a.h = h;
     hipLaunchKernelGGL(k_round_prep, dim3(small_grid), dim3(256), 0, g.st, a);
-    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 0], g.st));
+    if (g.timing) (void)hipEventRecord(g.ev[4 * (h - 1) + 0], g.st);
     hipLaunchKernelGGL(k_expand, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_expand_heavy, dim3(g.grid), dim3(256), 0, g.st, a);
-    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 1], g.st));
+    if (g.timing) (void)hipEventRecord(g.ev[4 * (h - 1) + 1], g.st);
     hipLaunchKernelGGL(k_count, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
     hipLaunchKernelGGL(k_chunk_scan, dim3(1), dim3(1024), 0, g.st, a);
     hipLaunchKernelGGL(k_group, dim3(g.grid), dim3(kGroupThreads), 0, g.st, a);
-    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 2], g.st));
+    if (g.timing) (void)hipEventRecord(g.ev[4 * (h - 1) + 2], g.st);
     {
       // 16- or 32-bit tile values; 32- or 64-bit labels streamed
       auto kp = g.small_vals ? (a.pool32 ? k_prune<uint16_t, uint32_t>
@@ -1692,7 +1729,7 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t n
       // row items instead, when k_chunk_scan chose them (one of the two
       // launches returns at once); never for a one-hub batch unless forced,
       // so the plain form pays no second launch per round
-      if (B > 1 || a.row_policy == 2) {
+      if (rows) {
         auto kr = g.small_vals ? (a.pool32 ? k_prune_rows<uint16_t, uint32_t>
                                            : k_prune_rows<uint16_t, unsigned long long>)
                                : (a.pool32 ? k_prune_rows<uint32_t, uint32_t>
@@ -1700,11 +1737,66 @@ static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t n
         hipLaunchKernelGGL(kr, dim3(g.prune_grid), dim3(kPruneThreads), g.lds_bytes, g.st, a);
       }
     }
-    if (g.timing) HSDL_HIP(hipEventRecord(g.ev[4 * (h - 1) + 3], g.st));
+    if (g.timing) (void)hipEventRecord(g.ev[4 * (h - 1) + 3], g.st);
     hipLaunchKernelGGL(k_reserve, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_append, dim3(g.grid), dim3(256), 0, g.st, a);
     hipLaunchKernelGGL(k_finalize, dim3(g.grid), dim3(256), 0, g.st, a);
     a.cur ^= 1;
+  }
+}
+
+// The buffers a captured batch has baked in (they change when one grows).
+static bool same_buffers(const GenArgs &x, const GenArgs &y) {
+  return x.pool == y.pool && x.pool_cap == y.pool_cap && x.pool32 == y.pool32 &&
+         x.gtile == y.gtile && x.gtile_cap == y.gtile_cap && x.cflat == y.cflat &&
+         x.cgroup == y.cgroup && x.ccap == y.ccap && x.F0 == y.F0 && x.F1 == y.F1 &&
+         x.fcap == y.fcap;
+}
+
+// Captures the batch sequence (with or without the row-item prune) into a
+// HIP graph, once per buffer state; false when capture is not possible here
+// (the batch then runs as separate launches).
+static bool batch_graph(Gen &g, uint32_t K, bool rows) {
+  const int v = rows ? 1 : 0;
+  if (g.gexec[v] && same_buffers(g.gkey[v], g.a)) return true;
+  if (g.gexec[v]) {
+    (void)hipGraphExecDestroy(g.gexec[v]);
+    g.gexec[v] = nullptr;
+  }
+  hipGraph_t graph = nullptr;
+  if (hipStreamBeginCapture(g.st, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  launch_rounds(g, g.a, K, rows);
+  const hipError_t e = hipStreamEndCapture(g.st, &graph);
+  hipGraphExec_t ex = nullptr;
+  const bool ok = e == hipSuccess && graph &&
+                  hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0) == hipSuccess;
+  if (graph) (void)hipGraphDestroy(graph);
+  if (!ok) {
+    (void)hipGetLastError();
+    if (g.verbose) fprintf(stderr, "[hsdl] batch graph capture failed; separate launches\n");
+    return false;
+  }
+  g.gexec[v] = ex;
+  g.gkey[v] = g.a;
+  ++g.graph_captures;
+  return true;
+}
+
+static int enqueue_batch(Gen &g, uint32_t s0, uint32_t B, uint32_t K, uint32_t nhubs) {
+  // k_batch_begin writes the B hubs into the first frontier unchecked
+  if (B == 0 || B > g.a.fcap || B > g.bmax || s0 + (uint64_t)B > nhubs)
+    return fail(HSDL_EINVAL, "hub batch does not fit the frontier list or the hub sequence");
+  hipLaunchKernelGGL(k_set_batch, dim3(1), dim3(1), 0, g.st, g.a.ctrl, s0, B, g.a.tag);
+  const bool rows = B > 1 || g.a.row_policy == 2;
+  // one graph launch per batch instead of 1 + 11 K launches (the plain form
+  // is bound by the host's launch rate); not with per-launch timing events
+  if (g.use_graph && !g.timing && batch_graph(g, K, rows)) {
+    HSDL_HIP(hipGraphLaunch(g.gexec[rows ? 1 : 0], g.st));
+  } else {
+    launch_rounds(g, g.a, K, rows);
   }
   HSDL_HIP(hipGetLastError());
   return HSDL_OK;
@@ -1840,6 +1932,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   uint32_t slot_lw = env_u32("HSDL_SLOT_WINDOW", 4, 0, 8);
   while (slot_lw && (1u << slot_lw) > Bmax) --slot_lw;
   Bmax &= ~((1u << slot_lw) - 1u);
+  g.bmax = Bmax;
+  // tuning knob HSDL_GRAPH=0: launch every kernel of a batch separately
+  g.use_graph = env_u32("HSDL_GRAPH", 1, 0, 1) != 0;
   list0 = std::max(list0, Bmax);  // a batch's first frontier (its hubs) must fit
   // heavy pieces per round (24 MB); when a round has more, the waves keep
   // the items that did not fit (HSDL_HEAVY_CAP lowers the cap)
@@ -1918,8 +2013,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // form): sorted-zone lengths and per-hop prefix counts
   const bool zones = env_u32("HSDL_HOP_ZONES", 1, 0, 1) && K >= 2 && !plain;
   // a list is sorted again once its append zone passes 1/resort_div of its
-  // sorted zone (HSDL_RESORT_DIV)
-  const uint32_t resort_div = env_u32("HSDL_RESORT_DIV", 8, 1, 1u << 20);
+  // sorted zone (HSDL_RESORT_DIV; R-MAT scale 18 / 20 builds: 2 0.551 /
+  // 5.34 s, 4 0.463 / 4.42 s, 8 0.471 / 4.49 s, 16 0.473 / 4.66 s)
+  const uint32_t resort_div = env_u32("HSDL_RESORT_DIV", 4, 1, 1u << 20);
   if (zones && ((rc = g.sorted_len.alloc(n * 4ull)) ||
                 (rc = g.hcnt.alloc((uint64_t)n * (K - 1) * 4))))
     return rc;
@@ -1988,12 +2084,14 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.cand_d = static_cast<uint32_t *>(mg->d_slots);
   a.slot_lw = slot_lw;
   {
-    // slot layout: candidate distances need bits for K * max weight; the
-    // rest of the word holds the attempt tag
+    // slot layout: candidate distances need bits for K * max weight, the
+    // round field bits for K; the rest of the word holds the attempt tag
     const uint64_t dmax = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1);
-    uint32_t dist_bits = 1;
+    uint32_t dist_bits = 1, rbits = 1;
     while ((1ull << dist_bits) <= dmax) ++dist_bits;
-    a.slot_shift = dist_bits + 1;  // <= 28: dmax < 2^27
+    while ((1u << rbits) <= K) ++rbits;
+    a.slot_rbits = rbits;
+    a.slot_shift = dist_bits + rbits;  // <= 31: dmax < 2^27, K <= 15
   }
   a.hub_ccnt = g.hub_ccnt.as<uint32_t>();
   a.hub_off = g.hub_off.as<uint32_t>();
@@ -2116,7 +2214,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
     const double work = (double)(c.cand_total - c_ok.cand_total) +
                         (double)(c.edge_total - c_ok.edge_total) / 8.0;
     c_ok = c;
-    if (zones && B > 1 && (rc = sort_zones(g, c.maxlen, resort_div))) return rc;
+    // (no prune follows the last batch: its lists stay as they are)
+    if (zones && B > 1 && s0 + B < nhubs && (rc = sort_zones(g, c.maxlen, resort_div)))
+      return rc;
     s0 += B;
     ++batches;
     const double dt = (now_ns() - tb) * 1e-9;
@@ -2188,10 +2288,10 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   idx->stats.t_parents_s = (t_end - t_sorted) * 1e-9;
   if (g.verbose)
     fprintf(stderr, "[hsdl] generate: %.3f s batches (%llu, %llu redo; host enqueue %.3f s, wait %.3f s, "
-            "grow %.3f s), %.3f s export (%.3f s sort), %llu labels\n",
+            "grow %.3f s; %llu graph captures), %.3f s export (%.3f s sort), %llu labels\n",
             (t_gen - t_csr) * 1e-9, (unsigned long long)batches, (unsigned long long)redos,
-            g.h_enqueue, g.h_wait, g.h_grow, (t_end - t_gen) * 1e-9, (t_end - t_sort0) * 1e-9,
-            (unsigned long long)total);
+            g.h_enqueue, g.h_wait, g.h_grow, (unsigned long long)g.graph_captures,
+            (t_end - t_gen) * 1e-9, (t_end - t_sort0) * 1e-9, (unsigned long long)total);
   idx->stats.labels_uncleaned = total;
   idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
   idx->stats.t_sort_s = (t_sorted - t_sort0) * 1e-9;

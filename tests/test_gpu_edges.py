@@ -1,5 +1,6 @@
-"""GPU edge cases: the exact 32-bit label packing, prune block sizes, HBM
-budget failures, server back-pressure and the label dump's byte layout.
+"""GPU edge cases: the exact 32-bit label packing, prune block sizes, batch
+graph replays, HBM budget failures, server back-pressure and the label
+dump's byte layout.
 Every comparison with the oracle is exact (integer work)."""
 import numpy as np
 import pytest
@@ -118,6 +119,24 @@ def test_row_block_sizes(monkeypatch, heavy13, ragged_grid, block, rows):
     same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=100), want)
     gg, gwant = ragged_grid
     same_labels(H.hop_constrained_two_hop_labels_generation(gg, 6), gwant)
+
+
+@pytest.mark.parametrize("graph", ["1", "0"])
+def test_batch_graphs_replay_exactly(monkeypatch, heavy13, ragged_grid, graph):
+    """Every batch runs as one replay of a captured HIP graph (HSDL_GRAPH=1,
+    the default) or as separate launches (0): the same labels on R-MAT scale
+    13 and the ragged grid, over many replays (batches of 7 and 100 hubs,
+    the autotuned schedule, the plain form), with the host's reads and
+    synchronisations between replays, and across re-captures after the
+    candidate and frontier lists overflow and grow (HSDL_LIST_INIT)."""
+    monkeypatch.setenv("HSDL_GRAPH", graph)
+    g, want = heavy13
+    for kw in ({}, {"hub_batch": 7}, {"hub_batch": 100}, {"mode": H.MODE_PLAIN}):
+        same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, **kw), want)
+    gg, gwant = ragged_grid
+    same_labels(H.hop_constrained_two_hop_labels_generation(gg, 6, hub_batch=500), gwant)
+    monkeypatch.setenv("HSDL_LIST_INIT", "4096")
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=300), want)
 
 
 def test_hbm_budget_failures_are_clean(golden_dir):
