@@ -291,9 +291,11 @@ def rooflines(H, stats, config, K, ref, clean_bytes):
         # witness) + 8 B per label of L(s) staged + 8 B per candidate
         "k_prune": (stats.get("prune_label_bytes") or 8) * stats["prune_label_reads"]
         + 8 * stats["prune_tile_reads"] + 8 * stats["candidates"],
-        # 28 B per frontier item + 12 B per scanned edge (8 B (col, w) + the
-        # 4 B dense slot) + 8 B per candidate appended
-        "k_expand": 28 * stats["frontier_items"] + 12 * stats["scanned_edges"]
+        # 8 B per frontier label (its label and its share of the group
+        # record) + 12 B per relaxed (label, edge) pair (8 B (col, w), which
+        # the labels of a group share, so an upper bound, + the 4 B dense
+        # slot) + 8 B per candidate appended
+        "k_expand": 8 * stats["frontier_items"] + 12 * stats["scanned_edges"]
         + 8 * stats["candidates"],
     }
     alg = None

@@ -88,8 +88,8 @@ struct Stream64 {
   using T = unsigned long long;
   const unsigned long long *lab;
   __device__ const T *list(unsigned long long b) const { return lab + b; }
-  // elements of L(u) a group of max hop `maxh` can use: the whole list
-  __device__ uint32_t usable(uint32_t, uint32_t, uint32_t len) const { return len; }
+  // elements of L(u) of hop <= maxhop: the whole list (not hop-sorted)
+  __device__ uint32_t upto(uint32_t, uint32_t, uint32_t len) const { return len; }
   __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
     w = label_hub(l);
     h = label_hop(l);
@@ -99,9 +99,11 @@ struct Stream64 {
 // The 32-bit copy keeps each list stably sorted by hop (hub order within a
 // hop), with hcnt[u * K + x] = elements of L(u) of hop <= x (x < K), when
 // hcnt is set: an element pairs with the tile through hop h1 + 1 or more
-// (w != v), so a group whose deepest label has hop maxh can use only L(u)'s
-// elements of hop <= maxh - 1 -- a prefix of the sorted list, which the
-// stream stops at (the self label, hop 0, is its first element).
+// when w != v, so a group whose deepest label has hop maxh can use only
+// L(u)'s elements of hop <= maxh - 1 -- a prefix of the sorted list, which
+// the stream stops at (the self label, hop 0, is its first element).  Only a
+// head u above v can hold w = v (pairing with v's self label, hop 0): its
+// stream goes to hop maxh.
 struct Stream32 {
   using T = uint32_t;
   const uint32_t *lab;
@@ -109,8 +111,8 @@ struct Stream32 {
   const uint32_t *hcnt;
   uint32_t K;
   __device__ const T *list(unsigned long long b) const { return lab + b; }
-  __device__ uint32_t usable(uint32_t u, uint32_t maxh, uint32_t len) const {
-    return hcnt ? hcnt[(uint64_t)u * K + (maxh - 1)] : len;
+  __device__ uint32_t upto(uint32_t u, uint32_t maxhop, uint32_t len) const {
+    return hcnt && maxhop < K ? hcnt[(uint64_t)u * K + maxhop] : len;
   }
   __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
     w = l >> (hop_bits + dist_bits);
@@ -213,7 +215,11 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       ++glen;
     }
     const unsigned long long ub = off[u];
-    const uint32_t ulen = str.usable(u, maxh, (uint32_t)(off[u + 1] - ub));
+    // an element (w,h1,d1) of L(u) pairs with the tile through hop >= h1 + 1
+    // when w != v; w = v (possible only for u > v, in imported label sets
+    // whose hubs may rank below their vertex) pairs with v's self label
+    const uint32_t hmin = u > v ? 0u : 1u;
+    const uint32_t ulen = str.upto(u, maxh - hmin, (uint32_t)(off[u + 1] - ub));
     if (ulen == 0) continue;
     const typename S::T *Lu = str.list(ub);
     // the group's hops and distances, decoded once per head into scalars for
@@ -237,7 +243,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       const bool in = j < ulen;
       // a pair through w < u has hop >= h1 + 1 (w != v) and dist >= d1; the
       // conditions are combined into one branch (one exec-mask change)
-      if (in & (w < u) & (h1 + 1 <= maxh) & (d1 <= maxd)) {
+      if (in & (w < u) & (h1 + hmin <= maxh) & (d1 <= maxd)) {
         const uint32_t slot = tile_slot(keys, cap, w);
         if (slot != kInf32) {
           const uint32_t base = mul_u24(slot, K1);
@@ -245,17 +251,17 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
 #pragma unroll
             for (uint32_t t = 0; t < kGroupFast; ++t) {
               if (t >= glen) break;
-              // a label no deeper than h1 reads the slot's first value and
-              // is masked
-              const bool deeper = gh[t] > h1;
-              const uint32_t m = vals[base + (deeper ? gh[t] - h1 : 0u)];
-              hit |= deeper & (m != TileVal<V>::kEmpty) & (d1 + m <= gd[t]);
+              // a label shallower than h1 reads the slot's first value and
+              // is masked; at equal hops only a hop-0 tile label pairs
+              const bool reach = gh[t] >= h1;
+              const uint32_t m = vals[base + (reach ? gh[t] - h1 : 0u)];
+              hit |= reach & (m != TileVal<V>::kEmpty) & (d1 + m <= gd[t]);
             }
           } else {
             for (uint32_t t = 0; t < glen; ++t) {
               const unsigned long long g = Lv[p + t];
               const uint32_t hg = label_hop(g);
-              if (hg > h1) {
+              if (hg >= h1) {
                 const uint32_t m = vals[base + (hg - h1)];
                 if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
               }
@@ -518,6 +524,9 @@ __global__ void __launch_bounds__(256) k_clean_seq(uint32_t K, const unsigned lo
           }
         }
       }
+      // L(u)'s own labels (hub u): found by search, since a list may also
+      // hold hubs ranked below its vertex (imported label sets)
+      const uint32_t ub0 = lb_hub(Lu, 0, ulen, u);
       uint32_t M[kMaxK + 1];
       uint32_t run = kInf32;
 #pragma unroll
@@ -540,8 +549,8 @@ __global__ void __launch_bounds__(256) k_clean_seq(uint32_t K, const unsigned lo
           if (!((newbits >> (k2 - gb)) & 1u)) continue;
           const unsigned long long q = Lv[k2];
           const uint32_t h2 = label_hop(q), d2 = label_dist(q);
-          for (uint32_t t = ulen; t > 0; --t) {
-            const unsigned long long r = Lu[t - 1];
+          for (uint32_t t = ub0; t < ulen; ++t) {
+            const unsigned long long r = Lu[t];
             if (label_hub(r) != u) break;
             if (label_hop(r) + h2 <= h) best = min(best, label_dist(r) + d2);
           }

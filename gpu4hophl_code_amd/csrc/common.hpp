@@ -28,6 +28,10 @@ struct hsdl_graph {
   // the pool grew to)
   void *d_pool = nullptr, *d_pool32 = nullptr;
   size_t d_pool_bytes = 0, d_pool32_bytes = 0;
+  // list and tile-arena sizes the last build on this graph grew to: the next
+  // build starts there instead of rolling batches back to grow them again
+  uint32_t hint_ccap = 0, hint_fcap = 0;
+  uint64_t hint_arena = 0;
   int d_device = -1;
   void (*d_free)(hsdl_graph *) = nullptr;
   ~hsdl_graph() {

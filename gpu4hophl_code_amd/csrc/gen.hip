@@ -1908,8 +1908,10 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   uint64_t pool0 = opts->pool_init_labels
                              ? std::max<uint64_t>(opts->pool_init_labels, (uint64_t)n * c0)
                              : std::max<uint64_t>((uint64_t)n * c0 * 4, 64ull << 20);
-  const uint64_t gtile0 = opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
-                                                 : (uint64_t)g.grid * 16384;
+  // the last build's arena size on this graph, unless a size is forced
+  const uint64_t gtile0 =
+      opts->arena_init_words ? std::max<uint64_t>(opts->arena_init_words, g.grid)
+                             : std::max<uint64_t>((uint64_t)g.grid * 16384, hg->hint_arena);
   // first candidate / frontier list size; HSDL_LIST_INIT starts them small
   // (the lists grow by rolling the batch back, like the pool)
   uint32_t list0 = env_u32(
@@ -1936,6 +1938,13 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // tuning knob HSDL_GRAPH=0: launch every kernel of a batch separately
   g.use_graph = env_u32("HSDL_GRAPH", 1, 0, 1) != 0;
   list0 = std::max(list0, Bmax);  // a batch's first frontier (its hubs) must fit
+  // the lists the last build on this graph grew to (HSDL_LIST_INIT forces a
+  // start size instead); each list pair's bytes are checked with the budget
+  uint32_t ccap0 = list0, fcap0 = list0;
+  if (!getenv("HSDL_LIST_INIT")) {
+    ccap0 = std::max(ccap0, hg->hint_ccap);
+    fcap0 = std::max(fcap0, hg->hint_fcap);
+  }
   // heavy pieces per round (24 MB); when a round has more, the waves keep
   // the items that did not fit (HSDL_HEAVY_CAP lowers the cap)
   const uint32_t heavy_cap = env_u32("HSDL_HEAVY_CAP", 1u << 20, 1, 1u << 20);
@@ -2102,7 +2111,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.ctrl = g.ctrl.as<Ctrl>();
   a.heavy = g.heavy.as<HeavyItem>();
   a.heavy_cap = heavy_cap;
-  if ((rc = alloc_lists(g, list0, list0))) return rc;
+  rc = alloc_lists(g, ccap0, fcap0);
+  if (rc && (ccap0 != list0 || fcap0 != list0)) rc = alloc_lists(g, list0, list0);
+  if (rc) return rc;
 
   {
     Ctrl c0v{};
@@ -2292,6 +2303,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
             (t_gen - t_csr) * 1e-9, (unsigned long long)batches, (unsigned long long)redos,
             g.h_enqueue, g.h_wait, g.h_grow, (unsigned long long)g.graph_captures,
             (t_end - t_gen) * 1e-9, (t_end - t_sort0) * 1e-9, (unsigned long long)total);
+  mg->hint_ccap = g.a.ccap;
+  mg->hint_fcap = g.a.fcap;
+  mg->hint_arena = g.a.gtile_cap;
   idx->stats.labels_uncleaned = total;
   idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
   idx->stats.t_sort_s = (t_sorted - t_sort0) * 1e-9;

@@ -428,3 +428,43 @@ def test_clean_hop_sorted_stream(monkeypatch, wide32, hop_sort):
     idx = H.Index.from_labels(gc.n, 8, off.astype(np.uint64), hub, hop, dist)
     idx.clean()
     same_labels(idx, O.clean(gred).arrays())
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_clean_of_imported_labels_with_hubs_below_their_vertex(seed):
+    """Imported label sets may hold hubs ranked below their vertex (the
+    reference's multi-threaded 2023WWW generation makes them).  Start from
+    the K-round labels of R-MAT scale 9 and add, for random labels (y,h,d) of
+    L(s) with y < s, the mirror label (s,h,d) to L(y), and some with a larger
+    distance: every added label is redundant, and lists then end in hubs
+    above their vertex, which the clean's own-hub lookup in L(u) must pass
+    over.  hsdl_clean equals the normative clean reading its input L(u)."""
+    g = H.Graph.rmat(9, 8, 5, 1, 20)
+    c = csr_of(g)
+    K = 4
+    off, hub, hop, dist = O.kround(c, K).arrays()
+    rng = np.random.default_rng(seed)
+    lists = [[(int(hub[i]), int(hop[i]), int(dist[i])) for i in range(off[v], off[v + 1])]
+             for v in range(c.n)]
+    added = 0
+    for s in range(c.n):
+        for (y, h, d) in list(lists[s]):
+            if y < s and rng.random() < 0.3:
+                lists[y].append((s, h, d + int(rng.integers(0, 3))))
+                added += 1
+    assert added > 100
+    lists = [sorted(set(l)) for l in lists]
+    # one label per (hub, hop): keep the smallest distance
+    lists = [[x for i, x in enumerate(l) if i == 0 or l[i - 1][:2] != x[:2]] for l in lists]
+    assert any(l and l[-1][0] > v for v, l in enumerate(lists))
+    noff = np.zeros(c.n + 1, np.uint64)
+    noff[1:] = np.cumsum([len(l) for l in lists])
+    flat = [x for l in lists for x in l]
+    nh = np.array([x[0] for x in flat], np.uint32)
+    nhop = np.array([x[1] for x in flat], np.uint8)
+    nd = np.array([x[2] for x in flat], np.int32)
+    idx = H.Index.from_labels(c.n, K, noff, nh, nhop, nd)
+    idx.clean()
+    want = O.clean(O.labels_from_arrays(c.n, noff.astype(np.int64), nh, nhop, nd), lu_mode=1)
+    same_labels(idx, want.arrays())
+    assert want.total() == off[-1]
