@@ -136,6 +136,7 @@ struct GenArgs {
   uint32_t row_density;  // "dense": candidates >= batch slots / row_density
   uint32_t row_run;      // y-blocks per row item (one tile build per item)
   uint32_t row_block;    // vertices per y-block (a multiple of 1024)
+  uint32_t claim_max;    // candidates a prune wave claims at once, at most
   uint32_t heavy_pairs;  // frontier groups with more pairs go to heavy pieces
   uint32_t piece_pairs;  // pairs per heavy piece (whole edges)
   uint32_t lds_slots;  // LDS tile capacity (slots)
@@ -1003,12 +1004,11 @@ __device__ inline void publish_accepted(const GenArgs &a, Item *Fn, Item *sh_acc
 // Candidates a prune wave claims at once, at most, and the divisor that sizes
 // a claim from the item's unclaimed candidates (claims shrink toward the
 // item's end, where the slowest wave decides when the workgroup moves on).
-constexpr uint32_t kClaimMax = 4;
 constexpr uint32_t kClaimSpread = 2 * (kPruneThreads / kWave);
 
 // The waves of a prune workgroup take the item's candidates [c_beg, c_end) of
 // hub slot b (L(y) lengths vary widely, so dynamically) and stage the
-// accepted ones in LDS.  A wave claims up to kClaimMax candidates at once and
+// accepted ones in LDS.  A wave claims up to a.claim_max candidates at once and
 // loads their vertices, then their slot distances and list extents, in one
 // round trip per level (lane i: claimed candidate i); the first 64 labels of
 // the next claimed candidate's list are loaded before the current one's probes.
@@ -1025,7 +1025,7 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
       const uint32_t seen =
           __hip_atomic_load(&sh_next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       const uint32_t rem = c_end - min(c_end, c_beg + seen);
-      k = min(kClaimMax, max(1u, rem / kClaimSpread));
+      k = min(a.claim_max, max(1u, rem / kClaimSpread));
       c = c_beg + atomicAdd(&sh_next, k);
     }
     c = __builtin_amdgcn_readfirstlane(__shfl(c, 0, kWave));
@@ -1440,22 +1440,75 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
 // they are: the query reads their append zone whole, which is always
 // correct.  Runs between batches (never while a batch can still be rolled
 // back, whose labels must stay at the tail of every list).
-__global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long *tmp,
-                                                  uint32_t *tmp32, uint32_t per_wave,
-                                                  uint32_t alpha_div) {
+// One list's stable scatter by hop into (t64, t32) (pos[x]: the first slot of
+// hop x), then back over the segment (whole wave).  T64 / T32: LDS or
+// global pointers.
+template <class T64, class T32>
+__device__ inline void hop_scatter(const GenArgs &a, unsigned long long st, uint32_t len,
+                                   uint32_t *pos, T64 t64, T32 t32, bool p32) {
   const uint32_t lane = (uint32_t)lane_id();
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
+  const uint32_t K1 = a.K + 1;
+  for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
+    const uint32_t j = j0 + lane;
+    unsigned long long l = 0;
+    uint32_t l32 = 0, hop = kInf32;
+    if (j < len) {
+      l = a.pool[st + j];
+      if (p32) l32 = a.pool32[st + j];
+      hop = label_hop(l);
+    }
+    uint32_t dst = 0;
+#pragma unroll
+    for (uint32_t x = 0; x <= kMaxK; ++x) {
+      if (x >= K1) continue;
+      const unsigned long long m = __ballot(hop == x);
+      if (hop == x) dst = pos[x] + (uint32_t)__popcll(m & lt_mask);
+      pos[x] += (uint32_t)__popcll(m);
+    }
+    if (j < len) {
+      t64[dst] = l;
+      if (p32) t32[dst] = l32;
+    }
+  }
+  // the scratch was written by other lanes of this wave
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  for (uint32_t j = lane; j < len; j += kWave) {
+    a.pool[st + j] = t64[j];
+    if (p32) a.pool32[st + j] = t32[j];
+  }
+  // the next list reuses the scratch
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Lists of up to lds_labels labels are scattered in the wave's LDS slice
+// (one read and one write of each label in HBM), longer ones in its slice of
+// `tmp` in HBM (two of each).
+__global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long *tmp,
+                                                  uint32_t *tmp32, uint32_t per_wave,
+                                                  uint32_t alpha_div, uint32_t lds_labels) {
+  extern __shared__ unsigned long long zl64[];
+  const uint32_t lane = (uint32_t)lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  const uint32_t wib = threadIdx.x / kWave;  // wave in block
+  const bool p32 = tmp32 != nullptr;
   unsigned long long *t64 = tmp + (uint64_t)wave * per_wave;
-  uint32_t *t32 = tmp32 ? tmp32 + (uint64_t)wave * per_wave : nullptr;
+  uint32_t *t32 = p32 ? tmp32 + (uint64_t)wave * per_wave : nullptr;
+  // LDS: 4 slices of lds_labels u64, then 4 of lds_labels u32
+  lds_ptr<unsigned long long> l64 = (lds_ptr<unsigned long long>)(zl64 + wib * lds_labels);
+  lds_ptr<uint32_t> l32 = (lds_ptr<uint32_t>)(reinterpret_cast<uint32_t *>(zl64 + 4 * lds_labels) +
+                                              wib * lds_labels);
   const uint32_t K1 = a.K + 1;
   for (uint64_t base = (uint64_t)wave * kWave; base < a.n; base += (uint64_t)nwaves * kWave) {
     const uint32_t v = (uint32_t)base + lane;
     bool need = false;
     if (v < a.n) {
       const uint32_t len = a.seg_len[v], sorted = a.sorted_len[v];
-      need = len > sorted && (uint64_t)(len - sorted) * alpha_div > sorted && len <= per_wave;
+      need = len > sorted && (uint64_t)(len - sorted) * alpha_div > sorted &&
+             (len <= per_wave || len <= lds_labels);
     }
     unsigned long long todo = __ballot(need);
     while (todo) {
@@ -1464,18 +1517,27 @@ __global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long 
       const uint32_t y = (uint32_t)base + i;
       const unsigned long long st = a.seg_start[y];
       const uint32_t len = a.seg_len[y];
-      // 1. labels per hop (wave-uniform counters)
+      // 1. labels per hop (wave-uniform counters), from the 32-bit copy
+      // when there is one
       uint32_t cnt[kMaxK + 1];
 #pragma unroll
       for (uint32_t x = 0; x <= kMaxK; ++x) cnt[x] = 0;
       for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
         const uint32_t j = j0 + lane;
-        const uint32_t hop = j < len ? label_hop(a.pool[st + j]) : kInf32;
+        uint32_t hop = kInf32;
+        if (j < len) {
+          if (p32) {
+            uint32_t hub, d;
+            unpack(a, a.pool32[st + j], hub, hop, d);
+          } else {
+            hop = label_hop(a.pool[st + j]);
+          }
+        }
 #pragma unroll
         for (uint32_t x = 0; x <= kMaxK; ++x)
           if (x < K1) cnt[x] += (uint32_t)__popcll(__ballot(hop == x));
       }
-      // 2. stable scatter by hop into the wave's slice
+      // 2. stable scatter by hop, back over the segment
       uint32_t pos[kMaxK + 1];
       uint32_t run = 0;
 #pragma unroll
@@ -1483,36 +1545,11 @@ __global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long 
         pos[x] = run;
         run += x < K1 ? cnt[x] : 0u;
       }
-      for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
-        const uint32_t j = j0 + lane;
-        unsigned long long l = 0;
-        uint32_t l32 = 0, hop = kInf32;
-        if (j < len) {
-          l = a.pool[st + j];
-          if (t32) l32 = a.pool32[st + j];
-          hop = label_hop(l);
-        }
-        uint32_t dst = 0;
-#pragma unroll
-        for (uint32_t x = 0; x <= kMaxK; ++x) {
-          if (x >= K1) continue;
-          const unsigned long long m = __ballot(hop == x);
-          if (hop == x) dst = pos[x] + (uint32_t)__popcll(m & lt_mask);
-          pos[x] += (uint32_t)__popcll(m);
-        }
-        if (j < len) {
-          t64[dst] = l;
-          if (t32) t32[dst] = l32;
-        }
-      }
-      // the slice was written by other lanes of this wave
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-      // 3. back over the segment, then the zone bounds
-      for (uint32_t j = lane; j < len; j += kWave) {
-        a.pool[st + j] = t64[j];
-        if (t32) a.pool32[st + j] = t32[j];
-      }
+      if (len <= lds_labels)
+        hop_scatter(a, st, len, pos, l64, l32, p32);
+      else
+        hop_scatter(a, st, len, pos, (glb_ptr<unsigned long long>)t64, (glb_ptr<uint32_t>)t32,
+                    p32);
       if (lane == 0) {
         uint32_t c = cnt[0];
         for (uint32_t x = 1; x < K1 - 1; ++x) {
@@ -1521,9 +1558,6 @@ __global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long 
         }
         a.sorted_len[y] = len;
       }
-      // the next list reuses the slice
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     }
   }
 }
@@ -1830,10 +1864,14 @@ static int sort_zones(Gen &g, uint32_t maxlen, uint32_t resort_div) {
       g.sort_per_wave = per;
     }
   }
-  hipLaunchKernelGGL(k_hop_sort, dim3(g.sort_grid), dim3(256), 0, g.st, g.a,
-                     g.sort_tmp.as<unsigned long long>(),
+  // tuning knob HSDL_SORT_LDS: labels per wave's LDS slice (12 B each,
+  // four waves per workgroup, at most 64 KB; R-MAT scale 20 builds: 0 4.252
+  // s, 1024 4.249 s, 1360 4.231 s)
+  const uint32_t lds_labels = env_u32("HSDL_SORT_LDS", 1360, 0, 1365);
+  hipLaunchKernelGGL(k_hop_sort, dim3(g.sort_grid), dim3(256), (size_t)lds_labels * 48, g.st,
+                     g.a, g.sort_tmp.as<unsigned long long>(),
                      g.a.pool32 ? g.sort_tmp32.as<uint32_t>() : nullptr, g.sort_per_wave,
-                     resort_div);
+                     resort_div, lds_labels);
   HSDL_HIP(hipGetLastError());
   return HSDL_OK;
 }
@@ -2060,6 +2098,10 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // 0.66, 128 / 1024 0.13 / 1.44 (every 64 groups of a wave wait for its
   // largest, so big light groups idle lanes; small pieces cost a claim each)
   a.heavy_pairs = env_u32("HSDL_HEAVY_PAIRS", 256, 1, 1u << 16);
+  // tuning knob HSDL_CLAIM_MAX: candidates a prune wave claims at once
+  // (R-MAT scale 20 builds: 2 4.33 s, 4 4.27 s, 8 4.25 s, 16 4.24 s; scale
+  // 18 the same within 1 %)
+  a.claim_max = env_u32("HSDL_CLAIM_MAX", 8, 1, 64);
   a.piece_pairs = env_u32("HSDL_PIECE_PAIRS", 8192, 1, 1u << 16);
   a.lds_slots = lds_slots;
   DevBuf hubs_buf, in_group;

@@ -139,6 +139,19 @@ def test_batch_graphs_replay_exactly(monkeypatch, heavy13, ragged_grid, graph):
     same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=300), want)
 
 
+@pytest.mark.parametrize("claim", ["1", "64"])
+def test_prune_claim_sizes(monkeypatch, heavy13, ragged_grid, claim):
+    """A prune wave claims 1 or 64 candidates at once (default 8), with row
+    items and list items: the same labels."""
+    monkeypatch.setenv("HSDL_CLAIM_MAX", claim)
+    g, want = heavy13
+    for rows in ("1", "2", "0"):
+        monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+        same_labels(H.hop_constrained_two_hop_labels_generation(g, 4), want)
+    gg, gwant = ragged_grid
+    same_labels(H.hop_constrained_two_hop_labels_generation(gg, 6, hub_batch=300), gwant)
+
+
 def test_hbm_budget_failures_are_clean(golden_dir):
     """A build under a device budget too small for it returns HSDL_ENOMEM
     (at set-up, or when the label pool, arena or lists must grow past it)
