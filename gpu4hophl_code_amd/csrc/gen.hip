@@ -264,7 +264,7 @@ __device__ inline uint32_t pack32(const GenArgs &a, uint32_t hub, uint32_t hop, 
 __device__ inline void unpack(const GenArgs &a, uint32_t l, uint32_t &hub, uint32_t &hop,
                               uint32_t &dist) {
   hub = l >> (a.p32_hop_bits + a.p32_dist_bits);
-  hop = (l >> a.p32_dist_bits) & ((1u << a.p32_hop_bits) - 1u);
+  hop = __builtin_amdgcn_ubfe(l, a.p32_dist_bits, a.p32_hop_bits);
   dist = l & ((1u << a.p32_dist_bits) - 1u);
 }
 __device__ inline void unpack(const GenArgs &, unsigned long long l, uint32_t &hub,
@@ -875,19 +875,37 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, u
 // waits about one memory round trip per 128 labels, not per 64.  The tile is
 // read through address-space pointers (KP / VP, device.hpp), so a probe's
 // wait does not also wait for the next step's loads.
+//
+// The probe is straight-line code but for the rare collision chain, since
+// the prune issues VALU instructions near the rate a CU sustains:
+// * lanes past the end of the list hold a copy of its last label, which
+//   gives the same answer as the label itself, so they need no mask;
+// * a label of hop ha > h reads the hop-budget-0 value, which only the
+//   tile's own hub s has (its self label) and which no (s, ha, .) of L(y)
+//   can reach: hub s labels L(y) in rounds 1..h-1 of this batch only;
+// * the chain ends at the hub or at an empty slot, whose values are empty,
+//   so the value read needs no select: a 16-bit empty value (0xFFFF) is
+//   above every candidate distance (small values: K * max weight < 0xFFFF),
+//   and a 32-bit one is tested.
 template <class V, class KP, class VP, class LT>
-__device__ inline bool probe_label(const GenArgs &a, LT l, bool valid, uint32_t dy, KP keys,
-                                   VP vals, uint32_t cap, uint32_t K1) {
+__device__ inline bool probe_label(const GenArgs &a, LT l, uint32_t dy, KP keys, VP vals,
+                                   uint32_t cap, uint32_t K1) {
   uint32_t hub, ha, da;
   unpack(a, l, hub, ha, da);
-  if (!valid || ha > a.h) return false;
-  const uint32_t v = tile_lookup<V>(keys, vals, cap, K1, hub, a.h - ha);
+  const uint32_t x = a.h > ha ? a.h - ha : 0u;
+  uint32_t slot = hash_slot(hub, cap);
+  uint32_t k = keys[slot];
+  while (k != hub && k != kInf32) {
+    slot = (slot + 1) & (cap - 1);
+    k = keys[slot];
+  }
+  const uint32_t v = vals[mul_u24(slot, K1) + x];
+  if (sizeof(V) == 2) return da + v <= dy;
   return v != TileVal<V>::kEmpty && da + v <= dy;
 }
 
 // Label `lane` of a list view, clamped to it (a lane past the end reloads
-// the last label and is masked in the probe); an empty view reads the list's
-// position 0, its self label.
+// the last label); an empty view reads the list's position 0, its self label.
 template <class LT>
 __device__ inline LT first_step(const LT *L, const ListView &v) {
   const uint32_t lane = (uint32_t)lane_id();
@@ -917,13 +935,13 @@ __device__ bool witness_search(const GenArgs &a, const LT *L, const ListView &vi
   const uint32_t last = len - 1;
   auto ld = [&](uint32_t j) { return L[view.at(min(j, last))]; };
   auto probe2 = [&](LT l0, LT l1, uint32_t j) {
-    const bool h0 = probe_label<V>(a, l0, j + lane < len, dy, keys, vals, cap, K1);
-    const bool h1 = probe_label<V>(a, l1, j + kWave + lane < len, dy, keys, vals, cap, K1);
+    const bool h0 = probe_label<V>(a, l0, dy, keys, vals, cap, K1);
+    const bool h1 = probe_label<V>(a, l1, dy, keys, vals, cap, K1);
     return __ballot(h0 || h1) != 0;
   };
   constexpr uint32_t kStep = 2 * kWave;
-  if (__ballot(probe_label<V>(a, f0, lane < len, dy, keys, vals, cap, K1))) {
-    reads = min(len, kWave);
+  if (__ballot(probe_label<V>(a, f0, dy, keys, vals, cap, K1))) {
+    reads = min(len, (uint32_t)kWave);
     return true;
   }
   if (len <= kWave) {
@@ -1032,37 +1050,47 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, kWave));
     if (c >= c_end) break;
     const uint32_t cn = min(k, c_end - c);
-    uint32_t y = 0, dy = 0, len = 0, sorted = 0, pre = 1;
+    // lane i: claimed candidate i's vertex, distance, list start and view
+    // (computed lane-parallel once; a candidate's values are read out into
+    // scalars once, when it is next)
+    uint32_t y = 0, dy = 0;
+    ListView vw{0, 0, 0, 0};
     unsigned long long st = 0;
     if (lane < cn) {
       y = cands[c + lane];
       dy = slot_dist(a, a.cand_d[slot_at(a, b, y)]);
       st = a.seg_start[y];
-      len = a.seg_len[y];
+      const uint32_t len = a.seg_len[y];
+      uint32_t sorted = 0, pre = 1;
       if (a.hcnt && a.h > 1) {
         sorted = a.sorted_len[y];
         pre = hop_prefix(a, y, a.h - 1);
       }
+      vw = cand_view(a, len, sorted, pre);
     }
-    auto start_of = [&](uint32_t i) {
+    const LT *pool = stream_pool<LT>(a);
+    auto list_of = [&](uint32_t i) {
       const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)st, i);
       const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(st >> 32), i);
-      return stream_pool<LT>(a) + (((unsigned long long)hi << 32) | lo);
+      return pool + (((unsigned long long)hi << 32) | lo);
     };
+    auto rl = [](uint32_t v, uint32_t i) { return (uint32_t)__builtin_amdgcn_readlane(v, i); };
     auto view_of = [&](uint32_t i) {
-      return cand_view(a, __builtin_amdgcn_readlane(len, i), __builtin_amdgcn_readlane(sorted, i),
-                       __builtin_amdgcn_readlane(pre, i));
+      return ListView{rl(vw.a0, i), rl(vw.na, i), rl(vw.b0, i), rl(vw.len, i)};
     };
-    LT f0 = first_step(start_of(0), view_of(0));
+    const LT *Lc = list_of(0);
+    ListView vc = view_of(0);
+    LT f0 = first_step(Lc, vc);
     for (uint32_t i = 0; i < cn; ++i) {
       // next candidate's first step (the last one reloads its own, so the
       // loads are unconditional)
       const uint32_t nx = min(i + 1, cn - 1);
-      const LT g0 = first_step(start_of(nx), view_of(nx));
+      const LT *Ln = list_of(nx);
+      const ListView vn = view_of(nx);
+      const LT g0 = first_step(Ln, vn);
       const uint32_t yi = __builtin_amdgcn_readlane(y, i), di = __builtin_amdgcn_readlane(dy, i);
       uint32_t reads = 0;
-      const bool pruned = witness_search<V>(a, start_of(i), view_of(i), f0, di, keys, vals, cap,
-                                            reads);
+      const bool pruned = witness_search<V>(a, Lc, vc, f0, di, keys, vals, cap, reads);
       wave_reads += reads;
       if (lane == 0) {
         if (!pruned) {
@@ -1070,6 +1098,8 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
           sh_acc[p] = Item{b, yi, di};
         }
       }
+      Lc = Ln;
+      vc = vn;
       f0 = g0;
     }
   }
