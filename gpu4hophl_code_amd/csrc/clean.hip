@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "device.hpp"
@@ -68,18 +69,6 @@ __device__ inline uint32_t lb_hub(const unsigned long long *lab, uint32_t lo, ui
 }
 
 // ---------------------------------------------------------------- pass 1
-// Slot of `hub` in a tile, or kInf32 when absent.
-template <class KP>
-__device__ inline uint32_t tile_slot(KP keys, uint32_t cap, uint32_t hub) {
-  uint32_t slot = hash_slot(hub, cap);
-  uint32_t k = keys[slot];
-  while (k != hub && k != kInf32) {
-    slot = (slot + 1) & (cap - 1);
-    k = keys[slot];
-  }
-  return k == hub ? slot : kInf32;
-}
-
 // The heads' lists L(u) are streamed from the index's 8-byte labels, or from
 // a 32-bit copy of them (hub | hop | dist fields) when the fields fit one
 // word: the first pass reads far more list labels than anything else and
@@ -116,7 +105,7 @@ struct Stream32 {
   }
   __device__ void dec(T l, uint32_t &w, uint32_t &h, uint32_t &d) const {
     w = l >> (hop_bits + dist_bits);
-    h = (l >> dist_bits) & ((1u << hop_bits) - 1u);
+    h = __builtin_amdgcn_ubfe(l, dist_bits, hop_bits);
     d = l & ((1u << dist_bits) - 1u);
   }
 };
@@ -201,6 +190,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
                                const S &str, const unsigned long long *__restrict__ Lv, uint32_t len,
                                uint32_t p0, uint32_t win, KP keys, VP vals, uint32_t cap,
                                uint64_t &streamed) {
+  constexpr bool kArena = !std::is_same<KP, lds_ptr<uint32_t>>::value;
   const uint32_t lane = (uint32_t)lane_id();
   bool hit = false;
   const uint32_t pend = min(p0 + win, len);
@@ -236,40 +226,24 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     }
     streamed += ulen;
     const uint32_t last = ulen - 1;
-    // one element (w,h1,d1) of L(u) at position j against the group
-    auto check = [&](typename S::T l, uint32_t j) {
+    // one element (w,h1,d1) of L(u) against the group.  The pass issues
+    // VALU instructions near the rate a CU sustains, so the check is
+    // straight-line code but for the rare collision chain:
+    // * a lane past the list's end holds a copy of its last element, which
+    //   gives the same answer as the element itself: no mask;
+    // * in an LDS tile the lookup runs for every lane: the chain ends at w
+    //   or at an empty slot (whose values count only when k == w), and the
+    //   shortcuts d1 <= maxd and h1 + hmin <= maxh are left to the pair test,
+    //   which implies them (d1 + m <= gd with m >= 0; budget gh - h1 = 0
+    //   holds only v's self label, hop-0 labels being self labels); an arena
+    //   tile's lookups are global loads, which the shortcuts still save;
+    // * a 16-bit empty value (0xFFFF) is above every label distance (small
+    //   values: the largest distance < 0xFFFF), so only 32-bit values test
+    //   for it.
+    auto check = [&](typename S::T l) {
       uint32_t w, h1, d1;
-      str.dec(l, w, h1, d1);  // a lane past the list's end holds a copy of its last element
-      const bool in = j < ulen;
-      // a pair through w < u has hop >= h1 + 1 (w != v) and dist >= d1; the
-      // conditions are combined into one branch (one exec-mask change)
-      if (in & (w < u) & (h1 + hmin <= maxh) & (d1 <= maxd)) {
-        const uint32_t slot = tile_slot(keys, cap, w);
-        if (slot != kInf32) {
-          const uint32_t base = mul_u24(slot, K1);
-          if (fast) {
-#pragma unroll
-            for (uint32_t t = 0; t < kGroupFast; ++t) {
-              if (t >= glen) break;
-              // a label shallower than h1 reads the slot's first value and
-              // is masked; at equal hops only a hop-0 tile label pairs
-              const bool reach = gh[t] >= h1;
-              const uint32_t m = vals[base + (reach ? gh[t] - h1 : 0u)];
-              hit |= reach & (m != TileVal<V>::kEmpty) & (d1 + m <= gd[t]);
-            }
-          } else {
-            for (uint32_t t = 0; t < glen; ++t) {
-              const unsigned long long g = Lv[p + t];
-              const uint32_t hg = label_hop(g);
-              if (hg >= h1) {
-                const uint32_t m = vals[base + (hg - h1)];
-                if (m != TileVal<V>::kEmpty && d1 + m <= label_dist(g)) hit = true;
-              }
-            }
-          }
-        }
-      }
-      if (in & (w == u)) {
+      str.dec(l, w, h1, d1);
+      if (w == u) {
         // through hub u: earlier labels of the group against later ones
         for (uint32_t t1 = 0; t1 < glen; ++t1) {
           const unsigned long long g1 = Lv[p + t1];
@@ -280,18 +254,48 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
           }
         }
       }
+      if (kArena && !((w < u) & (h1 + hmin <= maxh) & (d1 <= maxd))) return;
+      uint32_t slot = hash_slot(w, cap);
+      uint32_t k = keys[slot];
+      while (k != w && k != kInf32) {
+        slot = (slot + 1) & (cap - 1);
+        k = keys[slot];
+      }
+      const bool pair = (k == w) & (w < u);
+      const uint32_t base = mul_u24(slot, K1);
+      auto fits = [&](uint32_t m, uint32_t gd) -> bool {
+        if (sizeof(V) == 2) return d1 + m <= gd;
+        return (m != TileVal<V>::kEmpty) & (d1 + m <= gd);
+      };
+      if (fast) {
+#pragma unroll
+        for (uint32_t t = 0; t < kGroupFast; ++t) {
+          if (t >= glen) break;
+          // a label shallower than h1 reads the slot's first value and is
+          // masked; at equal hops only the hop-0 tile label (v's) pairs
+          const bool reach = gh[t] >= h1;
+          const uint32_t m = vals[base + (reach ? gh[t] - h1 : 0u)];
+          hit |= pair & reach & fits(m, gd[t]);
+        }
+      } else if (pair) {
+        for (uint32_t t = 0; t < glen; ++t) {
+          const unsigned long long g = Lv[p + t];
+          const uint32_t hg = label_hop(g);
+          if (hg >= h1 && fits(vals[base + (hg - h1)], label_dist(g))) hit = true;
+        }
+      }
     };
     auto ld = [&](uint32_t j) { return Lu[min(j, last)]; };
     typename S::T a0 = ld(lane), a1 = ld(kWave + lane);
     for (uint32_t jb = 0; jb < ulen; jb += 4 * kWave) {
       const typename S::T b0 = ld(jb + 2 * kWave + lane), b1 = ld(jb + 3 * kWave + lane);
-      check(a0, jb + lane);
-      check(a1, jb + kWave + lane);
+      check(a0);
+      check(a1);
       if (jb + 2 * kWave >= ulen) break;
       a0 = ld(jb + 4 * kWave + lane);
       a1 = ld(jb + 5 * kWave + lane);
-      check(b0, jb + 2 * kWave + lane);
-      check(b1, jb + 3 * kWave + lane);
+      check(b0);
+      check(b1);
     }
   }
   return __ballot(hit) != 0;
