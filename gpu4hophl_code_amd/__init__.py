@@ -69,7 +69,7 @@ class hsdl_stats(ctypes.Structure):
                 ("prune_label_reads", ctypes.c_uint64), ("prune_tile_reads", ctypes.c_uint64),
                 ("clean_seq_vertices", ctypes.c_uint64), ("clean_label_reads", ctypes.c_uint64),
                 ("t_parents_s", ctypes.c_double), ("prune_label_bytes", ctypes.c_uint32),
-                ("clean_label_bytes", ctypes.c_uint32)]
+                ("clean_label_bytes", ctypes.c_uint32), ("prune_row_slots", ctypes.c_uint64)]
 
 
 class hsdl_server_opts(ctypes.Structure):

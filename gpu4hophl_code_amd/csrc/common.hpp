@@ -23,6 +23,13 @@ struct hsdl_graph {
   // allocations that followed)
   void *d_slots = nullptr;
   size_t d_slots_bytes = 0;
+  // what d_slots hold after the last build that ended normally: the slot
+  // encoding they were written with (0: unknown, clear them first), the next
+  // attempt tag of the count-down, the hub rows cleared at the last full
+  // clear (rows above it may hold values of any tag) and the rows used since
+  // the last clear
+  uint64_t slot_key = 0, slot_tag_next = 0;
+  uint32_t slot_rows_valid = 0, slot_rows_used = 0;
   // the last build's label pool and its 32-bit copy, kept for the next build
   // on this graph for the same reason (and so that build starts at the size
   // the pool grew to)

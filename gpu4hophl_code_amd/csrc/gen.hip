@@ -96,6 +96,7 @@ struct Ctrl {
   unsigned long long edge_total;
   unsigned long long label_reads;  // labels of L(y) streamed by the prune
   unsigned long long tile_reads;   // labels of L(s) staged into tiles
+  unsigned long long row_slots;    // dense slots row items scanned
   uint32_t fcnt[2];
   uint32_t ccnt;  // flat candidates this round
   uint32_t touched_cnt;
@@ -1237,7 +1238,7 @@ k_prune_rows(GenArgs a) {
   const uint32_t K1 = a.K + 1;
   Item *Fn = frontier(a, a.cur ^ 1);
   uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
-  unsigned long long wave_reads = 0, tile_reads = 0;
+  unsigned long long wave_reads = 0, tile_reads = 0, row_slots = 0;
   const uint64_t gwords = a.gtile_cap / gridDim.x;
   uint32_t *gkeys = a.gtile + (uint64_t)blockIdx.x * gwords;
   const uint32_t nblk = (uint32_t)(((uint64_t)a.n + a.row_block - 1) / a.row_block);
@@ -1295,6 +1296,7 @@ k_prune_rows(GenArgs a) {
       const uint32_t y0 = kb * a.row_block + (i % kSubs) * kPruneThreads;
       const uint32_t y_end = min(a.n, (kb + 1) * a.row_block);
       if (y0 >= y_end) continue;
+      row_slots += min(kPruneThreads, y_end - y0);
       // this hub's candidates among [y0, y0 + 1024), compacted into sh_cand
       if (threadIdx.x == 0) sh_cnt = 0;
       __syncthreads();
@@ -1348,6 +1350,7 @@ k_prune_rows(GenArgs a) {
   publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
   if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
   if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
+  if (threadIdx.x == 0 && row_slots) atomicAdd(&a.ctrl->row_slots, row_slots);
 }
 
 // -------------------------------------------------------- reserve / append
@@ -1616,6 +1619,7 @@ static void free_device_graph(hsdl_graph *g) {
   if (cur >= 0 && cur != g->d_device) (void)hipSetDevice(cur);
   g->d_off = g->d_adj = g->d_slots = g->d_pool = g->d_pool32 = nullptr;
   g->d_slots_bytes = g->d_pool_bytes = g->d_pool32_bytes = 0;
+  g->slot_key = 0;
   g->d_device = -1;
 }
 
@@ -1997,11 +2001,16 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
   if (Bmax == 0) return fail(HSDL_ENOMEM, "HBM budget below one hub's dedupe slots");
   // slot windows of 2^slot_lw hub slots (tuning knob HSDL_SLOT_WINDOW, log2,
-  // default 4: 16 slots, one 64-byte sector), no wider than the batch cap,
-  // whose slots then fill whole windows
+  // default 4: 16 slots, one 64-byte sector), whose slots the batch cap then
+  // fills whole: a cap of the budget's is rounded down to whole windows, a
+  // caller's hub_batch is kept and the windows narrowed until they divide it
   uint32_t slot_lw = env_u32("HSDL_SLOT_WINDOW", 4, 0, 8);
-  while (slot_lw && (1u << slot_lw) > Bmax) --slot_lw;
-  Bmax &= ~((1u << slot_lw) - 1u);
+  if (opts->hub_batch) {
+    while (slot_lw && (Bmax & ((1u << slot_lw) - 1u))) --slot_lw;
+  } else {
+    while (slot_lw && (1u << slot_lw) > Bmax) --slot_lw;
+    Bmax &= ~((1u << slot_lw) - 1u);
+  }
   g.bmax = Bmax;
   // tuning knob HSDL_GRAPH=0: launch every kernel of a batch separately
   g.use_graph = env_u32("HSDL_GRAPH", 1, 0, 1) != 0;
@@ -2075,6 +2084,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       if (mg->d_slots) (void)hipFree(mg->d_slots);
       mg->d_slots = nullptr;
       mg->d_slots_bytes = 0;
+      mg->slot_key = 0;
       hipError_t e = hipMalloc(&mg->d_slots, need);
       if (e != hipSuccess) {
         mg->d_slots = nullptr;
@@ -2193,8 +2203,20 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
     HSDL_HIP(hipMemcpyAsync(g.ctrl.p, &c0v, sizeof(Ctrl), hipMemcpyHostToDevice, g.st));
   }
   hipLaunchKernelGGL(k_init_labels, dim3(1024), dim3(256), 0, g.st, a, c0);
-  hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, a.cand_d, (uint64_t)Bmax * n,
-                     kInf32);
+  // The slots hold the values of the last build on this graph.  When it used
+  // the same slot encoding and its cleared rows cover this build's Bmax, this
+  // build continues its tag count-down (those values carry larger tags, so
+  // they read as older attempts) instead of clearing Bmax * n words;
+  // otherwise the rows are cleared and the count-down starts again.
+  const uint64_t slot_key = (1ull << 63) | ((uint64_t)n << 24) | ((uint64_t)slot_lw << 16) |
+                            ((uint64_t)a.slot_rbits << 8) | a.slot_shift;
+  const bool slots_kept = mg->slot_key == slot_key && Bmax <= mg->slot_rows_valid;
+  mg->slot_key = 0;  // set again when this build ends normally
+  if (!slots_kept) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, a.cand_d, (uint64_t)Bmax * n,
+                       kInf32);
+    mg->slot_rows_valid = Bmax;
+  }
   HSDL_HIP(hipGetLastError());
 
   // HSDL_BATCH_TARGET_US: the time the tuner aims one batch at (default 50 ms:
@@ -2220,8 +2242,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // attempt tags count down from tag_top; when they run out, the rows used
   // since the last clear are reset to all ones (the oldest possible value)
   const uint32_t tag_top = (uint32_t)((1ull << (32 - g.a.slot_shift)) - 2);
-  uint64_t tag_next = tag_top;
-  uint32_t rows_used = 0;
+  uint64_t tag_next = slots_kept ? mg->slot_tag_next : tag_top;
+  uint32_t rows_used = slots_kept ? mg->slot_rows_used : 0u;
   while (s0 < nhubs) {
     const uint32_t B = std::min<uint32_t>(tuner.B, nhubs - s0);
     const uint64_t tb = now_ns();
@@ -2327,6 +2349,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
                        in_group.as<uint8_t>());
     HSDL_HIP(hipGetLastError());
   }
+  mg->slot_key = slot_key;
+  mg->slot_tag_next = tag_next;
+  mg->slot_rows_used = rows_used;
   const uint64_t t_gen = now_ns();
   // release batch buffers before the export needs its own
   g.cflat.release();
@@ -2393,6 +2418,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   idx->stats.prune_launches = g.n_prune;
   idx->stats.prune_label_reads = c.label_reads;
   idx->stats.prune_tile_reads = c.tile_reads;
+  idx->stats.prune_row_slots = c.row_slots;
   idx->stats.prune_label_bytes = a.pool32 ? 4 : 8;
   // the pools stay with the graph for its next build (a pool released for
   // the sort leaves nothing to keep: the next build allocates anew)

@@ -220,6 +220,8 @@ typedef struct hsdl_stats {
                                        label pool), else 8                  */
   uint32_t clean_label_bytes;       /* same for the heads' lists the clean's
                                        first pass streamed (0: no clean)   */
+  uint64_t prune_row_slots;         /* dense (hub, vertex) slots the prune's
+                                       row items scanned for candidates     */
 } hsdl_stats;
 int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
 

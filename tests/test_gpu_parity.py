@@ -178,6 +178,27 @@ def test_slot_tags_wrap_with_wide_weights(mode, hub_batch):
     same_labels(idx, want.arrays())
 
 
+def test_slot_tags_continue_across_builds():
+    """A build on a graph whose slots were last written with the same slot
+    encoding keeps their contents and continues the tag count-down (their
+    values carry larger tags, so they read as older attempts) instead of
+    clearing them, unless its batch cap reaches rows above those the last
+    clear covered (16 -> 64 hubs).  With few tag bits the count-down wraps
+    inside and across builds; K = 2 (another encoding: fewer distance and
+    round bits) clears them, and a build that fails in between changes
+    nothing.  The labels never change."""
+    g = H.Graph.rmat(9, 8, 777, 1, 1 << 24)
+    c = csr_of(g)
+    want = {K: O.kround(c, K).arrays() for K in (2, 4, 5)}
+    for K, kw in ((5, {"hub_batch": 16}), (5, {"hub_batch": 64}), (5, {}), (5, {"hub_batch": 7}),
+                  (2, {}), (4, {}), (5, {"mode": H.MODE_PLAIN}), (5, {"hub_batch": 3}),
+                  (5, {})):
+        same_labels(H.hop_constrained_two_hop_labels_generation(g, K, **kw), want[K])
+    with pytest.raises(H.HSDLError):
+        H.hop_constrained_two_hop_labels_generation(g, 5, hbm_budget=1 << 20)
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 5), want[5])
+
+
 @pytest.fixture(scope="module")
 def rmat15():
     """R-MAT scale 15, edge factor 16, K=5 (32,768 vertices, the CPU-baseline
