@@ -288,9 +288,11 @@ def rooflines(H, stats, config, K, ref, clean_bytes):
     launches = max(1, stats["prune_launches"])  # one expand and one prune per round
     own = {
         # 4 or 8 B per label of L(y) streamed (up to the step with the
-        # witness) + 8 B per label of L(s) staged + 8 B per candidate
+        # witness) + 8 B per label of L(s) staged + 8 B per candidate + 4 B
+        # per dense slot row items scanned
         "k_prune": (stats.get("prune_label_bytes") or 8) * stats["prune_label_reads"]
-        + 8 * stats["prune_tile_reads"] + 8 * stats["candidates"],
+        + 8 * stats["prune_tile_reads"] + 8 * stats["candidates"]
+        + 4 * stats.get("prune_row_slots", 0),
         # 8 B per frontier label (its label and its share of the group
         # record) + 12 B per relaxed (label, edge) pair (8 B (col, w), which
         # the labels of a group share, so an upper bound, + the 4 B dense

@@ -285,18 +285,43 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
         }
       }
     };
+    // a group of one label (most groups): its hop and distance are the
+    // group's maxima, no pair runs through u inside the group, and the
+    // per-label loop and its branches go
+    auto check1 = [&](typename S::T l) {
+      uint32_t w, h1, d1;
+      str.dec(l, w, h1, d1);
+      if (kArena && !((w < u) & (h1 + hmin <= maxh) & (d1 <= maxd))) return;
+      uint32_t slot = hash_slot(w, cap);
+      uint32_t k = keys[slot];
+      while (k != w && k != kInf32) {
+        slot = (slot + 1) & (cap - 1);
+        k = keys[slot];
+      }
+      const bool reach = gh[0] >= h1;
+      const uint32_t m = vals[mul_u24(slot, K1) + (reach ? gh[0] - h1 : 0u)];
+      const bool near = sizeof(V) == 2 ? d1 + m <= gd[0]
+                                       : (m != TileVal<V>::kEmpty) & (d1 + m <= gd[0]);
+      hit |= (k == w) & (w < u) & reach & near;
+    };
     auto ld = [&](uint32_t j) { return Lu[min(j, last)]; };
-    typename S::T a0 = ld(lane), a1 = ld(kWave + lane);
-    for (uint32_t jb = 0; jb < ulen; jb += 4 * kWave) {
-      const typename S::T b0 = ld(jb + 2 * kWave + lane), b1 = ld(jb + 3 * kWave + lane);
-      check(a0);
-      check(a1);
-      if (jb + 2 * kWave >= ulen) break;
-      a0 = ld(jb + 4 * kWave + lane);
-      a1 = ld(jb + 5 * kWave + lane);
-      check(b0);
-      check(b1);
-    }
+    auto stream = [&](auto &&chk) {
+      typename S::T a0 = ld(lane), a1 = ld(kWave + lane);
+      for (uint32_t jb = 0; jb < ulen; jb += 4 * kWave) {
+        const typename S::T b0 = ld(jb + 2 * kWave + lane), b1 = ld(jb + 3 * kWave + lane);
+        chk(a0);
+        chk(a1);
+        if (jb + 2 * kWave >= ulen) break;
+        a0 = ld(jb + 4 * kWave + lane);
+        a1 = ld(jb + 5 * kWave + lane);
+        chk(b0);
+        chk(b1);
+      }
+    };
+    if (glen == 1)
+      stream(check1);
+    else
+      stream(check);
   }
   return __ballot(hit) != 0;
 }

@@ -212,6 +212,16 @@ __device__ inline uint32_t wave_incl_scan_u32(uint32_t v) {
   return v;
 }
 
+__device__ inline unsigned long long wave_incl_scan_u64(unsigned long long v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    unsigned long long t = __shfl_up(v, o, kWave);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
 }  // namespace hsdl
 
 // ---------------------------------------------------------------- index

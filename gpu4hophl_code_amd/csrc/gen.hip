@@ -850,8 +850,20 @@ template <class V>
 __device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, uint32_t *keys,
                            V *vals, uint32_t cap) {
   const uint32_t K1 = a.K + 1;
-  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) keys[i] = kInf32;
-  for (uint32_t i = threadIdx.x; i < cap * K1; i += blockDim.x) vals[i] = (V)TileVal<V>::kEmpty;
+  // empty keys and empty values are all ones: the tile (keys, then values)
+  // is filled 16 bytes per store where it is so aligned (a row item builds a
+  // tile for every few dozen candidates, and filling an LDS tile of 4096
+  // slots a 16-bit value at a time cost more than the candidates' probes)
+  const uint64_t words = tile_words<V>(cap, K1);
+  if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) {
+    uint4 *q = reinterpret_cast<uint4 *>(keys);
+    const uint4 ones = make_uint4(kInf32, kInf32, kInf32, kInf32);
+    for (uint32_t i = threadIdx.x; i < words / 4; i += blockDim.x) q[i] = ones;
+    for (uint32_t i = (uint32_t)(words & ~3ull) + threadIdx.x; i < words; i += blockDim.x)
+      keys[i] = kInf32;
+  } else {
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) keys[i] = kInf32;
+  }
   __syncthreads();
   const unsigned long long st = a.seg_start[s];
   for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
@@ -1360,25 +1372,44 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
   const uint32_t cnt = a.ctrl->touched_cnt;
-  for (uint32_t t = wave; t < cnt; t += nwaves) {
-    const uint32_t y = a.touched[t];
-    const uint32_t len = a.seg_len[y], cap = a.seg_cap[y];
-    const uint32_t need = len + a.add_cnt[y];
-    if (need <= cap) continue;
-    uint32_t ncap = max(need, cap * 2u);
-    ncap = (ncap + 7u) & ~7u;
-    unsigned long long ns = 0;
-    if (lane == 0) ns = atomicAdd(&a.ctrl->pool_top, (unsigned long long)ncap);
-    ns = __shfl(ns, 0, kWave);
-    if (ns + ncap > a.pool_cap) {
+  // 64 touched vertices per wave step (lane i: the step's vertex i), with one
+  // pool reservation for all of them that outgrow their segments: in a
+  // batch's first rounds most vertices grow, and one reservation per vertex
+  // on the single pool counter serialised them
+  for (uint32_t t0 = wave * kWave; t0 < cnt; t0 += nwaves * kWave) {
+    const uint32_t t = t0 + (uint32_t)lane;
+    uint32_t y = 0, len = 0, ncap = 0;
+    if (t < cnt) {
+      y = a.touched[t];
+      len = a.seg_len[y];
+      const uint32_t cap = a.seg_cap[y];
+      const uint32_t need = len + a.add_cnt[y];
+      if (need > cap) ncap = (max(need, cap * 2u) + 7u) & ~7u;
+    }
+    const unsigned long long grow = __ballot(ncap != 0);
+    if (!grow) continue;
+    const unsigned long long incl = wave_incl_scan_u64(ncap);
+    const unsigned long long total = __shfl(incl, kWave - 1, kWave);
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(&a.ctrl->pool_top, total);
+    base = __shfl(base, 0, kWave);
+    if (base + total > a.pool_cap) {
       if (lane == 0) atomicOr(&a.ctrl->overflow, kOvfPool);
       continue;
     }
-    const unsigned long long os = a.seg_start[y];
-    for (uint32_t i = lane; i < len; i += kWave) a.pool[ns + i] = a.pool[os + i];
-    if (a.pool32)
-      for (uint32_t i = lane; i < len; i += kWave) a.pool32[ns + i] = a.pool32[os + i];
-    if (lane == 0) {
+    const unsigned long long ns = base + incl - ncap;
+    // each growing vertex's labels move to its new segment, the whole wave
+    // copying one vertex at a time
+    for (unsigned long long m = grow; m; m &= m - 1) {
+      const int j = __ffsll((long long)m) - 1;
+      const uint32_t yj = __shfl(y, j, kWave), lj = __shfl(len, j, kWave);
+      const unsigned long long nsj = __shfl(ns, j, kWave);
+      const unsigned long long os = a.seg_start[yj];
+      for (uint32_t i = lane; i < lj; i += kWave) a.pool[nsj + i] = a.pool[os + i];
+      if (a.pool32)
+        for (uint32_t i = lane; i < lj; i += kWave) a.pool32[nsj + i] = a.pool32[os + i];
+    }
+    if (ncap) {
       a.seg_start[y] = ns;
       a.seg_cap[y] = ncap;
     }
