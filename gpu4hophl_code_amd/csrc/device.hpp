@@ -116,25 +116,6 @@ __device__ __host__ inline uint64_t tile_words(uint32_t cap, uint32_t K1) {
 template <class T> using lds_ptr = __attribute__((address_space(3))) T *;
 template <class T> using glb_ptr = __attribute__((address_space(1))) T *;
 
-// Value for (hub, hop budget x), or the empty value when the hub is absent.
-// KP / VP: lds_ptr or glb_ptr of const uint32_t / const V.
-// The first probe is straight-line code; the loop runs only for lanes whose
-// home slot holds another hub.  The value read is unconditional (slot < cap)
-// and selected afterwards.
-template <class V, class KP, class VP>
-__device__ inline uint32_t tile_lookup(KP keys, VP vals, uint32_t cap,
-                                       uint32_t K1, uint32_t hub, uint32_t x) {
-  const uint32_t mask = cap - 1;
-  uint32_t slot = hash_slot(hub, cap);
-  uint32_t k = keys[slot];
-  while (k != hub && k != kInf32) {
-    slot = (slot + 1) & mask;
-    k = keys[slot];
-  }
-  const uint32_t v = vals[mul_u24(slot, K1) + x];
-  return k == hub ? v : TileVal<V>::kEmpty;
-}
-
 // Device buffer owned by RAII.
 struct DevBuf {
   void *p = nullptr;

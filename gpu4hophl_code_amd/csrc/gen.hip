@@ -846,23 +846,52 @@ __device__ inline void atomic_min_v(uint16_t *p, uint32_t v) {
 }
 __device__ inline void atomic_min_v(uint32_t *p, uint32_t v) { atomicMin(p, v); }
 
+// A prune tile of round h holds one record per slot: the hub w (the key) and
+// K + 1 values, value ha = the smallest distance of w's labels in L(s) of hop
+// <= h - ha (empty when ha > h): the value a label (w, ha, da) of L(y) pairs
+// with.  Records are 2^rec_shift words (16 bytes for K <= 5 and 16-bit
+// values), so a probe reaches the key with one shift-add from the slot and
+// the value at ha * sizeof(V) bytes past the key's end.
+template <class V> __host__ __device__ inline uint32_t rec_shift(uint32_t K1) {
+  const uint32_t w = 1u + (K1 * (uint32_t)sizeof(V) + 3u) / 4u;
+  uint32_t sh = 0;
+  while ((1u << sh) < w) ++sh;
+  return sh;
+}
+template <class V> __host__ __device__ inline uint32_t rec_words(uint32_t K1) {
+  return 1u << rec_shift<V>(K1);
+}
+template <class V> __host__ __device__ inline uint64_t rec_tile_words(uint32_t cap, uint32_t K1) {
+  return (uint64_t)cap * rec_words<V>(K1);
+}
+// ha * sizeof(V) for label l of hop ha, straight from its hop field
+template <class V> __device__ inline uint32_t hop_off(const GenArgs &a, uint32_t l) {
+  constexpr uint32_t vs = sizeof(V) == 2 ? 1u : 2u;  // 32-bit values: dist_bits >= 16
+  return (l >> (a.p32_dist_bits - vs)) & (((1u << a.p32_hop_bits) - 1u) << vs);
+}
+template <class V> __device__ inline uint32_t hop_off(const GenArgs &, unsigned long long l) {
+  constexpr uint32_t vs = sizeof(V) == 2 ? 1u : 2u;
+  return label_hop(l) << vs;
+}
+
 template <class V>
-__device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, uint32_t *keys,
-                           V *vals, uint32_t cap) {
+__device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, uint32_t *rec,
+                           uint32_t cap) {
   const uint32_t K1 = a.K + 1;
-  // empty keys and empty values are all ones: the tile (keys, then values)
-  // is filled 16 bytes per store where it is so aligned (a row item builds a
-  // tile for every few dozen candidates, and filling an LDS tile of 4096
-  // slots a 16-bit value at a time cost more than the candidates' probes)
-  const uint64_t words = tile_words<V>(cap, K1);
-  if ((reinterpret_cast<uintptr_t>(keys) & 15u) == 0) {
-    uint4 *q = reinterpret_cast<uint4 *>(keys);
+  const uint32_t stride = rec_words<V>(K1);
+  // empty keys and empty values are all ones: the tile is filled 16 bytes
+  // per store where it is so aligned (a row item builds a tile for every few
+  // dozen candidates, and filling an LDS tile of 4096 slots a 16-bit value
+  // at a time cost more than the candidates' probes)
+  const uint64_t words = rec_tile_words<V>(cap, K1);
+  if ((reinterpret_cast<uintptr_t>(rec) & 15u) == 0) {
+    uint4 *q = reinterpret_cast<uint4 *>(rec);
     const uint4 ones = make_uint4(kInf32, kInf32, kInf32, kInf32);
     for (uint32_t i = threadIdx.x; i < words / 4; i += blockDim.x) q[i] = ones;
     for (uint32_t i = (uint32_t)(words & ~3ull) + threadIdx.x; i < words; i += blockDim.x)
-      keys[i] = kInf32;
+      rec[i] = kInf32;
   } else {
-    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) keys[i] = kInf32;
+    for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) rec[i] = kInf32;
   }
   __syncthreads();
   const unsigned long long st = a.seg_start[s];
@@ -871,11 +900,12 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, u
     const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
     uint32_t slot = hash_slot(hub, cap);
     for (;;) {
-      const uint32_t prev = atomicCAS(&keys[slot], kInf32, hub);
+      const uint32_t prev = atomicCAS(&rec[slot * stride], kInf32, hub);
       if (prev == kInf32 || prev == hub) break;
       slot = (slot + 1) & (cap - 1);
     }
-    for (uint32_t x = hb; x < K1; ++x) atomic_min_v(&vals[slot * K1 + x], db);
+    V *vals = reinterpret_cast<V *>(rec + slot * stride + 1);
+    for (uint32_t ha = 0; ha + hb <= a.h; ++ha) atomic_min_v(&vals[ha], db);
   }
   __syncthreads();
 }
@@ -893,28 +923,52 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, u
 // the prune issues VALU instructions near the rate a CU sustains:
 // * lanes past the end of the list hold a copy of its last label, which
 //   gives the same answer as the label itself, so they need no mask;
-// * a label of hop ha > h reads the hop-budget-0 value, which only the
-//   tile's own hub s has (its self label) and which no (s, ha, .) of L(y)
-//   can reach: hub s labels L(y) in rounds 1..h-1 of this batch only;
+// * a label of hop ha > h reads its record's value ha, which is empty;
 // * the chain ends at the hub or at an empty slot, whose values are empty,
 //   so the value read needs no select: a 16-bit empty value (0xFFFF) is
 //   above every candidate distance (small values: K * max weight < 0xFFFF),
 //   and a 32-bit one is tested.
-template <class V, class KP, class VP, class LT>
-__device__ inline bool probe_label(const GenArgs &a, LT l, uint32_t dy, KP keys, VP vals,
-                                   uint32_t cap, uint32_t K1) {
-  uint32_t hub, ha, da;
-  unpack(a, l, hub, ha, da);
-  const uint32_t x = a.h > ha ? a.h - ha : 0u;
-  uint32_t slot = hash_slot(hub, cap);
-  uint32_t k = keys[slot];
-  while (k != hub && k != kInf32) {
-    slot = (slot + 1) & (cap - 1);
-    k = keys[slot];
-  }
-  const uint32_t v = vals[mul_u24(slot, K1) + x];
+template <class V>
+__device__ inline bool within(uint32_t da, uint32_t v, uint32_t dy) {
   if (sizeof(V) == 2) return da + v <= dy;
   return v != TileVal<V>::kEmpty && da + v <= dy;
+}
+// LDS tile: addresses as 32-bit LDS byte offsets, so the value's address is
+// the key's plus the hop offset (the record's base is computed once)
+template <class V, class LT>
+__device__ inline bool probe_label(const GenArgs &a, LT l, uint32_t dy,
+                                   lds_ptr<const uint32_t> rec, uint32_t cap, uint32_t sh) {
+  uint32_t hub, ha, da;
+  unpack(a, l, hub, ha, da);
+  const uint32_t base = (uint32_t)(uintptr_t)rec;
+  const uint32_t rsh = sh + 2;
+  uint32_t slot = hash_slot(hub, cap);
+  uint32_t ka = base + (slot << rsh);
+  uint32_t k = *(lds_ptr<const uint32_t>)(uintptr_t)ka;
+  while (k != hub && k != kInf32) {
+    slot = (slot + 1) & (cap - 1);
+    ka = base + (slot << rsh);
+    k = *(lds_ptr<const uint32_t>)(uintptr_t)ka;
+  }
+  const uint32_t v = *(lds_ptr<const V>)(uintptr_t)(ka + 4u + hop_off<V>(a, l));
+  return within<V>(da, v, dy);
+}
+// arena tile
+template <class V, class LT>
+__device__ inline bool probe_label(const GenArgs &a, LT l, uint32_t dy,
+                                   glb_ptr<const uint32_t> rec, uint32_t cap, uint32_t sh) {
+  uint32_t hub, ha, da;
+  unpack(a, l, hub, ha, da);
+  uint32_t slot = hash_slot(hub, cap);
+  glb_ptr<const uint32_t> r = rec + (slot << sh);
+  uint32_t k = *r;
+  while (k != hub && k != kInf32) {
+    slot = (slot + 1) & (cap - 1);
+    r = rec + (slot << sh);
+    k = *r;
+  }
+  const uint32_t v = *(glb_ptr<const V>)((glb_ptr<const char>)(r + 1) + hop_off<V>(a, l));
+  return within<V>(da, v, dy);
 }
 
 // Label `lane` of a list view, clamped to it (a lane past the end reloads
@@ -935,11 +989,11 @@ __device__ inline LT first_step(const LT *L, const ListView &v) {
 // end) keep the next 128-label step in flight while one is probed.  No load
 // sits behind a branch and the step buffers alternate (a, b) instead of
 // being copied, so a wait leaves the step in flight alone.
-template <class V, class KP, class VP, class LT>
+template <class V, class KP, class LT>
 __device__ bool witness_search(const GenArgs &a, const LT *L, const ListView &view, LT f0,
-                               uint32_t dy, KP keys, VP vals, uint32_t cap, uint32_t &reads) {
+                               uint32_t dy, KP rec, uint32_t cap, uint32_t sh,
+                               uint32_t &reads) {
   const uint32_t lane = (uint32_t)lane_id();
-  const uint32_t K1 = a.K + 1;
   const uint32_t len = view.len;
   if (len == 0) {
     reads = 0;
@@ -948,12 +1002,12 @@ __device__ bool witness_search(const GenArgs &a, const LT *L, const ListView &vi
   const uint32_t last = len - 1;
   auto ld = [&](uint32_t j) { return L[view.at(min(j, last))]; };
   auto probe2 = [&](LT l0, LT l1, uint32_t j) {
-    const bool h0 = probe_label<V>(a, l0, dy, keys, vals, cap, K1);
-    const bool h1 = probe_label<V>(a, l1, dy, keys, vals, cap, K1);
+    const bool h0 = probe_label<V>(a, l0, dy, rec, cap, sh);
+    const bool h1 = probe_label<V>(a, l1, dy, rec, cap, sh);
     return __ballot(h0 || h1) != 0;
   };
   constexpr uint32_t kStep = 2 * kWave;
-  if (__ballot(probe_label<V>(a, f0, dy, keys, vals, cap, K1))) {
+  if (__ballot(probe_label<V>(a, f0, dy, rec, cap, sh))) {
     reads = min(len, (uint32_t)kWave);
     return true;
   }
@@ -1044,11 +1098,12 @@ constexpr uint32_t kClaimSpread = 2 * (kPruneThreads / kWave);
 // round trip per level (lane i: claimed candidate i); the first 64 labels of
 // the next claimed candidate's list are loaded before the current one's probes.
 // Instantiated once for an LDS tile and once for an arena tile.
-template <class V, class LT, class KP, class VP>
+template <class V, class LT, class KP>
 __device__ __attribute__((always_inline)) inline void prune_candidates(
-    const GenArgs &a, uint32_t b, uint32_t c_beg, uint32_t c_end, const uint32_t *cands, KP keys,
-    VP vals, uint32_t cap, uint32_t &sh_next, uint32_t &sh_acc_n, Item *sh_acc,
+    const GenArgs &a, uint32_t b, uint32_t c_beg, uint32_t c_end, const uint32_t *cands, KP rec,
+    uint32_t cap, uint32_t &sh_next, uint32_t &sh_acc_n, Item *sh_acc,
     unsigned long long &wave_reads) {
+  const uint32_t sh = rec_shift<V>(a.K + 1);
   const uint32_t lane = (uint32_t)lane_id();
   for (;;) {
     uint32_t c = 0, k = 0;
@@ -1103,7 +1158,7 @@ __device__ __attribute__((always_inline)) inline void prune_candidates(
       const LT g0 = first_step(Ln, vn);
       const uint32_t yi = __builtin_amdgcn_readlane(y, i), di = __builtin_amdgcn_readlane(dy, i);
       uint32_t reads = 0;
-      const bool pruned = witness_search<V>(a, Lc, vc, f0, di, keys, vals, cap, reads);
+      const bool pruned = witness_search<V>(a, Lc, vc, f0, di, rec, cap, sh, reads);
       wave_reads += reads;
       if (lane == 0) {
         if (!pruned) {
@@ -1190,30 +1245,28 @@ k_prune(GenArgs a) {
     if (cap <= a.lds_slots) {
       const uint32_t lcap = lds_tile_cap(sv.len, a.lds_slots);
       if (tile_b != b) {
-        build_tile<V>(a, s, sv, lds, reinterpret_cast<V *>(lds + lcap), lcap);
+        build_tile<V>(a, s, sv, lds, lcap);
         tile_b = b;
         tile_reads += sv.len;
       }
-      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds,
-                          (lds_ptr<const V>)(lds + lcap), lcap, sh_next, sh_acc_n, sh_acc,
-                          wave_reads);
+      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (lds_ptr<const uint32_t>)lds, lcap,
+                              sh_next, sh_acc_n, sh_acc, wave_reads);
     } else {
-      if (tile_words<V>(cap, K1) > gwords) {
+      if (rec_tile_words<V>(cap, K1) > gwords) {
         // arena too small: the host rolls the batch back, grows the arena
         // and runs the batch again
         if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, kOvfArena);
         __syncthreads();
         continue;
       }
-      build_tile<V>(a, s, sv, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
+      build_tile<V>(a, s, sv, gkeys, cap);
       // the tile was written through L2 (atomics); drop stale L1 lines of
       // this workgroup's arena slice before plain loads read it
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
       tile_b = kInf32;
       tile_reads += sv.len;
-      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (glb_ptr<const uint32_t>)gkeys,
-                          (glb_ptr<const V>)(gkeys + cap), cap, sh_next, sh_acc_n, sh_acc,
-                          wave_reads);
+      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, (glb_ptr<const uint32_t>)gkeys, cap,
+                              sh_next, sh_acc_n, sh_acc, wave_reads);
     }
     __syncthreads();
   }
@@ -1292,7 +1345,7 @@ k_prune_rows(GenArgs a) {
     const uint32_t cap = tile_cap_for(sv.len);
     const bool in_lds = cap <= a.lds_slots;
     const uint32_t lcap = in_lds ? lds_tile_cap(sv.len, a.lds_slots) : cap;
-    if (!in_lds && tile_words<V>(cap, K1) > gwords) {
+    if (!in_lds && rec_tile_words<V>(cap, K1) > gwords) {
       // arena too small: the host rolls the batch back, grows the arena and
       // runs the batch again
       if (threadIdx.x == 0) atomicOr(&a.ctrl->overflow, kOvfArena);
@@ -1326,12 +1379,12 @@ k_prune_rows(GenArgs a) {
       if (!built) {
         if (in_lds) {
           if (tile_b != b) {
-            build_tile<V>(a, s, sv, lds, reinterpret_cast<V *>(lds + lcap), lcap);
+            build_tile<V>(a, s, sv, lds, lcap);
             tile_b = b;
             tile_reads += sv.len;
           }
         } else {
-          build_tile<V>(a, s, sv, gkeys, reinterpret_cast<V *>(gkeys + cap), cap);
+          build_tile<V>(a, s, sv, gkeys, cap);
           // written through L2 (atomics): drop stale L1 lines of the slice
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
           tile_b = kInf32;
@@ -1348,13 +1401,11 @@ k_prune_rows(GenArgs a) {
         if (threadIdx.x == 0) sh_next = 0;
         __syncthreads();
         if (in_lds)
-          prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (lds_ptr<const uint32_t>)lds,
-                                  (lds_ptr<const V>)(lds + lcap), lcap, sh_next, sh_acc_n,
-                                  sh_acc, wave_reads);
+          prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (lds_ptr<const uint32_t>)lds, lcap,
+                                  sh_next, sh_acc_n, sh_acc, wave_reads);
         else
-          prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (glb_ptr<const uint32_t>)gkeys,
-                                  (glb_ptr<const V>)(gkeys + cap), cap, sh_next, sh_acc_n,
-                                  sh_acc, wave_reads);
+          prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (glb_ptr<const uint32_t>)gkeys, cap,
+                                  sh_next, sh_acc_n, sh_acc, wave_reads);
         __syncthreads();
       }
     }
@@ -2143,8 +2194,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   g.small_vals = (uint64_t)K * (uint64_t)std::max(hg->max_w, 1) < 0xFFFFull;
   const uint64_t lds_words = 64 * 1024 / 4;  // + ~4.5 KB of static LDS; two workgroups per CU
   uint32_t lds_slots = 64;
-  while ((g.small_vals ? tile_words<uint16_t>(lds_slots + 64, K1)
-                       : tile_words<uint32_t>(lds_slots + 64, K1)) <= lds_words)
+  while ((g.small_vals ? rec_tile_words<uint16_t>(lds_slots + 64, K1)
+                       : rec_tile_words<uint32_t>(lds_slots + 64, K1)) <= lds_words)
     lds_slots += 64;
   if (opts->lds_tile_slots) lds_slots = std::min(lds_slots, opts->lds_tile_slots);
   g.lds_bytes = (size_t)lds_words * 4;
