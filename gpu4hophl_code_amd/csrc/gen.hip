@@ -2286,17 +2286,24 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   }
   hipLaunchKernelGGL(k_init_labels, dim3(1024), dim3(256), 0, g.st, a, c0);
   // The slots hold the values of the last build on this graph.  When it used
-  // the same slot encoding and its cleared rows cover this build's Bmax, this
-  // build continues its tag count-down (those values carry larger tags, so
-  // they read as older attempts) instead of clearing Bmax * n words;
-  // otherwise the rows are cleared and the count-down starts again.
+  // the same slot encoding, this build continues its tag count-down (those
+  // values carry larger tags, so they read as older attempts) instead of
+  // clearing Bmax * n words, clearing only the rows above those the last
+  // clear covered (Bmax follows the free HBM, which moves a little between
+  // builds); otherwise the rows are cleared and the count-down starts again.
   const uint64_t slot_key = (1ull << 63) | ((uint64_t)n << 24) | ((uint64_t)slot_lw << 16) |
                             ((uint64_t)a.slot_rbits << 8) | a.slot_shift;
-  const bool slots_kept = mg->slot_key == slot_key && Bmax <= mg->slot_rows_valid;
+  const bool slots_kept = mg->slot_key == slot_key;
   mg->slot_key = 0;  // set again when this build ends normally
   if (!slots_kept) {
     hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st, a.cand_d, (uint64_t)Bmax * n,
                        kInf32);
+    mg->slot_rows_valid = Bmax;
+  } else if (Bmax > mg->slot_rows_valid) {
+    // rows [valid, Bmax) are whole windows: words [valid * n, Bmax * n)
+    hipLaunchKernelGGL(k_fill_u32, dim3(2048), dim3(256), 0, g.st,
+                       a.cand_d + (uint64_t)mg->slot_rows_valid * n,
+                       (uint64_t)(Bmax - mg->slot_rows_valid) * n, kInf32);
     mg->slot_rows_valid = Bmax;
   }
   HSDL_HIP(hipGetLastError());
