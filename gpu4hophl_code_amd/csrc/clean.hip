@@ -50,6 +50,15 @@ constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vert
 constexpr uint32_t kTileWindow = 4;            // labels of L(v) per work unit (default)
 constexpr uint32_t kMaxWindow = 64;            // largest window
 constexpr uint32_t kTileLdsBytes = 64 * 1024 + 16;  // LDS tile budget (two workgroups per CU)
+// Indexed LDS tiles (graphs of at most 2^20 vertices): a key slot holds
+// hub << kIdxBits | row, the hub's K + 1 values sit in a compact row array
+// (one row per distinct hub, not per slot), so the 64 KB hold 4-byte key
+// slots at load factor <= 1/8 for lists of up to about 2,000 hubs instead of
+// 16-byte slots at up to 2/3: a wave's probe chain is its longest lane's,
+// and chains grow fast with the load factor.  Rows < 2^kIdxBits - 1 keep a
+// key slot from ever reading as empty (all ones).
+constexpr uint32_t kIdxBits = 12;
+constexpr uint32_t kIdxRows = (1u << kIdxBits) - 1;
 constexpr uint32_t kGroupFast = 8;             // group sizes with the unrolled check
 
 __device__ inline void wave_lds_sync() {
@@ -185,12 +194,34 @@ __global__ void __launch_bounds__(256) k_pack32_hop(uint32_t n, uint32_t K,
 // alternate (a, b) and loads are clamped to the list (lanes past the end are
 // masked), so a step's probes run while the next step's loads are in flight.
 // KP / VP: the tile's address space (LDS or the global arena).
-template <class V, class S, class KP, class VP>
+template <class V, bool kIdx, class S, class KP, class VP>
 __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off,
                                const S &str, const unsigned long long *__restrict__ Lv, uint32_t len,
                                uint32_t p0, uint32_t win, KP keys, VP vals, uint32_t cap,
-                               uint64_t &streamed) {
+                               uint32_t last_row, uint64_t &streamed) {
   constexpr bool kArena = !std::is_same<KP, lds_ptr<uint32_t>>::value;
+  // hub w's key slot: sets `found` and returns the index of its value row
+  // (an empty slot's is any in-bounds row: the caller masks it)
+  auto find = [&](uint32_t w, bool &found) -> uint32_t {
+    uint32_t slot = hash_slot(w, cap);
+    uint32_t k = keys[slot];
+    if (kIdx) {
+      while ((k >> kIdxBits) != w && k != kInf32) {
+        slot = (slot + 1) & (cap - 1);
+        k = keys[slot];
+      }
+      // an empty slot reads hub 2^20 - 1, which ranks below every head u
+      // that the pair test also asks for (w < u)
+      found = (k >> kIdxBits) == w;
+      return mul_u24(min(k & kIdxRows, last_row), K1);
+    }
+    while (k != w && k != kInf32) {
+      slot = (slot + 1) & (cap - 1);
+      k = keys[slot];
+    }
+    found = k == w;
+    return mul_u24(slot, K1);
+  };
   const uint32_t lane = (uint32_t)lane_id();
   bool hit = false;
   const uint32_t pend = min(p0 + win, len);
@@ -255,14 +286,9 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
         }
       }
       if (kArena && !((w < u) & (h1 + hmin <= maxh) & (d1 <= maxd))) return;
-      uint32_t slot = hash_slot(w, cap);
-      uint32_t k = keys[slot];
-      while (k != w && k != kInf32) {
-        slot = (slot + 1) & (cap - 1);
-        k = keys[slot];
-      }
-      const bool pair = (k == w) & (w < u);
-      const uint32_t base = mul_u24(slot, K1);
+      bool found;
+      const uint32_t base = find(w, found);
+      const bool pair = found & (w < u);
       auto fits = [&](uint32_t m, uint32_t gd) -> bool {
         if (sizeof(V) == 2) return d1 + m <= gd;
         return (m != TileVal<V>::kEmpty) & (d1 + m <= gd);
@@ -292,17 +318,13 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       uint32_t w, h1, d1;
       str.dec(l, w, h1, d1);
       if (kArena && !((w < u) & (h1 + hmin <= maxh) & (d1 <= maxd))) return;
-      uint32_t slot = hash_slot(w, cap);
-      uint32_t k = keys[slot];
-      while (k != w && k != kInf32) {
-        slot = (slot + 1) & (cap - 1);
-        k = keys[slot];
-      }
+      bool found;
+      const uint32_t base = find(w, found);
       const bool reach = gh[0] >= h1;
-      const uint32_t m = vals[mul_u24(slot, K1) + (reach ? gh[0] - h1 : 0u)];
+      const uint32_t m = vals[base + (reach ? gh[0] - h1 : 0u)];
       const bool near = sizeof(V) == 2 ? d1 + m <= gd[0]
                                        : (m != TileVal<V>::kEmpty) & (d1 + m <= gd[0]);
-      hit |= (k == w) & (w < u) & reach & near;
+      hit |= found & (w < u) & reach & near;
     };
     auto ld = [&](uint32_t j) { return Lu[min(j, last)]; };
     auto stream = [&](auto &&chk) {
@@ -329,21 +351,30 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
 // Builds the tile of L(v) (keys, vals in LDS or in the workgroup's arena
 // slice) and runs the windows of L(v) over it; true when some window removes
 // a label.  Whole workgroup.
-template <class V, class S, class KP, class VP>
+template <class V, bool kIdx, class S, class KP, class VP>
 __device__ __attribute__((always_inline)) inline bool clean_vertex(
     uint32_t v, uint32_t K1, const unsigned long long *__restrict__ off, const S &str,
     const unsigned long long *__restrict__ Lv,
-    uint32_t len, KP keys, VP vals, uint32_t cap, bool in_arena, uint32_t win, uint32_t &sh_win,
-    uint32_t &sh_removed, uint64_t &streamed) {
+    uint32_t len, KP keys, VP vals, uint32_t cap, uint32_t rows, bool in_arena, uint32_t win,
+    uint32_t &sh_win, uint32_t &sh_removed, uint32_t &sh_row, uint64_t &streamed) {
   const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id();
   for (uint32_t i = tid; i < cap; i += kTileThreads) keys[i] = kInf32;
+  if (tid == 0) sh_row = 0;
   __syncthreads();
-  // one thread per group: claim the hub's slot, write its per-hop prefix minima
+  // one thread per group: claim the hub's slot (and, indexed, a value row),
+  // write its per-hop prefix minima
   for (uint32_t i = tid; i < len; i += kTileThreads) {
     const uint32_t w = label_hub(Lv[i]);
     if (i > 0 && label_hub(Lv[i - 1]) == w) continue;
     uint32_t slot = hash_slot(w, cap);
-    while (atomicCAS((uint32_t *)&keys[slot], kInf32, w) != kInf32) slot = (slot + 1) & (cap - 1);
+    if (kIdx) {
+      const uint32_t row = atomicAdd(&sh_row, 1u);
+      while (atomicCAS((uint32_t *)&keys[slot], kInf32, (w << kIdxBits) | row) != kInf32)
+        slot = (slot + 1) & (cap - 1);
+      slot = row;  // the values go to the row
+    } else {
+      while (atomicCAS((uint32_t *)&keys[slot], kInf32, w) != kInf32) slot = (slot + 1) & (cap - 1);
+    }
     uint32_t m[kMaxK + 1];
 #pragma unroll
     for (uint32_t x = 0; x <= kMaxK; ++x) m[x] = kInf32;
@@ -369,7 +400,8 @@ __device__ __attribute__((always_inline)) inline bool clean_vertex(
     if (lane == 0) p0 = atomicAdd(&sh_win, win);
     p0 = __builtin_amdgcn_readfirstlane(__shfl(p0, 0, kWave));
     if (p0 >= len) break;
-    if (window_removes<V>(v, K1, off, str, Lv, len, p0, win, keys, vals, cap, streamed)) {
+    if (window_removes<V, kIdx>(v, K1, off, str, Lv, len, p0, win, keys, vals, cap, rows - 1,
+                                streamed)) {
       if (lane == 0) sh_removed = 1;
       break;
     }
@@ -385,9 +417,11 @@ __global__ void __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_
 k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
              const unsigned long long *__restrict__ lab, S str,
              uint8_t *keep, uint32_t *new_len, uint32_t *redo, uint32_t *ctr, uint32_t lds_slots,
-             uint32_t *gtile, uint64_t gwords, uint32_t win) {
+             uint32_t *gtile, uint64_t gwords, uint32_t win, uint32_t indexed) {
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t sh_v, sh_win, sh_heads, sh_removed;
+  __shared__ uint32_t sh_v, sh_win, sh_heads, sh_removed, sh_row;
+  // LDS bytes for a tile (the launch gives lds_slots slots of key + values)
+  const uint32_t lds_tile_bytes = lds_slots * (4u + (K + 1) * (uint32_t)sizeof(V));
   const uint32_t K1 = K + 1;
   const uint32_t tid = threadIdx.x, lane = (uint32_t)lane_id();
   uint32_t *gkeys = gtile + (uint64_t)blockIdx.x * gwords;
@@ -415,14 +449,31 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
       mine = wave_sum_u32(mine);
       if (lane == 0 && mine) atomicAdd(&sh_heads, mine);
       __syncthreads();
-      const uint32_t cap = tile_cap_for(sh_heads);
-      if (cap <= lds_slots) {
-        const uint32_t lcap = lds_tile_cap(sh_heads, lds_slots);
-        clean_vertex<V>(v, K1, off, str, Lv, len, (lds_ptr<uint32_t>)lds,
-                        (lds_ptr<V>)(lds + lcap), lcap, false, win, sh_win, sh_removed, streamed);
-      } else
-        clean_vertex<V>(v, K1, off, str, Lv, len, (glb_ptr<uint32_t>)gkeys,
-                        (glb_ptr<V>)(gkeys + cap), cap, true, win, sh_win, sh_removed, streamed);
+      const uint32_t heads = sh_heads;
+      const uint32_t cap = tile_cap_for(heads);
+      // indexed: key slots at load factor <= 1/8 where the rows leave room
+      uint32_t icap = 0;
+      if (indexed && heads <= kIdxRows) {
+        const uint32_t row_bytes = (heads * K1 * (uint32_t)sizeof(V) + 15u) & ~15u;
+        if (row_bytes + 4u * cap <= lds_tile_bytes) {
+          icap = pow2_at_least(8ull * heads + 4);
+          while (row_bytes + 4u * icap > lds_tile_bytes) icap >>= 1;
+        }
+      }
+      if (icap) {
+        clean_vertex<V, true>(v, K1, off, str, Lv, len, (lds_ptr<uint32_t>)lds,
+                              (lds_ptr<V>)(lds + icap), icap, heads, false, win, sh_win,
+                              sh_removed, sh_row, streamed);
+      } else if (cap <= lds_slots) {
+        const uint32_t lcap = lds_tile_cap(heads, lds_slots);
+        clean_vertex<V, false>(v, K1, off, str, Lv, len, (lds_ptr<uint32_t>)lds,
+                               (lds_ptr<V>)(lds + lcap), lcap, lcap, false, win, sh_win,
+                               sh_removed, sh_row, streamed);
+      } else {
+        clean_vertex<V, false>(v, K1, off, str, Lv, len, (glb_ptr<uint32_t>)gkeys,
+                               (glb_ptr<V>)(gkeys + cap), cap, cap, true, win, sh_win,
+                               sh_removed, sh_row, streamed);
+      }
     }
     if (sh_removed) {
       if (tid == 0) redo[atomicAdd(&ctr[0], 1u)] = v;
@@ -667,6 +718,10 @@ int clean_index(hsdl_index *idx) {
   HSDL_HIP(hipStreamSynchronize(st));
   const uint32_t K1 = (uint32_t)idx->K + 1;
   const bool small_vals = mx[1] < 0xFFFFu;
+  // indexed LDS tiles need hub << kIdxBits to fit a word (tuning knob
+  // HSDL_CLEAN_INDEXED=0: per-slot values only)
+  const uint32_t indexed =
+      (uint64_t)n <= (1ull << (32 - kIdxBits)) ? env_u32("HSDL_CLEAN_INDEXED", 1, 0, 1) : 0u;
   const uint32_t slot_bytes = 4 + K1 * (small_vals ? 2 : 4);
   // tuning knob HSDL_CLEAN_LDS_SLOTS caps the LDS tile (small values send
   // every vertex to the global arena)
@@ -721,7 +776,7 @@ int clean_index(hsdl_index *idx) {
     hipLaunchKernelGGL(kern, dim3(tgrid), dim3(kTileThreads), lds_bytes, st, n, (uint32_t)idx->K,
                        idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(), str,
                        keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
-                       lds_slots, gtile.as<uint32_t>(), gwords, win);
+                       lds_slots, gtile.as<uint32_t>(), gwords, win, indexed);
   };
   if (small_vals) {
     if (use32) launch(k_clean_tile<uint16_t, Stream32>, s32);

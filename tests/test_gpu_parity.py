@@ -283,14 +283,18 @@ def test_grid_k8_matches_oracle():
     same_labels(idx, O.clean(want).arrays())
 
 
+@pytest.mark.parametrize("indexed", ["1", "0"])
 @pytest.mark.parametrize("stream32", ["1", "0"])
-def test_clean_removes_exactly_the_redundant_labels(monkeypatch, stream32):
+def test_clean_removes_exactly_the_redundant_labels(monkeypatch, stream32, indexed):
     """Inputs with redundancy (the reference's group-restricted pruning): the
     GPU clean must return the oracle clean, which equals canonical labels,
     whether its first pass streams the heads' lists from the 32-bit copy of
     the labels (the fields fit one word here) or from the 8-byte labels
-    (HSDL_STREAM32=0)."""
+    (HSDL_STREAM32=0), and whether its LDS tiles keep the values in per-hub
+    rows behind packed key slots (the default up to 2^20 vertices) or per
+    slot (HSDL_CLEAN_INDEXED=0)."""
     monkeypatch.setenv("HSDL_STREAM32", stream32)
+    monkeypatch.setenv("HSDL_CLEAN_INDEXED", indexed)
     for seed, (scale, K, group) in enumerate([(9, 3, 16), (10, 4, 64), (10, 5, 7), (11, 3, 128)]):
         g = H.Graph.rmat(scale, 8, 1000 + seed, 1, 20)
         c = csr_of(g)
