@@ -49,17 +49,23 @@ constexpr uint32_t kSeqWordsPerWave = kSeqBitsPerWave / 32;
 constexpr uint32_t kTileThreads = 1024;        // pass 1: sixteen waves per vertex
 constexpr uint32_t kTileWindow = 4;            // labels of L(v) per work unit (default)
 constexpr uint32_t kMaxWindow = 64;            // largest window
-constexpr uint32_t kTileLdsBytes = 64 * 1024 + 16;  // LDS tile budget (two workgroups per CU)
+// LDS tile budget: two workgroups per CU share its 160 KB
+constexpr uint32_t kTileLdsBytes = 78 * 1024 + 16;
 // Indexed LDS tiles (graphs of at most 2^20 vertices): a key slot holds
-// hub << kIdxBits | row, the hub's K + 1 values sit in a compact row array
-// (one row per distinct hub, not per slot), so the 64 KB hold 4-byte key
-// slots at load factor <= 1/8 for lists of up to about 2,000 hubs instead of
-// 16-byte slots at up to 2/3: a wave's probe chain is its longest lane's,
-// and chains grow fast with the load factor.  Rows < 2^kIdxBits - 1 keep a
-// key slot from ever reading as empty (all ones).
+// hub << kIdxBits | row, and the hub's values for hop budgets 1..K sit in a
+// compact row array (one row per distinct hub, not per slot; budget 0 holds
+// only v's own hub, which pairs only through a head u > v and which the
+// tile leaves out: a list with a hub above its vertex, possible only in
+// imported label sets, takes a per-slot tile).
+// So the LDS holds 4-byte key slots at load factor <= 1/8 for lists of up to
+// about 2,000 hubs, and lists of up to 4,094 hubs at all, instead of 16-byte
+// slots at up to 2/3 and lists of up to 2,730 (longer ones went to the
+// arena, at 5x the time per label): a wave's probe chain is its longest
+// lane's, and chains grow fast with the load factor.  Rows < 2^kIdxBits - 1
+// keep a key slot from ever reading as empty (all ones).
 constexpr uint32_t kIdxBits = 12;
 constexpr uint32_t kIdxRows = (1u << kIdxBits) - 1;
-constexpr uint32_t kGroupFast = 8;             // group sizes with the unrolled check
+constexpr uint32_t kGroupFast = 4;             // group sizes with the unrolled check
 
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -213,7 +219,7 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       // an empty slot reads hub 2^20 - 1, which ranks below every head u
       // that the pair test also asks for (w < u)
       found = (k >> kIdxBits) == w;
-      return mul_u24(min(k & kIdxRows, last_row), K1);
+      return mul_u24(min(k & kIdxRows, last_row), K1 - 1);
     }
     while (k != w && k != kInf32) {
       slot = (slot + 1) & (cap - 1);
@@ -221,6 +227,17 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
     }
     found = k == w;
     return mul_u24(slot, K1);
+  };
+  // the value of row `base` for a group label of hop g and an element of hop
+  // h1 (budget x = g - h1), and whether the budget can pair (reach): per-slot
+  // rows hold budgets 0..K, indexed rows 1..K (budget 0 holds only v's hub)
+  auto value_at = [&](uint32_t base, uint32_t g, uint32_t h1, bool &reach) -> uint32_t {
+    if (kIdx) {
+      reach = g > h1;
+      return vals[base + (reach ? g - 1u - h1 : 0u)];
+    }
+    reach = g >= h1;
+    return vals[base + (reach ? g - h1 : 0u)];
   };
   const uint32_t lane = (uint32_t)lane_id();
   bool hit = false;
@@ -297,17 +314,19 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
 #pragma unroll
         for (uint32_t t = 0; t < kGroupFast; ++t) {
           if (t >= glen) break;
-          // a label shallower than h1 reads the slot's first value and is
+          // a label shallower than h1 reads the row's first value and is
           // masked; at equal hops only the hop-0 tile label (v's) pairs
-          const bool reach = gh[t] >= h1;
-          const uint32_t m = vals[base + (reach ? gh[t] - h1 : 0u)];
+          bool reach;
+          const uint32_t m = value_at(base, gh[t], h1, reach);
           hit |= pair & reach & fits(m, gd[t]);
         }
-      } else if (pair) {
+      } else {
         for (uint32_t t = 0; t < glen; ++t) {
           const unsigned long long g = Lv[p + t];
           const uint32_t hg = label_hop(g);
-          if (hg >= h1 && fits(vals[base + (hg - h1)], label_dist(g))) hit = true;
+          bool reach;
+          const uint32_t m = value_at(base, hg, h1, reach);
+          if (pair && reach && fits(m, label_dist(g))) hit = true;
         }
       }
     };
@@ -318,10 +337,9 @@ __device__ bool window_removes(uint32_t v, uint32_t K1, const unsigned long long
       uint32_t w, h1, d1;
       str.dec(l, w, h1, d1);
       if (kArena && !((w < u) & (h1 + hmin <= maxh) & (d1 <= maxd))) return;
-      bool found;
+      bool found, reach;
       const uint32_t base = find(w, found);
-      const bool reach = gh[0] >= h1;
-      const uint32_t m = vals[base + (reach ? gh[0] - h1 : 0u)];
+      const uint32_t m = value_at(base, gh[0], h1, reach);
       const bool near = sizeof(V) == 2 ? d1 + m <= gd[0]
                                        : (m != TileVal<V>::kEmpty) & (d1 + m <= gd[0]);
       hit |= found & (w < u) & reach & near;
@@ -366,6 +384,7 @@ __device__ __attribute__((always_inline)) inline bool clean_vertex(
   for (uint32_t i = tid; i < len; i += kTileThreads) {
     const uint32_t w = label_hub(Lv[i]);
     if (i > 0 && label_hub(Lv[i - 1]) == w) continue;
+    if (kIdx && w == v) continue;  // v's own hub: no head u > v pairs with it here
     uint32_t slot = hash_slot(w, cap);
     if (kIdx) {
       const uint32_t row = atomicAdd(&sh_row, 1u);
@@ -388,7 +407,12 @@ __device__ __attribute__((always_inline)) inline bool clean_vertex(
 #pragma unroll
     for (uint32_t x = 0; x <= kMaxK; ++x) {
       run = min(run, m[x]);
-      if (x < K1) vals[slot * K1 + x] = (V)min(run, (uint32_t)TileVal<V>::kEmpty);
+      const V val = (V)min(run, (uint32_t)TileVal<V>::kEmpty);
+      if (kIdx) {
+        if (x >= 1 && x < K1) vals[slot * (K1 - 1) + (x - 1)] = val;  // budgets 1..K
+      } else {
+        if (x < K1) vals[slot * K1 + x] = val;
+      }
     }
   }
   // an arena tile was written through L2: drop stale L1 lines before reading
@@ -453,8 +477,9 @@ k_clean_tile(uint32_t n, uint32_t K, const unsigned long long *__restrict__ off,
       const uint32_t cap = tile_cap_for(heads);
       // indexed: key slots at load factor <= 1/8 where the rows leave room
       uint32_t icap = 0;
-      if (indexed && heads <= kIdxRows) {
-        const uint32_t row_bytes = (heads * K1 * (uint32_t)sizeof(V) + 15u) & ~15u;
+      // (lists are hub-sorted: the last hub is the largest)
+      if (indexed && heads <= kIdxRows && label_hub(Lv[len - 1]) <= v) {
+        const uint32_t row_bytes = (heads * K * (uint32_t)sizeof(V) + 15u) & ~15u;
         if (row_bytes + 4u * cap <= lds_tile_bytes) {
           icap = pow2_at_least(8ull * heads + 4);
           while (row_bytes + 4u * icap > lds_tile_bytes) icap >>= 1;
