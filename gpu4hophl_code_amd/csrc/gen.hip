@@ -138,6 +138,7 @@ struct GenArgs {
   uint32_t row_run;      // y-blocks per row item (one tile build per item)
   uint32_t row_block;    // vertices per y-block (a multiple of 1024)
   uint32_t claim_max;    // candidates a prune wave claims at once, at most
+  uint32_t idx_tiles;    // LDS tiles of more labels than this are indexed (kInf32: none)
   uint32_t heavy_pairs;  // frontier groups with more pairs go to heavy pieces
   uint32_t piece_pairs;  // pairs per heavy piece (whole edges)
   uint32_t lds_slots;  // LDS tile capacity (slots)
@@ -971,6 +972,91 @@ __device__ inline bool probe_label(const GenArgs &a, LT l, uint32_t dy,
   return within<V>(da, v, dy);
 }
 
+// Indexed LDS tiles, for views of more than kRecSmall labels (graphs of at
+// most 2^20 vertices): key slots hold hub << kIdxBits | row (0: empty) and
+// the values sit in one row of K + 1 per distinct hub (row 0: all empty), so
+// 4-byte key slots run at a lower load factor than 16-byte records in the
+// same LDS: a wave's probe chain is its longest lane's, and at scale 20 most
+// record tiles ran at load factors of 1/4 to 1/2.  An empty slot reads hub
+// 0 and row 0, so a lookup of an absent hub 0 ends at the empty row, and no
+// mask is needed (hub 0's own entry has a row >= 1 and lies before any
+// empty slot of its chain).
+constexpr uint32_t kIdxBits = 12;
+constexpr uint32_t kIdxRowMask = (1u << kIdxBits) - 1;
+constexpr uint32_t kIdxMaxRows = kIdxRowMask;  // rows 1..4094 for hubs, 0 empty
+constexpr uint32_t kRecSmall = 511;            // records reach load factor 1/8 up to here
+struct IdxTile {
+  lds_ptr<const uint32_t> keys;
+  uint32_t rows;     // LDS byte address of row 0
+  uint32_t row_bytes;
+};
+// key slots for an indexed tile of `len` labels in `lds_bytes`, or 0 when it
+// does not fit (rows for up to len distinct hubs plus the empty row)
+__device__ inline uint32_t idx_tile_cap(uint32_t len, uint32_t row_bytes, uint32_t lds_bytes) {
+  if (len + 1 > kIdxMaxRows) return 0;
+  const uint32_t rows = ((len + 1) * row_bytes + 15u) & ~15u;
+  uint32_t cap = pow2_at_least(8ull * len + 4);
+  while (cap >= 4 && 4u * cap + rows > lds_bytes) cap >>= 1;
+  return cap >= tile_cap_for(len) ? cap : 0u;
+}
+template <class V, class LT>
+__device__ inline bool probe_label(const GenArgs &a, LT l, uint32_t dy, IdxTile t, uint32_t cap,
+                                   uint32_t) {
+  uint32_t hub, ha, da;
+  unpack(a, l, hub, ha, da);
+  uint32_t slot = hash_slot(hub, cap);
+  uint32_t k = t.keys[slot];
+  while ((k >> kIdxBits) != hub && k != 0u) {
+    slot = (slot + 1) & (cap - 1);
+    k = t.keys[slot];
+  }
+  const uint32_t addr = t.rows + mul_u24(k & kIdxRowMask, t.row_bytes) + hop_off<V>(a, l);
+  return within<V>(da, *(lds_ptr<const V>)(uintptr_t)addr, dy);
+}
+// Whole workgroup: keys[0, cap) and rows after them (rows in the same LDS);
+// sh_rows: a workgroup counter.  Claims first (a hub's first claimer takes
+// the next row and empties it), then the per-hop minima by lookup.
+template <class V>
+__device__ void build_idx_tile(const GenArgs &a, uint32_t s, const ListView &view, uint32_t *keys,
+                               uint32_t cap, uint32_t &sh_rows) {
+  const uint32_t K1 = a.K + 1;
+  V *rows = reinterpret_cast<V *>(keys + cap);
+  for (uint32_t i = threadIdx.x; i < cap; i += blockDim.x) keys[i] = 0u;
+  if (threadIdx.x == 0) sh_rows = 0;
+  for (uint32_t x = threadIdx.x; x < K1; x += blockDim.x) rows[x] = (V)TileVal<V>::kEmpty;
+  __syncthreads();
+  const unsigned long long st = a.seg_start[s];
+  for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
+    const uint32_t hub = label_hub(a.pool[st + view.at(i)]);
+    uint32_t slot = hash_slot(hub, cap);
+    for (;;) {
+      const uint32_t prev = atomicCAS(&keys[slot], 0u, (hub << kIdxBits) | kIdxRowMask);
+      if (prev == 0u) {
+        const uint32_t row = atomicAdd(&sh_rows, 1u) + 1u;
+        for (uint32_t x = 0; x < K1; ++x) rows[row * K1 + x] = (V)TileVal<V>::kEmpty;
+        keys[slot] = (hub << kIdxBits) | row;
+        break;
+      }
+      if ((prev >> kIdxBits) == hub) break;
+      slot = (slot + 1) & (cap - 1);
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
+    const unsigned long long l = a.pool[st + view.at(i)];
+    const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
+    uint32_t slot = hash_slot(hub, cap);
+    uint32_t k = keys[slot];
+    while ((k >> kIdxBits) != hub) {
+      slot = (slot + 1) & (cap - 1);
+      k = keys[slot];
+    }
+    V *vals = rows + (k & kIdxRowMask) * K1;
+    for (uint32_t ha = 0; ha + hb <= a.h; ++ha) atomic_min_v(&vals[ha], db);
+  }
+  __syncthreads();
+}
+
 // Label `lane` of a list view, clamped to it (a lane past the end reloads
 // the last label); an empty view reads the list's position 0, its self label.
 template <class LT>
@@ -1180,11 +1266,12 @@ k_prune(GenArgs a) {
   with_batch(a);
   extern __shared__ uint32_t lds[];
   __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next, sh_touch_n, sh_touch_base;
-  __shared__ uint32_t sh_item, sh_item_end;
+  __shared__ uint32_t sh_item, sh_item_end, sh_rows;
   __shared__ Item sh_acc[kAccSlots];
   __shared__ uint32_t sh_touch[kAccSlots];
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
+  const uint32_t lds_bytes = a.lds_slots << (rec_shift<V>(K1) + 2);
   Item *Fn = frontier(a, a.cur ^ 1);
   uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
   unsigned long long wave_reads = 0, tile_reads = 0;
@@ -1242,7 +1329,21 @@ k_prune(GenArgs a) {
     const uint32_t *cands = a.cgroup + a.hub_off[b];
     const ListView sv = tile_view(a, s);
     const uint32_t cap = tile_cap_for(sv.len);
-    if (cap <= a.lds_slots) {
+    const uint32_t icap = sv.len > a.idx_tiles
+                              ? idx_tile_cap(sv.len, K1 * (uint32_t)sizeof(V), lds_bytes)
+                              : 0u;
+    if (icap) {
+      if (tile_b != b) {
+        build_idx_tile<V>(a, s, sv, lds, icap, sh_rows);
+        tile_b = b;
+        tile_reads += sv.len;
+      }
+      const IdxTile it{(lds_ptr<const uint32_t>)lds,
+                       (uint32_t)(uintptr_t)(lds_ptr<const uint32_t>)(lds + icap),
+                       K1 * (uint32_t)sizeof(V)};
+      prune_candidates<V, LT>(a, b, c_beg, c_end, cands, it, icap, sh_next, sh_acc_n, sh_acc,
+                              wave_reads);
+    } else if (cap <= a.lds_slots) {
       const uint32_t lcap = lds_tile_cap(sv.len, a.lds_slots);
       if (tile_b != b) {
         build_tile<V>(a, s, sv, lds, lcap);
@@ -1293,7 +1394,7 @@ __global__ void __launch_bounds__(kPruneThreads) __attribute__((amdgpu_waves_per
 k_prune_rows(GenArgs a) {
   with_batch(a);
   extern __shared__ uint32_t lds[];
-  __shared__ uint32_t sh_b, sh_k, sh_kn, sh_q, sh_next, sh_cnt;
+  __shared__ uint32_t sh_b, sh_k, sh_kn, sh_q, sh_next, sh_cnt, sh_rows;
   __shared__ uint32_t sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base;
   __shared__ Item sh_acc[kAccSlots];
   __shared__ uint32_t sh_touch[kAccSlots];
@@ -1301,6 +1402,7 @@ k_prune_rows(GenArgs a) {
   if (!a.ctrl->row_mode) return;
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
+  const uint32_t lds_bytes = a.lds_slots << (rec_shift<V>(K1) + 2);
   Item *Fn = frontier(a, a.cur ^ 1);
   uint32_t tile_b = kInf32;  // hub slot whose tile is in LDS
   unsigned long long wave_reads = 0, tile_reads = 0, row_slots = 0;
@@ -1343,8 +1445,11 @@ k_prune_rows(GenArgs a) {
     const uint32_t s = hub_of(a, b);
     const ListView sv = tile_view(a, s);
     const uint32_t cap = tile_cap_for(sv.len);
-    const bool in_lds = cap <= a.lds_slots;
-    const uint32_t lcap = in_lds ? lds_tile_cap(sv.len, a.lds_slots) : cap;
+    const uint32_t icap = sv.len > a.idx_tiles
+                              ? idx_tile_cap(sv.len, K1 * (uint32_t)sizeof(V), lds_bytes)
+                              : 0u;
+    const bool in_lds = icap || cap <= a.lds_slots;
+    const uint32_t lcap = icap ? icap : in_lds ? lds_tile_cap(sv.len, a.lds_slots) : cap;
     if (!in_lds && rec_tile_words<V>(cap, K1) > gwords) {
       // arena too small: the host rolls the batch back, grows the arena and
       // runs the batch again
@@ -1379,7 +1484,10 @@ k_prune_rows(GenArgs a) {
       if (!built) {
         if (in_lds) {
           if (tile_b != b) {
-            build_tile<V>(a, s, sv, lds, lcap);
+            if (icap)
+              build_idx_tile<V>(a, s, sv, lds, icap, sh_rows);
+            else
+              build_tile<V>(a, s, sv, lds, lcap);
             tile_b = b;
             tile_reads += sv.len;
           }
@@ -1400,7 +1508,13 @@ k_prune_rows(GenArgs a) {
                            sh_touch_base);
         if (threadIdx.x == 0) sh_next = 0;
         __syncthreads();
-        if (in_lds)
+        if (icap)
+          prune_candidates<V, LT>(a, b, c0, c1, sh_cand,
+                                  IdxTile{(lds_ptr<const uint32_t>)lds,
+                                          (uint32_t)(uintptr_t)(lds_ptr<const uint32_t>)(lds + icap),
+                                          K1 * (uint32_t)sizeof(V)},
+                                  icap, sh_next, sh_acc_n, sh_acc, wave_reads);
+        else if (in_lds)
           prune_candidates<V, LT>(a, b, c0, c1, sh_cand, (lds_ptr<const uint32_t>)lds, lcap,
                                   sh_next, sh_acc_n, sh_acc, wave_reads);
         else
@@ -2224,6 +2338,13 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // (R-MAT scale 20 builds: 2 4.33 s, 4 4.27 s, 8 4.25 s, 16 4.24 s; scale
   // 18 the same within 1 %)
   a.claim_max = env_u32("HSDL_CLAIM_MAX", 8, 1, 64);
+  // indexed tiles need hub << kIdxBits to fit a word (tuning knob
+  // HSDL_PRUNE_INDEXED=0: records only; 2: every LDS tile indexed)
+  {
+    const uint32_t mode = (uint64_t)n <= (1ull << (32 - kIdxBits))
+                              ? env_u32("HSDL_PRUNE_INDEXED", 1, 0, 2) : 0u;
+    a.idx_tiles = mode == 0 ? kInf32 : mode == 2 ? 0u : kRecSmall;
+  }
   a.piece_pairs = env_u32("HSDL_PIECE_PAIRS", 8192, 1, 1u << 16);
   a.lds_slots = lds_slots;
   DevBuf hubs_buf, in_group;

@@ -121,6 +121,22 @@ def test_prune_label_stream_widths(monkeypatch, heavy13, stream32, slots, rows):
     same_labels(idx, want)
 
 
+@pytest.mark.parametrize("indexed", ["0", "2"])
+@pytest.mark.parametrize("rows", ["1", "2"])
+def test_prune_indexed_tiles(monkeypatch, heavy13, indexed, rows):
+    """LDS tiles of more than 511 labels are indexed by default (key slots of
+    hub << 12 | row, one row of values per hub, row 0 empty);
+    HSDL_PRUNE_INDEXED=2 indexes every LDS tile (hub 0 among the keys and
+    absent from them, chains ending at the empty row), 0 none.  With list
+    items and row items, batched and in batches of 7, the labels must not
+    change."""
+    monkeypatch.setenv("HSDL_PRUNE_INDEXED", indexed)
+    monkeypatch.setenv("HSDL_PRUNE_ROWS", rows)
+    g, want = heavy13
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4), want)
+    same_labels(H.hop_constrained_two_hop_labels_generation(g, 4, hub_batch=7), want)
+
+
 @pytest.mark.parametrize("rows,run", [("2", "1"), ("2", "64"), ("2", None), ("0", None)])
 @pytest.mark.parametrize("slots,hub_batch", [(0, 0), (64, 0), (0, 7)])
 def test_prune_row_items(monkeypatch, heavy13, rows, run, slots, hub_batch):
