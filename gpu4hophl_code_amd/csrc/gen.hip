@@ -2437,13 +2437,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // HSDL_TUNER=time tunes on batch durations (HSDL_BATCH_TARGET_US); the
   // default tunes on work, HSDL_BATCH_TARGET_WORK units (candidates plus an
   // eighth of the scanned edges) per batch, so the schedule repeats from run
-  // to run.  At 2e8 R-MAT scale 20 K=5 generates in 3.87 s against 4.12 s at
-  // 1e8 and 4.61 s at 5e7; the 1024x1024 grid is the same at all three.
+  // to run.  Builds of R-MAT scale 18 / 20 K=5: 1.6e9 0.347 / 3.13 s, 8e8
+  // 0.347 / 3.15 s, 4e8 0.346 / 3.23 s, 2e8 0.360 / 3.37 s, 1e8 0.381 /
+  // 3.60 s (larger targets reach the slots' batch cap sooner); the 1024x1024
+  // grid is the same at all of them.
   const char *tm = getenv("HSDL_TUNER");
   const bool tune_time = tm && !strcmp(tm, "time");
   const double tune_target =
       tune_time ? env_u32("HSDL_BATCH_TARGET_US", 50000, 100, 10000000) * 1e-6
-                : (double)env_u32("HSDL_BATCH_TARGET_WORK", 200000000u, 1000, 4000000000u);
+                : (double)env_u32("HSDL_BATCH_TARGET_WORK", 1600000000u, 1000, 4000000000u);
   BatchTuner tuner{plain ? 1u : std::min<uint32_t>(tune_cap, 64u), tune_cap, tune_target};
   if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
   uint64_t batches = 0, redos = 0;
@@ -2497,9 +2499,21 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       if (c.overflow & (kOvfCand | kOvfFront)) {
         const uint64_t nc = std::max<uint64_t>((uint64_t)g.a.ccap * 2, (uint64_t)c.ccnt * 5 / 4);
         const uint64_t nf = std::max<uint64_t>((uint64_t)g.a.fcap * 2, nc);
-        if (nc > 0xFFFFFFF0ull || nf > 0xFFFFFFF0ull)
-          return fail(HSDL_ENOMEM, "a round holds more than 2^32 candidates; lower hub_batch");
-        if ((rc = alloc_lists(g, (uint32_t)nc, (uint32_t)nf))) return rc;
+        const uint32_t occ = g.a.ccap, ofc = g.a.fcap;
+        const bool grown = nc <= 0xFFFFFFF0ull && nf <= 0xFFFFFFF0ull &&
+                           alloc_lists(g, (uint32_t)nc, (uint32_t)nf) == HSDL_OK;
+        if (!grown) {
+          // lists of twice the size do not fit: the batch runs again at half
+          // its hubs with lists of the old sizes, which fitted (a chosen
+          // hub_batch or a one-hub batch cannot shrink)
+          (void)hipGetLastError();
+          if (B == 1 || opts->hub_batch)
+            return fail(HSDL_ENOMEM, "a batch's candidate or frontier lists exceed the HBM "
+                                     "budget; lower hub_batch");
+          g.a.ccap = g.a.fcap = 0;  // the old buffers may be gone
+          if ((rc = alloc_lists(g, occ, ofc))) return rc;
+          tuner.B = std::max(1u, B / 2);
+        }
       }
       // the counters restart from the last completed batch (the attempt's
       // work is redone); the prune's read counters keep it, matching the
