@@ -106,6 +106,7 @@ _SIGNATURES = [
                                           ctypes.c_uint64, ctypes.c_int32, _u32p,
                                           ctypes.POINTER(ctypes.c_uint32)]),
     ("hsdl_clean", ctypes.c_int, [_vp]),
+    ("hsdl_release_scratch", ctypes.c_int, [ctypes.c_int32]),
     ("hsdl_index_stats", ctypes.c_int, [_vp, ctypes.POINTER(hsdl_stats)]),
     ("hsdl_query", ctypes.c_int, [_vp, ctypes.c_size_t, _u32p, _u32p, _i8p, _i32p]),
     ("hsdl_query_device", ctypes.c_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp]),
@@ -468,6 +469,12 @@ def hop_constrained_two_hop_labels_generation(graph, K, mode=MODE_BATCHED, hub_b
     if clean:
         idx.clean()
     return idx
+
+
+def release_scratch(device=-1):
+    """Frees the working buffers the clean keeps on a device for the next
+    clean (-1: every device)."""
+    _check(lib().hsdl_release_scratch(device))
 
 
 def hop_constrained_clean_L(index):

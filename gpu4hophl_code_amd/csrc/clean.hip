@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -708,6 +709,55 @@ __global__ void __launch_bounds__(256) k_compact(uint32_t n, const unsigned long
   }
 }
 
+// The clean's working buffers (keep flags, counts, the 32-bit copy and its
+// hop counts, the tile arena) and the index arrays a clean replaces are kept
+// per device for the next clean, sized for the largest so far: freeing and
+// allocating gigabytes inside every clean now and then cost a tenth of a
+// second there.  hsdl_release_scratch frees them.
+enum CleanScratchId : int {
+  kCsKeep, kCsNewLen, kCsRedo, kCsGtile, kCsLab32, kCsHcnt, kCsOff, kCsLab, kCsCount
+};
+constexpr int kCsDevices = 64;
+struct CleanScratch {
+  void *p[kCsCount] = {};
+  size_t bytes[kCsCount] = {};
+};
+static CleanScratch g_clean_scratch[kCsDevices];
+static std::mutex g_clean_scratch_mu;
+
+// b <- a kept buffer of at least n bytes when there is one, else a new one
+static int take_scratch(int dev, int id, DevBuf &b, size_t n) {
+  if (dev >= 0 && dev < kCsDevices) {
+    std::lock_guard<std::mutex> lk(g_clean_scratch_mu);
+    CleanScratch &c = g_clean_scratch[dev];
+    if (c.p[id] && c.bytes[id] >= n) {
+      b.release();
+      b.p = c.p[id];
+      b.bytes = c.bytes[id];
+      c.p[id] = nullptr;
+      c.bytes[id] = 0;
+      return HSDL_OK;
+    }
+    if (c.p[id]) {
+      (void)hipFree(c.p[id]);
+      c.p[id] = nullptr;
+      c.bytes[id] = 0;
+    }
+  }
+  return b.alloc(n);
+}
+// b goes back to the device's kept buffers (or is freed when the slot is taken)
+static void give_scratch(int dev, int id, DevBuf &b) {
+  if (!b.p || dev < 0 || dev >= kCsDevices) return;
+  std::lock_guard<std::mutex> lk(g_clean_scratch_mu);
+  CleanScratch &c = g_clean_scratch[dev];
+  if (c.p[id]) return;  // b frees itself
+  c.p[id] = b.p;
+  c.bytes[id] = b.bytes;
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
 int clean_index(hsdl_index *idx) {
   int rc = use_device(idx->device, nullptr);
   if (rc) return rc;
@@ -719,10 +769,22 @@ int clean_index(hsdl_index *idx) {
     hipStream_t s;
     ~G() { (void)hipStreamDestroy(s); }
   } guard{st};
-  DevBuf keep, new_len, noff, nlab, redo, ctr;
-  if ((rc = keep.alloc(std::max<uint64_t>(idx->total, 1))) || (rc = new_len.alloc(n * 4ull)) ||
-      (rc = noff.alloc((n + 1ull) * 8)) || (rc = redo.alloc(n * 4ull + 4)) ||
-      (rc = ctr.alloc(32)))
+  DevBuf keep, new_len, noff, nlab, redo, ctr, gtile, lab32, hcnt;
+  const int dev = idx->device;
+  // every buffer taken from the kept ones goes back when the clean ends
+  struct Give {
+    int dev;
+    DevBuf *b[6];
+    int id[6];
+    ~Give() {
+      for (int i = 0; i < 6; ++i) give_scratch(dev, id[i], *b[i]);
+    }
+  } give{dev, {&keep, &new_len, &redo, &gtile, &lab32, &hcnt},
+         {kCsKeep, kCsNewLen, kCsRedo, kCsGtile, kCsLab32, kCsHcnt}};
+  if ((rc = take_scratch(dev, kCsKeep, keep, std::max<uint64_t>(idx->total, 1))) ||
+      (rc = take_scratch(dev, kCsNewLen, new_len, n * 4ull)) ||
+      (rc = take_scratch(dev, kCsOff, noff, (n + 1ull) * 8)) ||
+      (rc = take_scratch(dev, kCsRedo, redo, n * 4ull + 4)) || (rc = ctr.alloc(32)))
     return rc;
   HSDL_HIP(hipMemsetAsync(ctr.p, 0, 32, st));
   int cus = 256;
@@ -758,8 +820,7 @@ int clean_index(hsdl_index *idx) {
   const uint32_t win = env_u32("HSDL_CLEAN_WINDOW", kTileWindow, 1, kMaxWindow);
   const uint32_t gcap = tile_cap_for(mx[0]);
   const uint64_t gwords = small_vals ? tile_words<uint16_t>(gcap, K1) : tile_words<uint32_t>(gcap, K1);
-  DevBuf gtile;
-  if ((rc = gtile.alloc((uint64_t)tgrid * gwords * 4))) return rc;
+  if ((rc = take_scratch(dev, kCsGtile, gtile, (uint64_t)tgrid * gwords * 4))) return rc;
   // 32-bit copy of the labels for the heads' list streams, when hub, hop and
   // distance fields fit one word (tuning knob HSDL_STREAM32=0 keeps the
   // 8-byte stream; so does a failed allocation of the copy)
@@ -767,19 +828,15 @@ int clean_index(hsdl_index *idx) {
   while ((1ull << hub_bits) <= (uint64_t)(n ? n - 1 : 0)) ++hub_bits;
   while ((1ull << hop_bits) <= (uint64_t)idx->K) ++hop_bits;
   while ((1ull << dist_bits) <= (uint64_t)mx[1]) ++dist_bits;
-  DevBuf lab32, hcnt;
   bool use32 = env_u32("HSDL_STREAM32", 1, 0, 1) && idx->total > 0 &&
                hub_bits + hop_bits + dist_bits <= 32;
   // tuning knob HSDL_CLEAN_HOP_SORT=0 keeps the 32-bit copy in the index's
   // (hub, hop, dist) order and streams whole lists
   const bool hop_sort = use32 && env_u32("HSDL_CLEAN_HOP_SORT", 1, 0, 1);
   if (use32) {
-    void *p = nullptr;
-    if (hipMalloc(&p, idx->total * 4) == hipSuccess) {
-      lab32.p = p;
-      lab32.bytes = idx->total * 4;
+    if (take_scratch(dev, kCsLab32, lab32, idx->total * 4) == HSDL_OK) {
       if (hop_sort) {
-        if ((rc = hcnt.alloc((uint64_t)n * idx->K * 4))) return rc;
+        if ((rc = take_scratch(dev, kCsHcnt, hcnt, (uint64_t)n * idx->K * 4))) return rc;
         hipLaunchKernelGGL(k_pack32_hop, dim3(cus * 8), dim3(256), 0, st, n, (uint32_t)idx->K,
                            idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
                            lab32.as<uint32_t>(), hcnt.as<uint32_t>(), hop_bits, dist_bits);
@@ -803,6 +860,7 @@ int clean_index(hsdl_index *idx) {
                        keep.as<uint8_t>(), new_len.as<uint32_t>(), redo.as<uint32_t>(), cnt,
                        lds_slots, gtile.as<uint32_t>(), gwords, win, indexed);
   };
+  const uint64_t t_kern = now_ns();  // buffers and the 32-bit copy's set-up end here
   if (small_vals) {
     if (use32) launch(k_clean_tile<uint16_t, Stream32>, s32);
     else launch(k_clean_tile<uint16_t, Stream64>, s64);
@@ -833,12 +891,13 @@ int clean_index(hsdl_index *idx) {
   uint32_t hc[6] = {0, 0, 0, 0, 0, 0};
   HSDL_HIP(hipMemcpyAsync(hc, ctr.p, 24, hipMemcpyDeviceToHost, st));
   HSDL_HIP(hipStreamSynchronize(st));
+  const uint64_t t_passes = now_ns();
   if (hc[1] & 2u) return fail(HSDL_EINVAL, "a vertex holds more than 32 labels of one hub");
   if (hc[1]) return fail(HSDL_EDEVICE, "clean: keep-bit scratch smaller than a list");
   if ((rc = scan_counts_u32(new_len.as<uint32_t>(), noff.as<uint64_t>(), n, st))) return rc;
   uint64_t total = 0;
   HSDL_HIP(hipMemcpy(&total, noff.as<uint64_t>() + n, 8, hipMemcpyDeviceToHost));
-  if ((rc = nlab.alloc(std::max<uint64_t>(total, 1) * 8))) return rc;
+  if ((rc = take_scratch(dev, kCsLab, nlab, std::max<uint64_t>(total, 1) * 8))) return rc;
   DevBuf npar;
   if (idx->has_parents && (rc = npar.alloc(std::max<uint64_t>(total, 1) * 4))) return rc;
   hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, st, n, idx->off.as<unsigned long long>(),
@@ -848,8 +907,11 @@ int clean_index(hsdl_index *idx) {
                      idx->has_parents ? npar.as<uint32_t>() : nullptr);
   HSDL_HIP(hipGetLastError());
   HSDL_HIP(hipStreamSynchronize(st));
-  idx->off = std::move(noff);
-  idx->lab = std::move(nlab);
+  // the replaced arrays are kept for the next clean's new ones
+  std::swap(idx->off, noff);
+  std::swap(idx->lab, nlab);
+  give_scratch(dev, kCsOff, noff);
+  give_scratch(dev, kCsLab, nlab);
   if (idx->has_parents) idx->par = std::move(npar);
   idx->total = total;
   idx->cleaned = true;
@@ -858,7 +920,11 @@ int clean_index(hsdl_index *idx) {
   idx->stats.clean_seq_vertices = hc[0];
   idx->stats.clean_label_reads = (uint64_t)hc[4] | ((uint64_t)hc[5] << 32);
   idx->stats.clean_label_bytes = use32 ? 4 : 8;
-  idx->stats.t_clean_s = (now_ns() - t0) * 1e-9;
+  const uint64_t t_end = now_ns();
+  idx->stats.t_clean_s = (t_end - t0) * 1e-9;
+  if (getenv("HSDL_VERBOSE"))
+    fprintf(stderr, "[hsdl] clean: %.3f s set-up, %.3f s passes, %.3f s compaction\n",
+            (t_kern - t0) * 1e-9, (t_passes - t_kern) * 1e-9, (t_end - t_passes) * 1e-9);
   return HSDL_OK;
 }
 
@@ -871,4 +937,29 @@ static int clean_body(hsdl_index *idx) {
 
 extern "C" int hsdl_clean(hsdl_index *idx) {
   return hsdl::guarded([&] { return clean_body(idx); });
+}
+
+static int release_scratch_body(int32_t device) {
+  std::lock_guard<std::mutex> lk(hsdl::g_clean_scratch_mu);
+  int cur = -1;
+  (void)hipGetDevice(&cur);
+  for (int d = 0; d < hsdl::kCsDevices; ++d) {
+    if (device >= 0 && d != device) continue;
+    hsdl::CleanScratch &c = hsdl::g_clean_scratch[d];
+    bool any = false;
+    for (int i = 0; i < hsdl::kCsCount; ++i) any |= c.p[i] != nullptr;
+    if (!any) continue;
+    (void)hipSetDevice(d);
+    for (int i = 0; i < hsdl::kCsCount; ++i) {
+      if (c.p[i]) (void)hipFree(c.p[i]);
+      c.p[i] = nullptr;
+      c.bytes[i] = 0;
+    }
+  }
+  if (cur >= 0) (void)hipSetDevice(cur);
+  return HSDL_OK;
+}
+
+extern "C" int hsdl_release_scratch(int32_t device) {
+  return hsdl::guarded([&] { return release_scratch_body(device); });
 }

@@ -39,6 +39,13 @@ struct hsdl_graph {
   // build starts there instead of rolling batches back to grow them again
   uint32_t hint_ccap = 0, hint_fcap = 0;
   uint64_t hint_arena = 0;
+  // the generation's other working buffers (per-vertex state, candidate and
+  // frontier lists, tile arena, re-sort slices), kept for the next build on
+  // this graph: freeing and allocating gigabytes inside every build now and
+  // then cost up to a second there
+  static constexpr int kScratch = 24;
+  void *scratch_p[kScratch] = {};
+  size_t scratch_bytes[kScratch] = {};
   int d_device = -1;
   void (*d_free)(hsdl_graph *) = nullptr;
   ~hsdl_graph() {

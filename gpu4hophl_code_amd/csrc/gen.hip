@@ -1812,6 +1812,11 @@ static void free_device_graph(hsdl_graph *g) {
   if (g->d_slots) (void)hipFree(g->d_slots);
   if (g->d_pool) (void)hipFree(g->d_pool);
   if (g->d_pool32) (void)hipFree(g->d_pool32);
+  for (int i = 0; i < hsdl_graph::kScratch; ++i) {
+    if (g->scratch_p[i]) (void)hipFree(g->scratch_p[i]);
+    g->scratch_p[i] = nullptr;
+    g->scratch_bytes[i] = 0;
+  }
   if (cur >= 0 && cur != g->d_device) (void)hipSetDevice(cur);
   g->d_off = g->d_adj = g->d_slots = g->d_pool = g->d_pool32 = nullptr;
   g->d_slots_bytes = g->d_pool_bytes = g->d_pool32_bytes = 0;
@@ -1843,8 +1848,17 @@ static int upload_graph(hsdl_graph *g, int dev) {
   return HSDL_OK;
 }
 
+// Slots of hsdl_graph::scratch for Gen's buffers.
+enum ScratchId : int {
+  kScSegStart, kScSegLen, kScSegCap, kScAddCnt, kScAddCur, kScTouched, kScFgrp, kScHubSlot,
+  kScSortedLen, kScHcnt, kScSortTmp, kScSortTmp32, kScCflat, kScCgroup, kScHubCcnt, kScHubOff,
+  kScHubCur, kScChunkOff, kScF0, kScF1, kScGtile, kScCtrl, kScHeavy, kScCount
+};
+static_assert(kScCount <= hsdl_graph::kScratch, "scratch slots");
+
 struct Gen {
   GenArgs a{};
+  hsdl_graph *mg = nullptr;  // the graph whose scratch the buffers come from and go back to
   DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched, fgrp, hub_slot;
   DevBuf sorted_len, hcnt, sort_tmp, sort_tmp32;  // hop zones
   uint32_t sort_per_wave = 0;                     // labels per wave slice of sort_tmp
@@ -1875,10 +1889,45 @@ struct Gen {
     return fixed + pool.bytes + pool32.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes +
            gtile.bytes + sort_tmp.bytes + sort_tmp32.bytes;
   }
+  DevBuf *by_id(int id) {
+    DevBuf *t[kScCount] = {&seg_start, &seg_len, &seg_cap, &add_cnt, &add_cur, &touched, &fgrp,
+                           &hub_slot, &sorted_len, &hcnt, &sort_tmp, &sort_tmp32, &cflat, &cgroup,
+                           &hub_ccnt, &hub_off, &hub_cur, &chunk_off, &F0, &F1, &gtile, &ctrl,
+                           &heavy};
+    return t[id];
+  }
+  // Buffer `id` of at least n bytes: the graph's kept one when it is large
+  // enough, else a new one (the kept one is freed).
+  int take(int id, size_t n) {
+    DevBuf &b = *by_id(id);
+    if (mg && mg->scratch_p[id] && mg->scratch_bytes[id] >= n) {
+      b.release();
+      b.p = mg->scratch_p[id];
+      b.bytes = mg->scratch_bytes[id];
+      mg->scratch_p[id] = nullptr;
+      mg->scratch_bytes[id] = 0;
+      return HSDL_OK;
+    }
+    if (mg && mg->scratch_p[id]) {
+      (void)hipFree(mg->scratch_p[id]);
+      mg->scratch_p[id] = nullptr;
+      mg->scratch_bytes[id] = 0;
+    }
+    return b.alloc(n);
+  }
   ~Gen() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (hipGraphExec_t x : gexec)
       if (x) (void)hipGraphExecDestroy(x);
+    // the buffers go back to the graph for its next build
+    for (int id = 0; mg && id < kScCount; ++id) {
+      DevBuf &b = *by_id(id);
+      if (!b.p || mg->scratch_p[id]) continue;
+      mg->scratch_p[id] = b.p;
+      mg->scratch_bytes[id] = b.bytes;
+      b.p = nullptr;
+      b.bytes = 0;
+    }
   }
 };
 
@@ -1928,7 +1977,7 @@ static int grow_gtile(Gen &g) {
   if (g.held() - g.gtile.bytes + nw * 4 > g.budget)
     return fail(HSDL_ENOMEM, "tile arena exceeds the HBM budget");
   g.gtile.release();
-  int rc = g.gtile.alloc(nw * 4);
+  int rc = g.take(kScGtile, nw * 4);
   if (rc) return rc;
   g.a.gtile = g.gtile.as<uint32_t>();
   g.a.gtile_cap = nw;
@@ -1942,7 +1991,7 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
     g.cgroup.release();
     if (g.held() + (uint64_t)ccap * 12 > g.budget)
       return fail(HSDL_ENOMEM, "candidate lists exceed the HBM budget");
-    if ((rc = g.cflat.alloc((uint64_t)ccap * 8)) || (rc = g.cgroup.alloc((uint64_t)ccap * 4)))
+    if ((rc = g.take(kScCflat, (uint64_t)ccap * 8)) || (rc = g.take(kScCgroup, (uint64_t)ccap * 4)))
       return rc;
     g.a.cflat = g.cflat.as<unsigned long long>();
     g.a.cgroup = g.cgroup.as<uint32_t>();
@@ -1953,8 +2002,8 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
     g.F1.release();
     if (g.held() + (uint64_t)fcap * 2 * sizeof(Item) > g.budget)
       return fail(HSDL_ENOMEM, "frontier lists exceed the HBM budget");
-    if ((rc = g.F0.alloc((uint64_t)fcap * sizeof(Item))) ||
-        (rc = g.F1.alloc((uint64_t)fcap * sizeof(Item))))
+    if ((rc = g.take(kScF0, (uint64_t)fcap * sizeof(Item))) ||
+        (rc = g.take(kScF1, (uint64_t)fcap * sizeof(Item))))
       return rc;
     g.a.F0 = g.F0.as<Item>();
     g.a.F1 = g.F1.as<Item>();
@@ -2088,8 +2137,8 @@ static int sort_zones(Gen &g, uint32_t maxlen, uint32_t resort_div) {
       HSDL_HIP(hipStreamSynchronize(g.st));
       g.sort_tmp.release();
       g.sort_tmp32.release();
-      int rc = g.sort_tmp.alloc(waves * per * 8);
-      if (!rc && p32) rc = g.sort_tmp32.alloc(waves * per * 4);
+      int rc = g.take(kScSortTmp, waves * per * 8);
+      if (!rc && p32) rc = g.take(kScSortTmp32, waves * per * 4);
       if (rc) return rc;
       g.sort_per_wave = per;
     }
@@ -2171,6 +2220,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // count as free: this build reuses or replaces them
   hsdl_graph *mg = const_cast<hsdl_graph *>(hg);
   free_b += mg->d_slots_bytes + mg->d_pool_bytes + mg->d_pool32_bytes;
+  for (int id = 0; id < kScCount; ++id) free_b += mg->scratch_bytes[id];
+  g.mg = mg;
   g.budget = opts->hbm_budget ? opts->hbm_budget : (uint64_t)(free_b * 0.8);
   const uint32_t c0 = 16;  // initial label slots per vertex
   uint64_t pool0 = opts->pool_init_labels
@@ -2263,15 +2314,15 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   if (g.budget < g.fixed + pool0 * per_label + gtile0 * 4 + (uint64_t)list0 * 36)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
-  if ((!g.pool.p && (rc = g.pool.alloc(pool0 * 8))) || (rc = g.seg_start.alloc(n * 8ull)) ||
-      (rc = g.seg_len.alloc(n * 4ull)) || (rc = g.seg_cap.alloc(n * 4ull)) ||
-      (rc = g.add_cnt.alloc(n * 4ull)) || (rc = g.add_cur.alloc(n * 4ull)) ||
-      (rc = g.touched.alloc(n * 4ull)) || (rc = g.fgrp.alloc(n * 16ull)) ||
-      (group && (rc = g.hub_slot.alloc(n * 4ull))) ||
-      (rc = g.hub_ccnt.alloc((Bmax + 1ull) * 4)) || (rc = g.hub_off.alloc((Bmax + 1ull) * 4)) ||
-      (rc = g.hub_cur.alloc((Bmax + 1ull) * 4)) || (rc = g.chunk_off.alloc((Bmax + 1ull) * 4)) ||
-      (rc = g.gtile.alloc(gtile0 * 4)) || (rc = g.ctrl.alloc(sizeof(Ctrl))) ||
-      (rc = g.heavy.alloc((uint64_t)heavy_cap * sizeof(HeavyItem))) ||
+  if ((!g.pool.p && (rc = g.pool.alloc(pool0 * 8))) || (rc = g.take(kScSegStart, n * 8ull)) ||
+      (rc = g.take(kScSegLen, n * 4ull)) || (rc = g.take(kScSegCap, n * 4ull)) ||
+      (rc = g.take(kScAddCnt, n * 4ull)) || (rc = g.take(kScAddCur, n * 4ull)) ||
+      (rc = g.take(kScTouched, n * 4ull)) || (rc = g.take(kScFgrp, n * 16ull)) ||
+      (group && (rc = g.take(kScHubSlot, n * 4ull))) ||
+      (rc = g.take(kScHubCcnt, (Bmax + 1ull) * 4)) || (rc = g.take(kScHubOff, (Bmax + 1ull) * 4)) ||
+      (rc = g.take(kScHubCur, (Bmax + 1ull) * 4)) || (rc = g.take(kScChunkOff, (Bmax + 1ull) * 4)) ||
+      (rc = g.take(kScGtile, gtile0 * 4)) || (rc = g.take(kScCtrl, sizeof(Ctrl))) ||
+      (rc = g.take(kScHeavy, (uint64_t)heavy_cap * sizeof(HeavyItem))) ||
       (use32 && !g.pool32.p && (rc = g.pool32.alloc(pool0 * 4))))
     return rc;
   {
@@ -2299,8 +2350,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // sorted zone (HSDL_RESORT_DIV; R-MAT scale 18 / 20 builds: 2 0.551 /
   // 5.34 s, 4 0.463 / 4.42 s, 8 0.471 / 4.49 s, 16 0.473 / 4.66 s)
   const uint32_t resort_div = env_u32("HSDL_RESORT_DIV", 4, 1, 1u << 20);
-  if (zones && ((rc = g.sorted_len.alloc(n * 4ull)) ||
-                (rc = g.hcnt.alloc((uint64_t)n * (K - 1) * 4))))
+  if (zones && ((rc = g.take(kScSortedLen, n * 4ull)) ||
+                (rc = g.take(kScHcnt, (uint64_t)n * (K - 1) * 4))))
     return rc;
 
   // LDS tile per workgroup: 64 KB; 16-bit distances when they fit
@@ -2449,6 +2500,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   BatchTuner tuner{plain ? 1u : std::min<uint32_t>(tune_cap, 64u), tune_cap, tune_target};
   if (opts->hub_batch) tuner.B = std::min(opts->hub_batch, Bmax);
   uint64_t batches = 0, redos = 0;
+  const uint64_t t_loop = now_ns();  // set-up (buffers, lists) ends here
   uint32_t s0 = 0;
   Ctrl c{}, c_ok{};  // c_ok: counters after the last batch that completed
   // attempt tags count down from tag_top; when they run out, the rows used
@@ -2577,12 +2629,23 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   mg->slot_tag_next = tag_next;
   mg->slot_rows_used = rows_used;
   const uint64_t t_gen = now_ns();
-  // release batch buffers before the export needs its own
-  g.cflat.release();
-  g.cgroup.release();
-  g.F0.release();
-  g.F1.release();
-  g.gtile.release();
+  // the batch buffers stay for the graph's next build unless the export
+  // (labels, and the sort's double buffer and scratch) needs their memory
+  {
+    uint64_t total_est = 0;
+    if (hipMemcpy(&total_est, &g.ctrl.as<Ctrl>()->pool_top, 8, hipMemcpyDeviceToHost) !=
+        hipSuccess)
+      return fail(HSDL_EDEVICE, "reading the pool top failed");
+    size_t fb = 0, tb = 0;
+    HSDL_HIP(hipMemGetInfo(&fb, &tb));
+    if ((uint64_t)fb < total_est * 24 + (1ull << 30)) {
+      g.cflat.release();
+      g.cgroup.release();
+      g.F0.release();
+      g.F1.release();
+      g.gtile.release();
+    }
+  }
 
   // final CSR sorted per vertex
   hsdl_index *idx = new hsdl_index();
@@ -2619,9 +2682,11 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   const uint64_t t_end = now_ns();
   idx->stats.t_parents_s = (t_end - t_sorted) * 1e-9;
   if (g.verbose)
-    fprintf(stderr, "[hsdl] generate: %.3f s batches (%llu, %llu redo; host enqueue %.3f s, wait %.3f s, "
-            "grow %.3f s; %llu graph captures), %.3f s export (%.3f s sort), %llu labels\n",
-            (t_gen - t_csr) * 1e-9, (unsigned long long)batches, (unsigned long long)redos,
+    fprintf(stderr, "[hsdl] generate: %.3f s set-up, %.3f s batches (%llu, %llu redo; host enqueue "
+            "%.3f s, wait %.3f s, grow %.3f s; %llu graph captures), %.3f s export (%.3f s sort), "
+            "%llu labels\n",
+            (t_loop - t_csr) * 1e-9, (t_gen - t_loop) * 1e-9, (unsigned long long)batches,
+            (unsigned long long)redos,
             g.h_enqueue, g.h_wait, g.h_grow, (unsigned long long)g.graph_captures,
             (t_end - t_gen) * 1e-9, (t_end - t_sort0) * 1e-9, (unsigned long long)total);
   mg->hint_ccap = g.a.ccap;

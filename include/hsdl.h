@@ -192,6 +192,13 @@ int hsdl_graph_groups(const hsdl_graph *g, uint32_t method, uint32_t max_size, u
  * Clean_pc/src/gpu_clean.cu:140). */
 int hsdl_clean(hsdl_index *idx);
 
+/* Frees the working buffers hsdl_clean keeps on a device (-1: every device)
+ * for the next clean: its flags, counts, 32-bit label copy and tile arena,
+ * and the index arrays a clean replaced.  No counterpart in the reference
+ * (memory management of this library); the graph's build buffers go with
+ * hsdl_graph_release_device. */
+int hsdl_release_scratch(int32_t device);
+
 /* Build statistics of the last generate/clean on this index. */
 typedef struct hsdl_stats {
   uint64_t labels_uncleaned;

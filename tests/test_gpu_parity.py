@@ -325,6 +325,29 @@ def test_clean_removes_exactly_the_redundant_labels(monkeypatch, stream32, index
         assert idx.stats()["labels_cleaned"] == want.total()
 
 
+def test_clean_scratch_kept_between_cleans():
+    """The clean keeps its working buffers and the arrays it replaces for the
+    next clean on the device (taken when large enough, else replaced), and
+    the build keeps its own with the graph: cleans of a larger, a smaller and
+    a larger redundant index again, then after hsdl_release_scratch, and
+    repeated builds, all give the oracle's labels."""
+    cases = [(11, 4, 64, 1), (9, 3, 16, 2), (11, 4, 64, 1)]
+    for i, (scale, K, group, seed) in enumerate(cases + cases[:1]):
+        if i == 3:
+            H.release_scratch()
+        g = H.Graph.rmat(scale, 8, 2000 + seed, 1, 20)
+        c = csr_of(g)
+        red = O.kround_grouped(c, K, group)
+        off, hub, hop, dist = red.arrays()
+        idx = H.Index.from_labels(c.n, K, off.astype(np.uint64), hub, hop, dist)
+        idx.clean()
+        same_labels(idx, O.clean(red).arrays())
+        want = O.clean(O.kround(c, K)).arrays()
+        for _ in range(2):
+            same_labels(H.hop_constrained_two_hop_labels_generation(g, K, clean=True), want)
+    H.release_scratch(0)
+
+
 def test_queries_special_cases_and_bf():
     g = H.Graph.rmat(10, 8, 77, 1, 100)
     c = csr_of(g)
