@@ -910,6 +910,8 @@ int clean_index(hsdl_index *idx) {
   // the replaced arrays are kept for the next clean's new ones
   std::swap(idx->off, noff);
   std::swap(idx->lab, nlab);
+  idx->lab32.release();
+  idx->q32 = 0;
   give_scratch(dev, kCsOff, noff);
   give_scratch(dev, kCsLab, nlab);
   if (idx->has_parents) idx->par = std::move(npar);

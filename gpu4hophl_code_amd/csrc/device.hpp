@@ -219,6 +219,13 @@ struct hsdl_index {
   bool has_parents = false;
   bool cleaned = false;
   hsdl_stats stats{};
+  uint32_t dist_max = hsdl::kMaxDist;  // bound on every label's dist
+  // Query copy of `lab`: hub << (hop_bits + dist_bits) | hop << dist_bits |
+  // dist in 32 bits (same per-vertex order), made by the first query call that
+  // may allocate when the fields fit; released whenever `lab` changes.
+  // q32: 0 not made yet, 1 made, 2 does not fit or switched off.
+  mutable hsdl::DevBuf lab32;
+  mutable uint32_t q32 = 0, q32_hop_bits = 0, q32_dist_bits = 0;
 };
 
 namespace hsdl {
@@ -236,7 +243,12 @@ int scan_counts_u64(const uint64_t *d_cnt, uint64_t *d_off, uint32_t n, hipStrea
 int derive_parents(hsdl_index *idx, const hsdl_graph *g, hipStream_t st);
 
 // Query launch used by hsdl_query_device and the server.
+// may_alloc: the call may make the index's 32-bit query copy (device memory
+// of 4 B per label, kept with the index); without it the query reads the
+// copy only when an earlier call made it.
 int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,
-                 const int8_t *d_k, int32_t *d_out, hipStream_t st);
+                 const int8_t *d_k, int32_t *d_out, hipStream_t st, bool may_alloc = true);
+// Makes the index's 32-bit query copy now, when it fits (synchronous).
+int prepare_query(const hsdl_index *idx);
 
 }  // namespace hsdl

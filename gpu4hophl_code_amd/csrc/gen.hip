@@ -2693,6 +2693,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   mg->hint_fcap = g.a.fcap;
   mg->hint_arena = g.a.gtile_cap;
   idx->stats.labels_uncleaned = total;
+  idx->dist_max = (uint32_t)std::min<uint64_t>(kMaxDist, (uint64_t)K * std::max(hg->max_w, 1));
   idx->stats.t_generate_s = (t_end - t_csr) * 1e-9;
   idx->stats.t_sort_s = (t_sorted - t_sort0) * 1e-9;
   idx->stats.hub_batch = tuner.B;

@@ -138,11 +138,13 @@ static int index_from_labels_body(uint32_t n, uint8_t K, const uint64_t *off, co
   for (uint32_t v = 0; v < n; ++v)
     if (off[v + 1] < off[v]) return fail(HSDL_EINVAL, "offsets must be non-decreasing");
   std::vector<uint64_t> lab(total);
+  uint32_t dist_max = 0;
   for (uint64_t i = 0; i < total; ++i) {
     if (hub[i] >= n) return fail(HSDL_EINVAL, "hub id out of range");
     if (hop[i] > K) return fail(HSDL_ERANGE, "hop above K");
     if (dist[i] < 0 || (uint32_t)dist[i] > kMaxDist) return fail(HSDL_ERANGE, "dist outside 0..2^27-1");
     lab[i] = pack_label(hub[i], hop[i], (uint32_t)dist[i]);
+    dist_max = std::max(dist_max, (uint32_t)dist[i]);
   }
   // per-vertex (hub, hop, dist) order; one label per (hub, hop)
   for (uint32_t v = 0; v < n; ++v) {
@@ -162,6 +164,7 @@ static int index_from_labels_body(uint32_t n, uint8_t K, const uint64_t *off, co
   idx->n = n;
   idx->K = K;
   idx->total = total;
+  idx->dist_max = dist_max;
   if ((rc = idx->off.alloc(sizeof(uint64_t) * (n + 1ull))) ||
       (rc = idx->lab.alloc(sizeof(uint64_t) * std::max<uint64_t>(total, 1)))) {
     delete idx;

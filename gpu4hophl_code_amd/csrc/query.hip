@@ -78,10 +78,29 @@ __device__ inline uint64_t load_ranges(uint32_t lane, const unsigned long long *
   return v;
 }
 
+// Label formats the kernel reads: the index's 8-byte labels, or its 32-bit
+// query copy (hub << (hop_bits + dist_bits) | hop << dist_bits | dist, same
+// per-vertex order), which halves the bytes, the registers of the label
+// prefetch and the LDS of the staging.
+struct Lab64 {
+  using T = unsigned long long;
+  __device__ uint32_t hub(T l) const { return label_hub(l); }
+  __device__ uint32_t hop(T l) const { return label_hop(l); }
+  __device__ uint32_t dist(T l) const { return label_dist(l); }
+};
+struct Lab32 {
+  using T = uint32_t;
+  uint32_t db, hsh, hmask, dmask;  // dist bits, hop + dist bits, field masks
+  __device__ uint32_t hub(T l) const { return l >> hsh; }
+  __device__ uint32_t hop(T l) const { return (l >> db) & hmask; }
+  __device__ uint32_t dist(T l) const { return l & dmask; }
+};
+
 // A query after its (s,t,k) and label offsets are known; uniform over the
 // workgroup.  fixed >= 0 is an answer that needs no join.
+template <class T>
 struct QJob {
-  const unsigned long long *A, *B;
+  const T *A, *B;
   uint32_t ls, lt, kk;
   int32_t fixed;
 };
@@ -90,9 +109,10 @@ __device__ inline bool needs_join(uint32_t n, uint32_t s, uint32_t t, int k) {
   return k > 0 && s < n && t < n && s != t;
 }
 
-__device__ inline QJob make_job(uint32_t n, const unsigned long long *lab, uint32_t s, uint32_t t,
-                                int k, uint64_t sb, uint64_t se, uint64_t tb, uint64_t te) {
-  QJob j;
+template <class T>
+__device__ inline QJob<T> make_job(uint32_t n, const T *lab, uint32_t s, uint32_t t, int k,
+                                   uint64_t sb, uint64_t se, uint64_t tb, uint64_t te) {
+  QJob<T> j;
   j.A = j.B = lab;
   j.ls = j.lt = 0;
   j.kk = 0;
@@ -101,9 +121,9 @@ __device__ inline QJob make_job(uint32_t n, const unsigned long long *lab, uint3
     return j;
   }
   uint32_t ls = (uint32_t)(se - sb), lt = (uint32_t)(te - tb);
-  const unsigned long long *A = lab + sb, *B = lab + tb;
+  const T *A = lab + sb, *B = lab + tb;
   if (ls > lt) {  // A = shorter list, B = longer list
-    const unsigned long long *T = A; A = B; B = T;
+    const T *X = A; A = B; B = X;
     const uint32_t tl = ls; ls = lt; lt = tl;
   }
   j.A = A; j.B = B; j.ls = ls; j.lt = lt; j.kk = (uint32_t)k;
@@ -111,17 +131,31 @@ __device__ inline QJob make_job(uint32_t n, const unsigned long long *lab, uint3
   return j;
 }
 
-__device__ inline bool merge_job(const QJob &j, uint32_t merge_cap) {
+template <class T>
+__device__ inline bool merge_job(const QJob<T> &j, uint32_t merge_cap) {
   return j.fixed < 0 && j.ls + j.lt <= merge_cap;
 }
 
-__global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned long long *off,
-                                                     const unsigned long long *lab, uint64_t nq,
+// LDS of a workgroup: kStageWords labels, then (32-bit labels) the four
+// wave minima, which the 8-byte form keeps in its last 16 bytes instead.
+template <class F>
+struct QStage {
+  static constexpr uint32_t kLabels = kStageWords + (sizeof(typename F::T) == 8 ? 0u : 4u);
+  static constexpr uint32_t kMinAt = sizeof(typename F::T) == 8 ? kMergeCap : kStageWords;
+  // hubs of B on the long path (u32 words)
+  static constexpr uint32_t kHubs = sizeof(typename F::T) == 8 ? kLongHubs : kStageWords;
+};
+
+template <class F>
+__global__ void __launch_bounds__(kQThreads) k_query(F fmt, uint32_t n,
+                                                     const unsigned long long *off,
+                                                     const typename F::T *lab, uint64_t nq,
                                                      const uint32_t *qs, const uint32_t *qt,
                                                      const int8_t *qk, int32_t *out,
                                                      uint32_t merge_cap, uint32_t lds_hubs) {
-  __shared__ unsigned long long stage[kStageWords];
-  uint32_t *wave_min = reinterpret_cast<uint32_t *>(stage + kMergeCap);
+  using LT = typename F::T;
+  __shared__ LT stage[QStage<F>::kLabels];
+  uint32_t *wave_min = reinterpret_cast<uint32_t *>(stage + QStage<F>::kMinAt);
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const uint64_t G = gridDim.x;
   uint64_t q = blockIdx.x;
@@ -132,15 +166,15 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
   uint32_t s = lane_u32(pv, 0), t = lane_u32(pv, 1);
   int k = (int)lane_u32(pv, 2);
   uint64_t rv = needs_join(n, s, t, k) ? load_ranges(lane, off, s, t) : 0;
-  QJob cur = make_job(n, lab, s, t, k, lane_u64(rv, 0), lane_u64(rv, 1), lane_u64(rv, 2),
-                      lane_u64(rv, 3));
+  QJob<LT> cur = make_job(n, lab, s, t, k, lane_u64(rv, 0), lane_u64(rv, 1), lane_u64(rv, 2),
+                         lane_u64(rv, 3));
   pv = q + G < nq ? load_params(lane, q + G, qs, qt, qk) : 0;
   uint32_t s1 = lane_u32(pv, 0), t1 = lane_u32(pv, 1);
   int k1 = q + G < nq ? (int)lane_u32(pv, 2) : -1;
   if (needs_join(n, s1, t1, k1)) rv = load_ranges(lane, off, s1, t1);
   if (q + 2 * G < nq) pv = load_params(lane, q + 2 * G, qs, qt, qk);
   // labels of q
-  unsigned long long v[kLoadPerLane];
+  LT v[kLoadPerLane];
   if (merge_job(cur, merge_cap)) {
     const uint32_t total = cur.ls + cur.lt;
 #pragma unroll
@@ -162,7 +196,7 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
     const uint32_t tbits = ls ? max(6u, 32u - (uint32_t)__builtin_clz(2 * ls - 1)) : 6u;
     const uint32_t tmask = (1u << tbits) - 1u;
     uint32_t *T = reinterpret_cast<uint32_t *>(stage + ls);
-    unsigned long long vc[kLoadPerLane];  // this query's labels; v takes the next one's
+    LT vc[kLoadPerLane];  // this query's labels; v takes the next one's
     if (is_merge) {
 #pragma unroll
       for (uint32_t u = 0; u < kLoadPerLane; ++u) {
@@ -175,8 +209,8 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
     __syncthreads();
     // ---- prefetch: labels of q+G, ranges of q+2G, (s,t,k) of q+3G
     const bool has1 = q + G < nq;
-    const QJob nxt = make_job(n, lab, s1, t1, k1, lane_u64(rv, 0), lane_u64(rv, 1),
-                              lane_u64(rv, 2), lane_u64(rv, 3));
+    const QJob<LT> nxt = make_job(n, lab, s1, t1, k1, lane_u64(rv, 0), lane_u64(rv, 1),
+                                 lane_u64(rv, 2), lane_u64(rv, 3));
     const bool has2 = q + 2 * G < nq;
     const uint32_t s2 = lane_u32(pv, 0), t2 = lane_u32(pv, 1);
     const int k2 = has2 ? (int)lane_u32(pv, 2) : -1;
@@ -196,11 +230,11 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
     uint32_t best = kInf32;
     const uint32_t kk = cur.kk;
     if (is_merge) {
-      const unsigned long long *SA = stage;
+      const LT *SA = stage;
       // insert the group heads of A (hubs are unique among heads)
       for (uint32_t i = tid; i < ls; i += kQThreads) {
-        const uint32_t w = label_hub(SA[i]);
-        if (i > 0 && label_hub(SA[i - 1]) == w) continue;
+        const uint32_t w = fmt.hub(SA[i]);
+        if (i > 0 && fmt.hub(SA[i - 1]) == w) continue;
         uint32_t slot = (w * 0x9E3779B1u) >> (32 - tbits);
         for (;;) {
           uint32_t *wp = &T[slot >> 1];
@@ -223,18 +257,18 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
       for (uint32_t u = 0; u < kLoadPerLane; ++u) {
         const uint32_t i = u * kQThreads + tid;
         if (i < ls || i >= total) continue;
-        const unsigned long long b = vc[u];
-        const uint32_t w = label_hub(b), h2 = label_hop(b), d2 = label_dist(b);
+        const LT b = vc[u];
+        const uint32_t w = fmt.hub(b), h2 = fmt.hop(b), d2 = fmt.dist(b);
         if (h2 > kk) continue;
         uint32_t slot = (w * 0x9E3779B1u) >> (32 - tbits);
         for (;;) {
           const uint32_t e = (T[slot >> 1] >> ((slot & 1u) << 4)) & 0xFFFFu;
           if (e == 0xFFFFu) break;
-          if (label_hub(SA[e]) == w) {
+          if (fmt.hub(SA[e]) == w) {
             for (uint32_t p = e; p < ls; ++p) {
-              const unsigned long long a = SA[p];
-              if (label_hub(a) != w || label_hop(a) + h2 > kk) break;
-              const uint32_t dsum = label_dist(a) + d2;
+              const LT a = SA[p];
+              if (fmt.hub(a) != w || fmt.hop(a) + h2 > kk) break;
+              const uint32_t dsum = fmt.dist(a) + d2;
               best = dsum < best ? dsum : best;
             }
             break;
@@ -244,26 +278,26 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
       }
     } else if (cur.fixed < 0) {
       // long path: LDS skip index of B's hubs (stride 1 up to lds_hubs)
-      const unsigned long long *A = cur.A, *B = cur.B;
+      const LT *A = cur.A, *B = cur.B;
       const uint32_t ls = cur.ls, lt = cur.lt;
       uint32_t *hubs = reinterpret_cast<uint32_t *>(stage);
       const uint32_t stride = (lt + lds_hubs - 1) / lds_hubs;
       const uint32_t nsm = (lt + stride - 1) / stride;
-      for (uint32_t i = tid; i < nsm; i += kQThreads) hubs[i] = label_hub(B[(uint64_t)i * stride]);
+      for (uint32_t i = tid; i < nsm; i += kQThreads) hubs[i] = fmt.hub(B[(uint64_t)i * stride]);
       __syncthreads();
       for (uint32_t j = tid; j < ls; j += kQThreads) {
-        const unsigned long long a = A[j];
-        const uint32_t w = label_hub(a), h1 = label_hop(a), d1 = label_dist(a);
+        const LT a = A[j];
+        const uint32_t w = fmt.hub(a), h1 = fmt.hop(a), d1 = fmt.dist(a);
         if (h1 > kk) continue;
         // first sample >= w, then the first position >= w inside its block
         const uint32_t si = lower_bound_by([&](uint32_t x) { return hubs[x]; }, nsm, w);
         const uint32_t lo = si == 0 ? 0 : (si - 1) * stride + 1;
         const uint32_t hi = si < nsm ? si * stride : lt;
-        uint32_t p = lo + lower_bound_by([&](uint32_t x) { return label_hub(B[lo + x]); }, hi - lo, w);
+        uint32_t p = lo + lower_bound_by([&](uint32_t x) { return fmt.hub(B[lo + x]); }, hi - lo, w);
         for (; p < lt; ++p) {
-          const unsigned long long b = B[p];
-          if (label_hub(b) != w || h1 + label_hop(b) > kk) break;  // hops ascend within a hub
-          const uint32_t dsum = d1 + label_dist(b);
+          const LT b = B[p];
+          if (fmt.hub(b) != w || h1 + fmt.hop(b) > kk) break;  // hops ascend within a hub
+          const uint32_t dsum = d1 + fmt.dist(b);
           best = dsum < best ? dsum : best;
         }
       }
@@ -282,16 +316,21 @@ __global__ void __launch_bounds__(kQThreads) k_query(uint32_t n, const unsigned 
   }
 }
 
-// Workgroups resident at once on a device (the persistent grid), cached.
+// Workgroups resident at once on a device (the persistent grid), cached per
+// kernel form.
+template <class F>
 static uint32_t query_grid(int dev) {
+  static std::mutex mu;
   static int cached_dev = -1;
   static uint32_t cached = 0;
+  std::lock_guard<std::mutex> g(mu);
   if (dev == cached_dev && cached) return cached;
   int cus = 0, per_cu = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
       cus <= 0)
     cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_query, kQThreads, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_query<F>, kQThreads, 0) !=
+          hipSuccess ||
       per_cu <= 0)
     per_cu = 4;
   cached = (uint32_t)(cus * per_cu);
@@ -299,17 +338,83 @@ static uint32_t query_grid(int dev) {
   return cached;
 }
 
+__global__ void __launch_bounds__(256) k_pack_query32(const unsigned long long *lab, uint64_t total,
+                                                      uint32_t *out, uint32_t hsh, uint32_t db) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const unsigned long long l = lab[i];
+    out[i] = (label_hub(l) << hsh) | (label_hop(l) << db) | label_dist(l);
+  }
+}
+
+static uint32_t bits_for(uint64_t x) {  // bits that hold 0..x, at least 1
+  uint32_t b = 1;
+  while (b < 64 && (x >> b)) ++b;
+  return b;
+}
+
+// Makes the index's 32-bit query copy on stream st when the fields fit 32
+// bits (hub bits of n - 1, hop bits of K, dist bits of the index's distance
+// bound), once per label set; tuning knob HSDL_QUERY32=0 keeps the 8-byte
+// labels.
+static std::mutex g_q32_mu;
+static int ensure_query32(const hsdl_index *idx, hipStream_t st) {
+  std::lock_guard<std::mutex> g(g_q32_mu);
+  if (idx->q32) return HSDL_OK;
+  const uint32_t hb = bits_for(idx->n ? idx->n - 1u : 0u), pb = bits_for(idx->K),
+                 db = bits_for(idx->dist_max);
+  if (hb + pb + db > 32) {
+    idx->q32 = 2;
+    return HSDL_OK;
+  }
+  int rc = idx->lab32.alloc(std::max<uint64_t>(idx->total, 1) * 4);
+  if (rc) return rc;
+  const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (idx->total + 255) / 256 + 1);
+  hipLaunchKernelGGL(k_pack_query32, dim3(grid), dim3(256), 0, st,
+                     idx->lab.as<unsigned long long>(), idx->total, idx->lab32.as<uint32_t>(),
+                     pb + db, db);
+  HSDL_HIP(hipGetLastError());
+  idx->q32_hop_bits = pb;
+  idx->q32_dist_bits = db;
+  idx->q32 = 1;
+  return HSDL_OK;
+}
+
+int prepare_query(const hsdl_index *idx) {
+  if (!env_u32("HSDL_QUERY32", 1, 0, 1) || idx->q32) return HSDL_OK;
+  const int rc = ensure_query32(idx, nullptr);
+  if (rc) return rc;
+  HSDL_HIP(hipStreamSynchronize(nullptr));
+  return HSDL_OK;
+}
+
 int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,
-                 const int8_t *d_k, int32_t *d_out, hipStream_t st) {
+                 const int8_t *d_k, int32_t *d_out, hipStream_t st, bool may_alloc) {
   if (nq == 0) return HSDL_OK;
   // tuning knobs: labels staged for the merge path (<= 4094; 0 sends every
   // query down the long path) and LDS skip-index capacity of the long path
   const uint32_t merge_cap = env_u32("HSDL_QUERY_MERGE_CAP", kMergeCap, 0, kMergeCap);
   const uint32_t lds_hubs = env_u32("HSDL_QUERY_LDS_HUBS", kLongHubs, 1, kLongHubs);
-  const uint32_t grid = (uint32_t)std::min<uint64_t>(nq, query_grid(idx->device));
-  hipLaunchKernelGGL(k_query, dim3(grid), dim3(kQThreads), 0, st, idx->n,
-                     idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
-                     (uint64_t)nq, d_s, d_t, d_k, d_out, merge_cap, lds_hubs);
+  const bool want32 = env_u32("HSDL_QUERY32", 1, 0, 1) != 0;
+  if (want32 && may_alloc && !idx->q32) {
+    const int rc = ensure_query32(idx, st);
+    if (rc) return rc;
+  }
+  if (want32 && idx->q32 == 1) {
+    const uint32_t db = idx->q32_dist_bits, pb = idx->q32_hop_bits;
+    const Lab32 fmt{db, pb + db, (1u << pb) - 1u, (1u << db) - 1u};
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(nq, query_grid<Lab32>(idx->device));
+    hipLaunchKernelGGL(k_query<Lab32>, dim3(grid), dim3(kQThreads), 0, st, fmt, idx->n,
+                       idx->off.as<unsigned long long>(), idx->lab32.as<uint32_t>(), (uint64_t)nq,
+                       d_s, d_t, d_k, d_out, merge_cap,
+                       std::min(lds_hubs, QStage<Lab32>::kHubs));
+  } else {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(nq, query_grid<Lab64>(idx->device));
+    hipLaunchKernelGGL(k_query<Lab64>, dim3(grid), dim3(kQThreads), 0, st, Lab64{}, idx->n,
+                       idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>(),
+                       (uint64_t)nq, d_s, d_t, d_k, d_out, merge_cap,
+                       std::min(lds_hubs, QStage<Lab64>::kHubs));
+  }
   HSDL_HIP(hipGetLastError());
   return HSDL_OK;
 }
@@ -420,7 +525,7 @@ static int query_noalloc_body(const hsdl_index *idx, size_t n, const uint32_t *s
   HSDL_HIP(hipMemcpy(ds, s, n * 4, hipMemcpyHostToDevice));
   HSDL_HIP(hipMemcpy(dt, t, n * 4, hipMemcpyHostToDevice));
   HSDL_HIP(hipMemcpy(dk, k, n, hipMemcpyHostToDevice));
-  if ((rc = launch_query(idx, n, ds, dt, dk, dout, nullptr))) return rc;
+  if ((rc = launch_query(idx, n, ds, dt, dk, dout, nullptr, false))) return rc;
   HSDL_HIP(hipMemcpy(out, dout, n * 4, hipMemcpyDeviceToHost));
   return HSDL_OK;
 }

@@ -229,6 +229,9 @@ static int server_create_body(const hsdl_index *idx, const hsdl_server_opts *opt
     return fail(HSDL_EINVAL, "unknown staging mode");
   int rc = use_device(idx->device, nullptr);
   if (rc) return rc;
+  // the index's query copy is made before serving starts, so no batch
+  // allocates
+  if ((rc = prepare_query(idx))) return rc;
   // admission by memory: each slot stages 13 B per request (device buffers
   // with copy staging; the same budget caps mapped host staging)
   const uint64_t per_slot = (uint64_t)o.max_batch * 13 + 4096;

@@ -373,11 +373,13 @@ def test_queries_special_cases_and_bf():
             assert np.array_equal(q, d)
 
 
+@pytest.mark.parametrize("q32", ["1", "0"])
 @pytest.mark.parametrize("lds_hubs", ["7", "64", "2048"])
-def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs):
+def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs, q32):
     """Small LDS capacities send the clean's tiles to the global arena and
-    force skip-index strides > 1 on the queries' long path: neither the clean
-    nor the queries may change."""
+    force skip-index strides > 1 on the queries' long path (32-bit query copy
+    and 8-byte labels): neither the clean nor the queries may change."""
+    monkeypatch.setenv("HSDL_QUERY32", q32)
     monkeypatch.setenv("HSDL_CLEAN_LDS_SLOTS", lds_hubs)
     monkeypatch.setenv("HSDL_QUERY_LDS_HUBS", lds_hubs)
     monkeypatch.setenv("HSDL_QUERY_MERGE_CAP", "0")
@@ -398,11 +400,14 @@ def test_skip_index_strides_clean_and_query(monkeypatch, lds_hubs):
     assert np.array_equal(canon.query(s, t, k), O.query(want, s, t, k))
 
 
+@pytest.mark.parametrize("q32", ["1", "0"])
 @pytest.mark.parametrize("merge_cap", ["0", "64", "150", "300", "4094"])
-def test_query_merge_and_long_paths(monkeypatch, c1, merge_cap):
+def test_query_merge_and_long_paths(monkeypatch, c1, merge_cap, q32):
     """The merge path (both lists in LDS) and the long path (skip index) give
-    the golden answers; the caps split the config-1 queries between them
-    (lists of 1..256 labels)."""
+    the golden answers, reading the 32-bit query copy or the 8-byte labels;
+    the caps split the config-1 queries between them (lists of 1..256
+    labels)."""
+    monkeypatch.setenv("HSDL_QUERY32", q32)
     monkeypatch.setenv("HSDL_QUERY_MERGE_CAP", merge_cap)
     g, z = c1
     idx = H.Index.from_labels(len(z["lab_off"]) - 1, 4, z["lab_off"].astype(np.uint64),
@@ -414,6 +419,41 @@ def test_query_merge_and_long_paths(monkeypatch, c1, merge_cap):
     k = rng.integers(-1, 9, 20000).astype(np.int8)
     want = O.query(O.labels_from_arrays(4096, z["lab_off"], z["hub"], z["hop"], z["dist"]), s, t, k)
     assert np.array_equal(idx.query(s, t, k), want)
+
+
+def test_query_copy_follows_the_clean():
+    """The 32-bit query copy is made by the first query and released by a
+    clean that removes labels: queries before and after the clean, and the
+    no-allocation entry point (which reads the copy only once a query made
+    it), all equal the oracle's answers; labels whose distances need 27
+    bits do not fit the copy and are queried from the 8-byte labels."""
+    g = H.Graph.rmat(10, 8, 321, 1, 20)
+    c = csr_of(g)
+    red = O.kround_grouped(c, 4, 32)
+    off, hub, hop, dist = red.arrays()
+    want = O.clean(red)
+    rng = np.random.default_rng(3)
+    s = rng.integers(0, c.n, 20000).astype(np.uint32)
+    t = rng.integers(0, c.n, 20000).astype(np.uint32)
+    k = rng.integers(-1, 6, 20000).astype(np.int8)
+    ans = O.query(want, s, t, k)
+    idx = H.Index.from_labels(c.n, 4, off.astype(np.uint64), hub, hop, dist)
+    assert np.array_equal(idx.query_noalloc(s, t, k), ans)
+    assert np.array_equal(idx.query(s, t, k), ans)
+    assert np.array_equal(idx.query_noalloc(s, t, k), ans)
+    total = idx.label_count()
+    idx.clean()
+    assert idx.label_count() < total
+    assert np.array_equal(idx.query_noalloc(s, t, k), ans)
+    assert np.array_equal(idx.query(s, t, k), ans)
+    same_labels(idx, want.arrays())
+    # distances scaled to need 27 bits: the copy does not fit
+    big = dist.astype(np.int64) * ((1 << 26) // max(1, int(dist.max())))
+    wide = H.Index.from_labels(c.n, 4, off.astype(np.uint64), hub, hop, big.astype(np.int32))
+    got = wide.query(s, t, k)
+    sc = (1 << 26) // max(1, int(dist.max()))
+    assert np.array_equal(got[ans == INF], ans[ans == INF])
+    assert np.array_equal(got[ans != INF].astype(np.int64), ans[ans != INF].astype(np.int64) * sc)
 
 
 def test_export_import_save_load_roundtrip(tmp_path, c1):
