@@ -911,7 +911,7 @@ int clean_index(hsdl_index *idx) {
   std::swap(idx->off, noff);
   std::swap(idx->lab, nlab);
   idx->lab32.release();
-  idx->q32 = 0;
+  idx->q32.store(0, std::memory_order_release);
   give_scratch(dev, kCsOff, noff);
   give_scratch(dev, kCsLab, nlab);
   if (idx->has_parents) idx->par = std::move(npar);

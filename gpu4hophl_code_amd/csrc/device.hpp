@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 
@@ -225,7 +226,8 @@ struct hsdl_index {
   // may allocate when the fields fit; released whenever `lab` changes.
   // q32: 0 not made yet, 1 made, 2 does not fit or switched off.
   mutable hsdl::DevBuf lab32;
-  mutable uint32_t q32 = 0, q32_hop_bits = 0, q32_dist_bits = 0;
+  mutable std::atomic<uint32_t> q32{0};  // set with release after the fields below
+  mutable uint32_t q32_hop_bits = 0, q32_dist_bits = 0;
 };
 
 namespace hsdl {

@@ -355,16 +355,16 @@ static uint32_t bits_for(uint64_t x) {  // bits that hold 0..x, at least 1
 
 // Makes the index's 32-bit query copy on stream st when the fields fit 32
 // bits (hub bits of n - 1, hop bits of K, dist bits of the index's distance
-// bound), once per label set; tuning knob HSDL_QUERY32=0 keeps the 8-byte
-// labels.
+// bound), once per label set, and waits for it; tuning knob HSDL_QUERY32=0
+// keeps the 8-byte labels.
 static std::mutex g_q32_mu;
 static int ensure_query32(const hsdl_index *idx, hipStream_t st) {
   std::lock_guard<std::mutex> g(g_q32_mu);
-  if (idx->q32) return HSDL_OK;
+  if (idx->q32.load(std::memory_order_acquire)) return HSDL_OK;
   const uint32_t hb = bits_for(idx->n ? idx->n - 1u : 0u), pb = bits_for(idx->K),
                  db = bits_for(idx->dist_max);
   if (hb + pb + db > 32) {
-    idx->q32 = 2;
+    idx->q32.store(2, std::memory_order_release);
     return HSDL_OK;
   }
   int rc = idx->lab32.alloc(std::max<uint64_t>(idx->total, 1) * 4);
@@ -374,18 +374,18 @@ static int ensure_query32(const hsdl_index *idx, hipStream_t st) {
                      idx->lab.as<unsigned long long>(), idx->total, idx->lab32.as<uint32_t>(),
                      pb + db, db);
   HSDL_HIP(hipGetLastError());
+  // queries on other streams may read the copy as soon as it is marked made
+  HSDL_HIP(hipStreamSynchronize(st));
   idx->q32_hop_bits = pb;
   idx->q32_dist_bits = db;
-  idx->q32 = 1;
+  idx->q32.store(1, std::memory_order_release);
   return HSDL_OK;
 }
 
 int prepare_query(const hsdl_index *idx) {
-  if (!env_u32("HSDL_QUERY32", 1, 0, 1) || idx->q32) return HSDL_OK;
-  const int rc = ensure_query32(idx, nullptr);
-  if (rc) return rc;
-  HSDL_HIP(hipStreamSynchronize(nullptr));
-  return HSDL_OK;
+  if (!env_u32("HSDL_QUERY32", 1, 0, 1) || idx->q32.load(std::memory_order_acquire))
+    return HSDL_OK;
+  return ensure_query32(idx, nullptr);
 }
 
 int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const uint32_t *d_t,
@@ -396,11 +396,16 @@ int launch_query(const hsdl_index *idx, size_t nq, const uint32_t *d_s, const ui
   const uint32_t merge_cap = env_u32("HSDL_QUERY_MERGE_CAP", kMergeCap, 0, kMergeCap);
   const uint32_t lds_hubs = env_u32("HSDL_QUERY_LDS_HUBS", kLongHubs, 1, kLongHubs);
   const bool want32 = env_u32("HSDL_QUERY32", 1, 0, 1) != 0;
-  if (want32 && may_alloc && !idx->q32) {
-    const int rc = ensure_query32(idx, st);
-    if (rc) return rc;
+  if (want32 && may_alloc && !idx->q32.load(std::memory_order_acquire)) {
+    // a stream under graph capture takes no allocation or synchronisation
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HSDL_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs == hipStreamCaptureStatusNone) {
+      const int rc = ensure_query32(idx, st);
+      if (rc) return rc;
+    }
   }
-  if (want32 && idx->q32 == 1) {
+  if (want32 && idx->q32.load(std::memory_order_acquire) == 1) {
     const uint32_t db = idx->q32_dist_bits, pb = idx->q32_hop_bits;
     const Lab32 fmt{db, pb + db, (1u << pb) - 1u, (1u << db) - 1u};
     const uint32_t grid = (uint32_t)std::min<uint64_t>(nq, query_grid<Lab32>(idx->device));

@@ -240,7 +240,8 @@ int hsdl_index_stats(const hsdl_index *idx, hsdl_stats *st);
  * gpu_query (Query_mdnd/src/gpu_query.cu:84-177).  k < 0 -> INT32_MAX;
  * s == t -> 0; k == 0 -> INT32_MAX.  The first call on an index makes a
  * 32-bit copy of its labels (4 B per label, kept with the index) when hub,
- * hop and distance fit one word; hsdl_query_device does the same. */
+ * hop and distance fit one word; hsdl_query_device does the same (waiting
+ * for the copy once), except on a stream under graph capture. */
 int hsdl_query(const hsdl_index *idx, size_t n, const uint32_t *s, const uint32_t *t,
                const int8_t *k, int32_t *out);
 

@@ -456,6 +456,115 @@ def test_query_copy_follows_the_clean():
     assert np.array_equal(got[ans != INF].astype(np.int64), ans[ans != INF].astype(np.int64) * sc)
 
 
+def _hip_runtime():
+    """The HIP runtime the library loaded (ctypes), for device buffers,
+    streams and graph capture without a second runtime in the process."""
+    import ctypes
+    H.lib()
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64.so" in line:
+                path = line.split()[-1]
+                break
+    assert path, "libamdhip64 not loaded"
+    rt = ctypes.CDLL(path)
+
+    def ok(rc):
+        assert rc == 0, f"HIP error {rc}"
+    return rt, ok
+
+
+def test_first_queries_on_two_streams_at_once():
+    """Two threads make the first device queries of an index on two streams
+    at once: one of them makes the 32-bit query copy and waits for it before
+    it marks it made, so the other stream never reads it half written; both
+    answers equal the oracle's, and so does a query under HIP graph capture
+    on an index without the copy (it reads the 8-byte labels)."""
+    import ctypes
+    import threading
+    rt, ok = _hip_runtime()
+    vp = ctypes.c_void_p
+    g = H.Graph.rmat(12, 8, 4242, 1, 50)
+    c = csr_of(g)
+    ol = O.clean(O.kround(c, 4))
+    rng = np.random.default_rng(11)
+    m = 1 << 16
+    s = rng.integers(0, c.n, m).astype(np.uint32)
+    t = rng.integers(0, c.n, m).astype(np.uint32)
+    k = rng.integers(1, 6, m).astype(np.int8)
+    want = O.query(ol, s, t, k)
+    bufs = []
+
+    def dev(arr):
+        p = vp()
+        ok(rt.hipMalloc(ctypes.byref(p), ctypes.c_size_t(max(arr.nbytes, 16))))
+        ok(rt.hipMemcpy(p, arr.ctypes.data_as(vp), ctypes.c_size_t(arr.nbytes), 1))
+        bufs.append(p)
+        return p
+
+    def host(p, n):
+        out = np.empty(n, np.int32)
+        ok(rt.hipMemcpy(out.ctypes.data_as(vp), p, ctypes.c_size_t(4 * n), 2))
+        return out
+    ds, dt, dk = dev(s), dev(t), dev(k)
+    outs = [dev(np.full(m, -7, np.int32)) for _ in range(3)]
+    streams = []
+    for _ in range(3):
+        st = vp()
+        ok(rt.hipStreamCreate(ctypes.byref(st)))
+        streams.append(st)
+    try:
+        for _ in range(3):
+            idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
+            ok(rt.hipDeviceSynchronize())
+            errs = []
+
+            def run(i):
+                try:
+                    idx.query_device(ds.value, dt.value, dk.value, outs[i].value, m,
+                                     streams[i].value)
+                except Exception as e:  # reported below
+                    errs.append(e)
+            th = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
+            ok(rt.hipDeviceSynchronize())
+            assert not errs
+            for i in range(2):
+                assert np.array_equal(host(outs[i], m), want)
+            idx.free()
+        # under capture the query makes no copy and reads the 8-byte labels
+        idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
+        st = streams[2]
+        ok(rt.hipStreamBeginCapture(st, 2))  # relaxed mode
+        idx.query_device(ds.value, dt.value, dk.value, outs[2].value, m, st.value)
+        graph, gexec = vp(), vp()
+        ok(rt.hipStreamEndCapture(st, ctypes.byref(graph)))
+        ok(rt.hipGraphInstantiate(ctypes.byref(gexec), graph, None, None, ctypes.c_size_t(0)))
+        for _ in range(3):
+            ok(rt.hipMemset(outs[2], 0x7F, ctypes.c_size_t(4 * m)))
+            ok(rt.hipDeviceSynchronize())
+            ok(rt.hipGraphLaunch(gexec, st))
+            ok(rt.hipStreamSynchronize(st))
+            assert np.array_equal(host(outs[2], m), want)
+        ok(rt.hipGraphExecDestroy(gexec))
+        ok(rt.hipGraphDestroy(graph))
+        # after capture, an ordinary call makes the copy; answers unchanged
+        idx.query_device(ds.value, dt.value, dk.value, outs[2].value, m, st.value)
+        ok(rt.hipStreamSynchronize(st))
+        assert np.array_equal(host(outs[2], m), want)
+        idx.free()
+    finally:
+        ok(rt.hipDeviceSynchronize())
+        for st in streams:
+            rt.hipStreamDestroy(st)
+        for p in bufs:
+            rt.hipFree(p)
+
+
 def test_export_import_save_load_roundtrip(tmp_path, c1):
     g, z = c1
     idx = H.hop_constrained_two_hop_labels_generation(g, 4, clean=True)
