@@ -43,7 +43,7 @@ struct hsdl_graph {
   // frontier lists, tile arena, re-sort slices), kept for the next build on
   // this graph: freeing and allocating gigabytes inside every build now and
   // then cost up to a second there
-  static constexpr int kScratch = 24;
+  static constexpr int kScratch = 32;
   void *scratch_p[kScratch] = {};
   size_t scratch_bytes[kScratch] = {};
   int d_device = -1;

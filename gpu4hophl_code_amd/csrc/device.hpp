@@ -237,6 +237,12 @@ namespace hsdl {
 // through a temporary buffer.
 int segmented_sort_labels(uint64_t *d_lab, uint64_t total, const uint64_t *d_off, uint32_t n,
                           hipStream_t st);
+// Sorts each vertex segment of 32-bit keys (bits [0, end_bit)) from `in` into
+// `out` with the caller's temporary storage; tmp == nullptr: only sets
+// tmp_bytes.
+int segmented_sort_u32(void *tmp, size_t &tmp_bytes, const uint32_t *in, uint32_t *out,
+                       uint64_t total, const uint64_t *d_off, uint32_t n, uint32_t end_bit,
+                       hipStream_t st);
 // Exclusive scan of u32 counts into u64 offsets (n+1 entries, last = total).
 int scan_counts_u32(const uint32_t *d_cnt, uint64_t *d_off, uint32_t n, hipStream_t st);
 int scan_counts_u64(const uint64_t *d_cnt, uint64_t *d_off, uint32_t n, hipStream_t st);

@@ -1659,6 +1659,31 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
   }
 }
 
+// The export's 32-bit path: the lists' 32-bit copies gathered into one
+// array (k_gather32), sorted per vertex as 32-bit keys, then widened into
+// the index's 8-byte labels (k_widen32); the packing keeps (hub, hop, dist)
+// order.
+__global__ void __launch_bounds__(256) k_gather32(GenArgs a, const unsigned long long *out_off,
+                                                  uint32_t *out) {
+  const int lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / kWave;
+  for (uint32_t v = wave; v < a.n; v += nwaves) {
+    const unsigned long long src = a.seg_start[v], dst = out_off[v];
+    const uint32_t len = a.seg_len[v];
+    for (uint32_t i = lane; i < len; i += kWave) out[dst + i] = a.pool32[src + i];
+  }
+}
+__global__ void __launch_bounds__(256) k_widen32(GenArgs a, const uint32_t *in, uint64_t total,
+                                                 unsigned long long *out) {
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    uint32_t hub, hop, dist;
+    unpack(a, in[i], hub, hop, dist);
+    out[i] = pack_label(hub, hop, dist);
+  }
+}
+
 // Hop zones: after a batch, every list whose append zone has grown past
 // 1/alpha_div of its sorted zone is sorted again, stably by hop, and its
 // per-hop prefix counts rewritten -- so a later round's query skips the
@@ -1852,7 +1877,8 @@ static int upload_graph(hsdl_graph *g, int dev) {
 enum ScratchId : int {
   kScSegStart, kScSegLen, kScSegCap, kScAddCnt, kScAddCur, kScTouched, kScFgrp, kScHubSlot,
   kScSortedLen, kScHcnt, kScSortTmp, kScSortTmp32, kScCflat, kScCgroup, kScHubCcnt, kScHubOff,
-  kScHubCur, kScChunkOff, kScF0, kScF1, kScGtile, kScCtrl, kScHeavy, kScCount
+  kScHubCur, kScChunkOff, kScF0, kScF1, kScGtile, kScCtrl, kScHeavy, kScExpKeys, kScExpSorted,
+  kScExpTmp, kScCount
 };
 static_assert(kScCount <= hsdl_graph::kScratch, "scratch slots");
 
@@ -1864,6 +1890,7 @@ struct Gen {
   uint32_t sort_per_wave = 0;                     // labels per wave slice of sort_tmp
   int sort_grid = 0;
   DevBuf cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
+  DevBuf exp_keys, exp_sorted, exp_tmp;  // export: 32-bit keys, sorted keys, sort space
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
@@ -1893,7 +1920,7 @@ struct Gen {
     DevBuf *t[kScCount] = {&seg_start, &seg_len, &seg_cap, &add_cnt, &add_cur, &touched, &fgrp,
                            &hub_slot, &sorted_len, &hcnt, &sort_tmp, &sort_tmp32, &cflat, &cgroup,
                            &hub_ccnt, &hub_off, &hub_cur, &chunk_off, &F0, &F1, &gtile, &ctrl,
-                           &heavy};
+                           &heavy, &exp_keys, &exp_sorted, &exp_tmp};
     return t[id];
   }
   // Buffer `id` of at least n bytes: the graph's kept one when it is large
@@ -2663,19 +2690,46 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
     return cleanup(fail(HSDL_EDEVICE, "reading label total failed"));
   idx->total = total;
   if ((rc = idx->lab.alloc(std::max<uint64_t>(total, 1) * 8))) return cleanup(rc);
-  hipLaunchKernelGGL(k_gather, dim3(g.grid), dim3(256), 0, g.st, a,
-                     idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>());
-  if (hipStreamSynchronize(g.st) != hipSuccess) return cleanup(fail(HSDL_EDEVICE, "gather failed"));
   const uint64_t t_sort0 = now_ns();
-  // the pools stay for the graph's next build; they are released first only
-  // when the sort cannot get its buffers otherwise
-  rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st);
-  if (rc == HSDL_ENOMEM) {
-    g.pool.release();
-    g.pool32.release();
-    rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st);
+  // the 32-bit path when the lists have 32-bit copies (tuning knob
+  // HSDL_EXPORT32=0: the 8-byte path); its buffers stay with the graph
+  size_t exp_tmp_bytes = 0;
+  const uint32_t key_bits = p32_hub_bits + p32_hop_bits + p32_dist_bits;
+  bool sorted32 = false;
+  if (a.pool32 && total && env_u32("HSDL_EXPORT32", 1, 0, 1) &&
+      segmented_sort_u32(nullptr, exp_tmp_bytes, nullptr, nullptr, total, nullptr, n, key_bits,
+                         g.st) == HSDL_OK &&
+      g.take(kScExpKeys, total * 4) == HSDL_OK && g.take(kScExpSorted, total * 4) == HSDL_OK &&
+      g.take(kScExpTmp, std::max<size_t>(exp_tmp_bytes, 16)) == HSDL_OK) {
+    hipLaunchKernelGGL(k_gather32, dim3(g.grid), dim3(256), 0, g.st, a,
+                       idx->off.as<unsigned long long>(), g.exp_keys.as<uint32_t>());
+    HSDL_HIP(hipGetLastError());
+    if ((rc = segmented_sort_u32(g.exp_tmp.p, exp_tmp_bytes, g.exp_keys.as<uint32_t>(),
+                                 g.exp_sorted.as<uint32_t>(), total, idx->off.as<uint64_t>(), n,
+                                 key_bits, g.st)))
+      return cleanup(rc);
+    hipLaunchKernelGGL(k_widen32, dim3(g.grid), dim3(256), 0, g.st, a,
+                       g.exp_sorted.as<uint32_t>(), total, idx->lab.as<unsigned long long>());
+    HSDL_HIP(hipGetLastError());
+    if (hipStreamSynchronize(g.st) != hipSuccess)
+      return cleanup(fail(HSDL_EDEVICE, "export failed"));
+    sorted32 = true;
   }
-  if (rc) return cleanup(rc);
+  if (!sorted32) {
+    hipLaunchKernelGGL(k_gather, dim3(g.grid), dim3(256), 0, g.st, a,
+                       idx->off.as<unsigned long long>(), idx->lab.as<unsigned long long>());
+    if (hipStreamSynchronize(g.st) != hipSuccess)
+      return cleanup(fail(HSDL_EDEVICE, "gather failed"));
+    // the pools stay for the graph's next build; they are released first
+    // only when the sort cannot get its buffers otherwise
+    rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st);
+    if (rc == HSDL_ENOMEM) {
+      g.pool.release();
+      g.pool32.release();
+      rc = segmented_sort_labels(idx->lab.as<uint64_t>(), total, idx->off.as<uint64_t>(), n, g.st);
+    }
+    if (rc) return cleanup(rc);
+  }
   const uint64_t t_sorted = now_ns();
   // parents (optional): derived from the sorted uncleaned labels and the graph
   if (opts->parents && (rc = derive_parents(idx, hg, g.st))) return cleanup(rc);

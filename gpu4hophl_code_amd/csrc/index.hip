@@ -62,6 +62,14 @@ int scan_counts_u64(const uint64_t *d_cnt, uint64_t *d_off, uint32_t n, hipStrea
   return scan_counts_impl(d_cnt, d_off, n, st);
 }
 
+int segmented_sort_u32(void *tmp, size_t &tmp_bytes, const uint32_t *in, uint32_t *out,
+                       uint64_t total, const uint64_t *d_off, uint32_t n, uint32_t end_bit,
+                       hipStream_t st) {
+  HSDL_HIP(rocprim::segmented_radix_sort_keys(tmp, tmp_bytes, in, out, (size_t)total, n, d_off,
+                                              d_off + 1, 0, (int)end_bit, st));
+  return HSDL_OK;
+}
+
 int segmented_sort_labels(uint64_t *d_lab, uint64_t total, const uint64_t *d_off, uint32_t n,
                           hipStream_t st) {
   if (total == 0 || n == 0) return HSDL_OK;
