@@ -276,6 +276,16 @@ __device__ inline void unpack(const GenArgs &, unsigned long long l, uint32_t &h
   dist = label_dist(l);
 }
 // the pool a prune streams: LT = uint32_t (pool32) or unsigned long long
+// Label i of the pool as (hub, hop, dist): a build that keeps the 32-bit
+// copy keeps no 8-byte pool (its labels are the copy's, widened where the
+// index needs them).
+__device__ inline void pool_label(const GenArgs &a, unsigned long long i, uint32_t &hub,
+                                  uint32_t &hop, uint32_t &dist) {
+  if (a.pool32)
+    unpack(a, a.pool32[i], hub, hop, dist);
+  else
+    unpack(a, a.pool[i], hub, hop, dist);
+}
 template <class LT> __device__ inline const LT *stream_pool(const GenArgs &a);
 template <> __device__ inline const uint32_t *stream_pool<uint32_t>(const GenArgs &a) {
   return a.pool32;
@@ -320,7 +330,7 @@ __global__ void k_init_labels(GenArgs a, uint32_t c0) {
     a.seg_cap[v] = c0;
     a.add_cnt[v] = 0;
     a.add_cur[v] = 0;
-    a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
+    if (a.pool) a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
     if (a.pool32) a.pool32[v * c0] = pack32(a, (uint32_t)v, 0, 0);
     if (a.hcnt) {
       a.sorted_len[v] = 1;
@@ -897,8 +907,8 @@ __device__ void build_tile(const GenArgs &a, uint32_t s, const ListView &view, u
   __syncthreads();
   const unsigned long long st = a.seg_start[s];
   for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
-    const unsigned long long l = a.pool[st + view.at(i)];
-    const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
+    uint32_t hub, hb, db;
+    pool_label(a, st + view.at(i), hub, hb, db);
     uint32_t slot = hash_slot(hub, cap);
     for (;;) {
       const uint32_t prev = atomicCAS(&rec[slot * stride], kInf32, hub);
@@ -1027,7 +1037,8 @@ __device__ void build_idx_tile(const GenArgs &a, uint32_t s, const ListView &vie
   __syncthreads();
   const unsigned long long st = a.seg_start[s];
   for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
-    const uint32_t hub = label_hub(a.pool[st + view.at(i)]);
+    uint32_t hub, hb, db;
+    pool_label(a, st + view.at(i), hub, hb, db);
     uint32_t slot = hash_slot(hub, cap);
     for (;;) {
       const uint32_t prev = atomicCAS(&keys[slot], 0u, (hub << kIdxBits) | kIdxRowMask);
@@ -1043,8 +1054,8 @@ __device__ void build_idx_tile(const GenArgs &a, uint32_t s, const ListView &vie
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < view.len; i += blockDim.x) {
-    const unsigned long long l = a.pool[st + view.at(i)];
-    const uint32_t hub = label_hub(l), hb = label_hop(l), db = label_dist(l);
+    uint32_t hub, hb, db;
+    pool_label(a, st + view.at(i), hub, hb, db);
     uint32_t slot = hash_slot(hub, cap);
     uint32_t k = keys[slot];
     while ((k >> kIdxBits) != hub) {
@@ -1570,7 +1581,8 @@ __global__ void __launch_bounds__(256) k_reserve(GenArgs a) {
       const uint32_t yj = __shfl(y, j, kWave), lj = __shfl(len, j, kWave);
       const unsigned long long nsj = __shfl(ns, j, kWave);
       const unsigned long long os = a.seg_start[yj];
-      for (uint32_t i = lane; i < lj; i += kWave) a.pool[nsj + i] = a.pool[os + i];
+      if (a.pool)
+        for (uint32_t i = lane; i < lj; i += kWave) a.pool[nsj + i] = a.pool[os + i];
       if (a.pool32)
         for (uint32_t i = lane; i < lj; i += kWave) a.pool32[nsj + i] = a.pool32[os + i];
     }
@@ -1591,7 +1603,7 @@ __global__ void __launch_bounds__(256) k_append(GenArgs a) {
     if (p < a.seg_cap[it.y]) {
       const unsigned long long at = a.seg_start[it.y] + p;
       const uint32_t hub = hub_of(a, it.b);
-      a.pool[at] = pack_label(hub, a.h, it.d);
+      if (a.pool) a.pool[at] = pack_label(hub, a.h, it.d);
       if (a.pool32) a.pool32[at] = pack32(a, hub, a.h, it.d);
     } else {
       atomicOr(&a.ctrl->overflow, kOvfPool);
@@ -1636,7 +1648,8 @@ __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
     uint32_t len = a.seg_len[v];
     const unsigned long long st = a.seg_start[v];
     while (len > 1) {
-      const uint32_t hub = label_hub(a.pool[st + len - 1]);
+      uint32_t hub, hop, dist;
+      pool_label(a, st + len - 1, hub, hop, dist);
       if (hub < lo || hub > hi) break;
       --len;
     }
@@ -1655,7 +1668,11 @@ __global__ void __launch_bounds__(256) k_gather(GenArgs a, const unsigned long l
   for (uint32_t v = wave; v < a.n; v += nwaves) {
     const unsigned long long src = a.seg_start[v], dst = out_off[v];
     const uint32_t len = a.seg_len[v];
-    for (uint32_t i = lane; i < len; i += kWave) out[dst + i] = a.pool[src + i];
+    for (uint32_t i = lane; i < len; i += kWave) {
+      uint32_t hub, hop, dist;
+      pool_label(a, src + i, hub, hop, dist);
+      out[dst + i] = pack_label(hub, hop, dist);
+    }
   }
 }
 
@@ -1703,14 +1720,21 @@ __device__ inline void hop_scatter(const GenArgs &a, unsigned long long st, uint
   const uint32_t lane = (uint32_t)lane_id();
   const unsigned long long lt_mask = (1ull << lane) - 1ull;
   const uint32_t K1 = a.K + 1;
+  const bool p64 = a.pool != nullptr;  // absent when the build keeps the 32-bit copy
   for (uint32_t j0 = 0; j0 < len; j0 += kWave) {
     const uint32_t j = j0 + lane;
     unsigned long long l = 0;
     uint32_t l32 = 0, hop = kInf32;
     if (j < len) {
-      l = a.pool[st + j];
-      if (p32) l32 = a.pool32[st + j];
-      hop = label_hop(l);
+      if (p64) {
+        l = a.pool[st + j];
+        hop = label_hop(l);
+      }
+      if (p32) {
+        l32 = a.pool32[st + j];
+        uint32_t hub, d;
+        unpack(a, l32, hub, hop, d);
+      }
     }
     uint32_t dst = 0;
 #pragma unroll
@@ -1721,7 +1745,7 @@ __device__ inline void hop_scatter(const GenArgs &a, unsigned long long st, uint
       pos[x] += (uint32_t)__popcll(m);
     }
     if (j < len) {
-      t64[dst] = l;
+      if (p64) t64[dst] = l;
       if (p32) t32[dst] = l32;
     }
   }
@@ -1729,7 +1753,7 @@ __device__ inline void hop_scatter(const GenArgs &a, unsigned long long st, uint
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   for (uint32_t j = lane; j < len; j += kWave) {
-    a.pool[st + j] = t64[j];
+    if (p64) a.pool[st + j] = t64[j];
     if (p32) a.pool32[st + j] = t32[j];
   }
   // the next list reuses the scratch
@@ -1977,20 +2001,22 @@ static int read_ctrl(Gen &g, Ctrl &c) {
 static int grow_pool(Gen &g, uint64_t used, uint64_t min_cap) {
   uint64_t ncap = g.a.pool_cap * 2;
   while (ncap < min_cap) ncap *= 2;
-  const bool p32 = g.a.pool32 != nullptr;
-  const uint64_t per_label = p32 ? 12 : 8;
+  const bool p32 = g.a.pool32 != nullptr;  // then there is no 8-byte pool
+  const uint64_t per_label = p32 ? 4 : 8;
   if (g.held() - g.pool.bytes - g.pool32.bytes + ncap * per_label > g.budget)
     return fail(HSDL_ENOMEM, "label pool exceeds the HBM budget");
   DevBuf np, np32;
-  int rc = np.alloc(ncap * 8);
-  if (!rc && p32) rc = np32.alloc(ncap * 4);
+  int rc = p32 ? np32.alloc(ncap * 4) : np.alloc(ncap * 8);
   if (rc) return rc;
-  HSDL_HIP(hipMemcpyAsync(np.p, g.pool.p, used * 8, hipMemcpyDeviceToDevice, g.st));
   if (p32)
     HSDL_HIP(hipMemcpyAsync(np32.p, g.pool32.p, used * 4, hipMemcpyDeviceToDevice, g.st));
+  else
+    HSDL_HIP(hipMemcpyAsync(np.p, g.pool.p, used * 8, hipMemcpyDeviceToDevice, g.st));
   HSDL_HIP(hipStreamSynchronize(g.st));
-  g.pool = std::move(np);
-  g.a.pool = g.pool.as<unsigned long long>();
+  if (!p32) {
+    g.pool = std::move(np);
+    g.a.pool = g.pool.as<unsigned long long>();
+  }
   if (p32) {
     g.pool32 = std::move(np32);
     g.a.pool32 = g.pool32.as<uint32_t>();
@@ -2311,26 +2337,29 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   while ((1ull << p32_hub_bits) <= (uint64_t)(n - 1)) ++p32_hub_bits;
   const bool use32 = env_u32("HSDL_STREAM32", 1, 0, 1) &&
                      p32_hub_bits + p32_hop_bits + p32_dist_bits <= 32;
-  const uint64_t per_label = use32 ? 12 : 8;
-  // the pools kept from the previous build on this graph are taken over when
-  // no initial pool size is forced and they fit the budget (no allocation,
-  // and no growth if this build needs no more); otherwise they are freed
+  // the 32-bit copy, when there is one, is the only pool (4 B per slot);
+  // otherwise the 8-byte pool (8 B)
+  const uint64_t per_label = use32 ? 4 : 8;
+  // the pool kept from the previous build on this graph is taken over when
+  // no initial pool size is forced and it fits the budget (no allocation,
+  // and no growth if this build needs no more); otherwise it is freed
   {
-    const uint64_t kept = mg->d_pool_bytes / 8;
-    const bool adopt = !opts->pool_init_labels && mg->d_pool && kept >= pool0 &&
-                       (!use32 || (mg->d_pool32 && mg->d_pool32_bytes >= kept * 4)) &&
+    const uint64_t kept = use32 ? mg->d_pool32_bytes / 4 : mg->d_pool_bytes / 8;
+    const bool adopt = !opts->pool_init_labels && (use32 ? mg->d_pool32 : mg->d_pool) &&
+                       kept >= pool0 &&
                        g.fixed + kept * per_label + gtile0 * 4 + (uint64_t)list0 * 36 <= g.budget;
     if (adopt) {
       pool0 = kept;
-      g.pool.p = mg->d_pool;
-      g.pool.bytes = mg->d_pool_bytes;
-      mg->d_pool = nullptr;
-      mg->d_pool_bytes = 0;
       if (use32) {
         g.pool32.p = mg->d_pool32;
         g.pool32.bytes = mg->d_pool32_bytes;
         mg->d_pool32 = nullptr;
         mg->d_pool32_bytes = 0;
+      } else {
+        g.pool.p = mg->d_pool;
+        g.pool.bytes = mg->d_pool_bytes;
+        mg->d_pool = nullptr;
+        mg->d_pool_bytes = 0;
       }
     }
     if (mg->d_pool) (void)hipFree(mg->d_pool);
@@ -2341,7 +2370,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   if (g.budget < g.fixed + pool0 * per_label + gtile0 * 4 + (uint64_t)list0 * 36)
     return fail(HSDL_ENOMEM, "HBM budget below the label pool, slots and lists");
 
-  if ((!g.pool.p && (rc = g.pool.alloc(pool0 * 8))) || (rc = g.take(kScSegStart, n * 8ull)) ||
+  if ((!use32 && !g.pool.p && (rc = g.pool.alloc(pool0 * 8))) ||
+      (rc = g.take(kScSegStart, n * 8ull)) ||
       (rc = g.take(kScSegLen, n * 4ull)) || (rc = g.take(kScSegCap, n * 4ull)) ||
       (rc = g.take(kScAddCnt, n * 4ull)) || (rc = g.take(kScAddCur, n * 4ull)) ||
       (rc = g.take(kScTouched, n * 4ull)) || (rc = g.take(kScFgrp, n * 16ull)) ||
@@ -2437,7 +2467,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.hubs = group ? hubs_buf.as<uint32_t>() : nullptr;
   a.off = static_cast<const unsigned long long *>(hg->d_off);
   a.adj = static_cast<const uint2 *>(hg->d_adj);
-  a.pool = g.pool.as<unsigned long long>();
+  a.pool = use32 ? nullptr : g.pool.as<unsigned long long>();
   a.pool_cap = pool0;
   a.pool32 = use32 ? g.pool32.as<uint32_t>() : nullptr;
   a.p32_hop_bits = p32_hop_bits;
@@ -2764,9 +2794,8 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   idx->stats.prune_tile_reads = c.tile_reads;
   idx->stats.prune_row_slots = c.row_slots;
   idx->stats.prune_label_bytes = a.pool32 ? 4 : 8;
-  // the pools stay with the graph for its next build (a pool released for
-  // the sort leaves nothing to keep: the next build allocates anew)
-  if (!g.pool.p) g.pool32.release();
+  // the pool stays with the graph for its next build (one released for the
+  // sort leaves nothing to keep: the next build allocates anew)
   mg->d_pool = g.pool.p;
   mg->d_pool_bytes = g.pool.bytes;
   g.pool.p = nullptr;
