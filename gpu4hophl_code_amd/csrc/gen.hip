@@ -2293,10 +2293,12 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   // per vertex: segment start, length, capacity, add counters, touched
   // entry, frontier group (16 B), hub slot, hop zone counts
   const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5 + 16 + 4 + 4 * (uint64_t)K);
-  // dense dedupe slots: at most a quarter of the budget, <= 65536 hubs
+  // dense dedupe slots: at most 1/slot_div of the budget (tuning knob
+  // HSDL_SLOT_DIV), <= 65536 hubs
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
+  const uint32_t slot_div = env_u32("HSDL_SLOT_DIV", 4, 2, 64);
   uint32_t Bmax =
-      (uint32_t)std::min<uint64_t>({g.budget / 4 / (4ull * n), (uint64_t)n, 1ull << 16});
+      (uint32_t)std::min<uint64_t>({g.budget / slot_div / (4ull * n), (uint64_t)n, 1ull << 16});
   if (plain) Bmax = 1;
   if (opts->hub_batch) Bmax = std::min<uint32_t>(opts->hub_batch, Bmax);
   if (Bmax == 0) return fail(HSDL_ENOMEM, "HBM budget below one hub's dedupe slots");
