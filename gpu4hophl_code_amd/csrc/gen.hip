@@ -1775,10 +1775,13 @@ __global__ void __launch_bounds__(256) k_hop_sort(GenArgs a, unsigned long long 
   const bool p32 = tmp32 != nullptr;
   unsigned long long *t64 = tmp + (uint64_t)wave * per_wave;
   uint32_t *t32 = p32 ? tmp32 + (uint64_t)wave * per_wave : nullptr;
-  // LDS: 4 slices of lds_labels u64, then 4 of lds_labels u32
+  // LDS: 4 slices of lds_labels u64, then 4 of lds_labels u32; a build
+  // with only the 32-bit pool has the u32 slices alone
+  const bool p64 = a.pool != nullptr;
   lds_ptr<unsigned long long> l64 = (lds_ptr<unsigned long long>)(zl64 + wib * lds_labels);
-  lds_ptr<uint32_t> l32 = (lds_ptr<uint32_t>)(reinterpret_cast<uint32_t *>(zl64 + 4 * lds_labels) +
-                                              wib * lds_labels);
+  lds_ptr<uint32_t> l32 =
+      (lds_ptr<uint32_t>)(reinterpret_cast<uint32_t *>(zl64 + (p64 ? 4 * lds_labels : 0u)) +
+                          wib * lds_labels);
   const uint32_t K1 = a.K + 1;
   for (uint64_t base = (uint64_t)wave * kWave; base < a.n; base += (uint64_t)nwaves * kWave) {
     const uint32_t v = (uint32_t)base + lane;
@@ -2181,8 +2184,8 @@ static int sort_zones(Gen &g, uint32_t maxlen, uint32_t resort_div) {
     uint32_t per = std::max<uint32_t>(1024, g.sort_per_wave);
     while (per < maxlen) per *= 2;
     per = std::min(per, slice_cap);
-    const bool p32 = g.a.pool32 != nullptr;
-    const uint64_t bytes = waves * per * (p32 ? 12ull : 8ull);
+    const bool p32 = g.a.pool32 != nullptr, p64 = g.a.pool != nullptr;
+    const uint64_t bytes = waves * per * ((p64 ? 8ull : 0ull) + (p32 ? 4ull : 0ull));
     if (g.held() - g.sort_tmp.bytes - g.sort_tmp32.bytes + bytes > g.budget) {
       // no room for bigger slices: the longer lists stay unsorted
       if (!g.sort_per_wave) return HSDL_OK;
@@ -2190,17 +2193,21 @@ static int sort_zones(Gen &g, uint32_t maxlen, uint32_t resort_div) {
       HSDL_HIP(hipStreamSynchronize(g.st));
       g.sort_tmp.release();
       g.sort_tmp32.release();
-      int rc = g.take(kScSortTmp, waves * per * 8);
+      int rc = p64 ? g.take(kScSortTmp, waves * per * 8) : HSDL_OK;
       if (!rc && p32) rc = g.take(kScSortTmp32, waves * per * 4);
       if (rc) return rc;
       g.sort_per_wave = per;
     }
   }
-  // tuning knob HSDL_SORT_LDS: labels per wave's LDS slice (12 B each,
-  // four waves per workgroup, at most 64 KB; R-MAT scale 20 builds: 0 4.252
+  // tuning knob HSDL_SORT_LDS: labels per wave's LDS slice (12 B each with
+  // the 8-byte pool, 4 B with the 32-bit pool alone; four waves per
+  // workgroup, at most 64 KB; R-MAT scale 20 builds with both pools: 0 4.252
   // s, 1024 4.249 s, 1360 4.231 s)
-  const uint32_t lds_labels = env_u32("HSDL_SORT_LDS", 1360, 0, 1365);
-  hipLaunchKernelGGL(k_hop_sort, dim3(g.sort_grid), dim3(256), (size_t)lds_labels * 48, g.st,
+  const bool only32 = g.a.pool == nullptr;
+  const uint32_t lds_labels = only32 ? env_u32("HSDL_SORT_LDS", 4080, 0, 4095)
+                                     : env_u32("HSDL_SORT_LDS", 1360, 0, 1365);
+  const size_t lds_bytes = (size_t)lds_labels * 4 * (only32 ? 4 : 12);
+  hipLaunchKernelGGL(k_hop_sort, dim3(g.sort_grid), dim3(256), lds_bytes, g.st,
                      g.a, g.sort_tmp.as<unsigned long long>(),
                      g.a.pool32 ? g.sort_tmp32.as<uint32_t>() : nullptr, g.sort_per_wave,
                      resort_div, lds_labels);
