@@ -730,44 +730,57 @@ __global__ void __launch_bounds__(kGroupThreads) k_count(GenArgs a) {
 
 // One workgroup: hub_off = exclusive scan of hub_ccnt; chunk_off[b] =
 // sum_{b'<b} ceil(hub_ccnt[b'] / tile).  An overflowed batch gets no work.
+// Each wave scans a contiguous range 64 entries per step (coalesced loads,
+// a wave scan per step, the carry in a register) after the waves' totals
+// give it its start.
 __global__ void __launch_bounds__(1024) k_chunk_scan(GenArgs a) {
   with_batch(a);
-  __shared__ uint32_t part[1024];
-  __shared__ uint32_t partc[1024];
-  const uint32_t t = threadIdx.x;
-  const uint32_t per = (a.B + blockDim.x - 1) / blockDim.x;
-  const uint32_t lo = min(a.B, t * per), hi = min(a.B, lo + per);
-  uint32_t sum = 0, csum = 0;
-  for (uint32_t b = lo; b < hi; ++b) {
-    sum += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
-    csum += a.hub_ccnt[b];
+  __shared__ uint32_t wsum[1024 / kWave], wcsum[1024 / kWave];
+  const uint32_t t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const uint32_t nw = blockDim.x / kWave;
+  const uint32_t B = a.B, tile = a.tile;
+  const uint32_t per_w = ((B + nw - 1) / nw + kWave - 1) & ~(kWave - 1);
+  const uint32_t lo = min(B, w * per_w), hi = min(B, lo + per_w);
+  uint32_t s = 0, cs = 0;
+  for (uint32_t b = lo + lane; b < hi; b += kWave) {
+    const uint32_t c = a.hub_ccnt[b];
+    s += (c + tile - 1) / tile;
+    cs += c;
   }
-  part[t] = sum;
-  partc[t] = csum;
+  s = wave_sum_u32(s);
+  cs = wave_sum_u32(cs);
+  if (lane == 0) {
+    wsum[w] = s;
+    wcsum[w] = cs;
+  }
   __syncthreads();
-  for (uint32_t o = 1; o < blockDim.x; o <<= 1) {
-    const uint32_t v = (t >= o) ? part[t - o] : 0;
-    const uint32_t vc = (t >= o) ? partc[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    partc[t] += vc;
-    __syncthreads();
+  uint32_t run = 0, runc = 0;
+  for (uint32_t x = 0; x < w; ++x) {
+    run += wsum[x];
+    runc += wcsum[x];
   }
-  uint32_t run = part[t] - sum, runc = partc[t] - csum;
-  for (uint32_t b = lo; b < hi; ++b) {
-    a.chunk_off[b] = run;
-    a.hub_off[b] = runc;
-    run += (a.hub_ccnt[b] + a.tile - 1) / a.tile;
-    runc += a.hub_ccnt[b];
+  for (uint32_t b0 = lo; b0 < hi; b0 += kWave) {
+    const uint32_t b = b0 + lane;
+    const uint32_t c = b < hi ? a.hub_ccnt[b] : 0u;
+    const uint32_t k = (c + tile - 1) / tile;
+    const uint32_t ik = wave_incl_scan_u32(k), ic = wave_incl_scan_u32(c);
+    if (b < hi) {
+      a.chunk_off[b] = run + ik - k;
+      a.hub_off[b] = runc + ic - c;
+    }
+    run += (uint32_t)__shfl((int)ik, kWave - 1, kWave);
+    runc += (uint32_t)__shfl((int)ic, kWave - 1, kWave);
   }
   if (t < 8) a.ctrl->row_ctr[t] = 0;
   if (t == blockDim.x - 1) {
+    // the last wave's carry ends at the totals (its range, possibly empty,
+    // is the last)
     const bool dead = a.ctrl->overflow != 0;
     const bool rows = !dead && rows_chosen(a);
-    a.chunk_off[a.B] = part[t];
-    a.hub_off[a.B] = partc[t];
+    a.chunk_off[B] = run;
+    a.hub_off[B] = runc;
     a.ctrl->row_mode = rows ? 1u : 0u;
-    a.ctrl->total_chunks = (dead || rows) ? 0u : part[t];
+    a.ctrl->total_chunks = (dead || rows) ? 0u : run;
     a.ctrl->work_ctr = 0;
     a.ctrl->cand_total += a.ctrl->cand_round;
     a.ctrl->item_total += min(a.ctrl->fcnt[a.cur], a.fcap);
