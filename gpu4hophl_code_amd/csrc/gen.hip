@@ -186,6 +186,7 @@ struct GenArgs {
   uint32_t *chunk_off;
   Item *F0;
   Item *F1;
+  uint32_t *frank;  // per item of the next frontier: its rank among its vertex's new labels
   HeavyItem *heavy;
   uint32_t heavy_cap;
   uint32_t *gtile;
@@ -1166,13 +1167,17 @@ constexpr uint32_t kAccSlots = 256;
 // for the first time this round gathered in LDS, one reservation each on the
 // frontier and the touched list, then the writes.  Empties the staging.
 __device__ inline void publish_accepted(const GenArgs &a, Item *Fn, Item *sh_acc,
-                                        uint32_t *sh_touch, uint32_t &sh_acc_n,
+                                        uint32_t *sh_rank, uint32_t *sh_touch, uint32_t &sh_acc_n,
                                         uint32_t &sh_acc_base, uint32_t &sh_touch_n,
                                         uint32_t &sh_touch_base) {
   const uint32_t nacc = sh_acc_n;
   for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x) {
     const uint32_t y = sh_acc[i].y;
-    if (atomicAdd(&a.add_cnt[y], 1u) == 0) sh_touch[atomicAdd(&sh_touch_n, 1u)] = y;
+    // the count before this label is its rank among y's new labels (the
+    // append's slot past y's list)
+    const uint32_t r = atomicAdd(&a.add_cnt[y], 1u);
+    if (r == 0) sh_touch[atomicAdd(&sh_touch_n, 1u)] = y;
+    sh_rank[i] = r;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1184,7 +1189,10 @@ __device__ inline void publish_accepted(const GenArgs &a, Item *Fn, Item *sh_acc
   }
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < nacc; i += blockDim.x)
-    if (sh_acc_base + i < a.fcap) Fn[sh_acc_base + i] = sh_acc[i];
+    if (sh_acc_base + i < a.fcap) {
+      Fn[sh_acc_base + i] = sh_acc[i];
+      a.frank[sh_acc_base + i] = sh_rank[i];
+    }
   const uint32_t ntouch = sh_touch_n;
   for (uint32_t i = threadIdx.x; i < ntouch; i += blockDim.x)
     a.touched[sh_touch_base + i] = sh_touch[i];
@@ -1292,6 +1300,7 @@ k_prune(GenArgs a) {
   __shared__ uint32_t sh_w, sh_acc_n, sh_acc_base, sh_next, sh_touch_n, sh_touch_base;
   __shared__ uint32_t sh_item, sh_item_end, sh_rows;
   __shared__ Item sh_acc[kAccSlots];
+  __shared__ uint32_t sh_rank[kAccSlots];
   __shared__ uint32_t sh_touch[kAccSlots];
   const int lane = lane_id();
   const uint32_t K1 = a.K + 1;
@@ -1349,7 +1358,7 @@ k_prune(GenArgs a) {
     // the staged labels of earlier items are published once this item's
     // could overflow the staging slots (c_end - c_beg <= tile <= kAccSlots)
     if (sh_acc_n + (c_end - c_beg) > kAccSlots)
-      publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
+      publish_accepted(a, Fn, sh_acc, sh_rank, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
     const uint32_t *cands = a.cgroup + a.hub_off[b];
     const ListView sv = tile_view(a, s);
     const uint32_t cap = tile_cap_for(sv.len);
@@ -1395,7 +1404,7 @@ k_prune(GenArgs a) {
     }
     __syncthreads();
   }
-  publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
+  publish_accepted(a, Fn, sh_acc, sh_rank, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
   if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
   if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
 }
@@ -1421,6 +1430,7 @@ k_prune_rows(GenArgs a) {
   __shared__ uint32_t sh_b, sh_k, sh_kn, sh_q, sh_next, sh_cnt, sh_rows;
   __shared__ uint32_t sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base;
   __shared__ Item sh_acc[kAccSlots];
+  __shared__ uint32_t sh_rank[kAccSlots];
   __shared__ uint32_t sh_touch[kAccSlots];
   __shared__ uint32_t sh_cand[kPruneThreads];
   if (!a.ctrl->row_mode) return;
@@ -1528,7 +1538,7 @@ k_prune_rows(GenArgs a) {
         const uint32_t c1 = min(cnt, c0 + a.tile);
         // staged labels are published once this chunk's could overflow them
         if (sh_acc_n + (c1 - c0) > kAccSlots)
-          publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n,
+          publish_accepted(a, Fn, sh_acc, sh_rank, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n,
                            sh_touch_base);
         if (threadIdx.x == 0) sh_next = 0;
         __syncthreads();
@@ -1548,7 +1558,7 @@ k_prune_rows(GenArgs a) {
       }
     }
   }
-  publish_accepted(a, Fn, sh_acc, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
+  publish_accepted(a, Fn, sh_acc, sh_rank, sh_touch, sh_acc_n, sh_acc_base, sh_touch_n, sh_touch_base);
   if (lane == 0 && wave_reads) atomicAdd(&a.ctrl->label_reads, wave_reads);
   if (threadIdx.x == 0 && tile_reads) atomicAdd(&a.ctrl->tile_reads, tile_reads);
   if (threadIdx.x == 0 && row_slots) atomicAdd(&a.ctrl->row_slots, row_slots);
@@ -1612,7 +1622,7 @@ __global__ void __launch_bounds__(256) k_append(GenArgs a) {
   const uint32_t cnt = min(a.ctrl->fcnt[a.cur ^ 1], a.fcap);
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
     const Item it = Fn[i];
-    const uint32_t p = a.seg_len[it.y] + atomicAdd(&a.add_cur[it.y], 1u);
+    const uint32_t p = a.seg_len[it.y] + a.frank[i];
     if (p < a.seg_cap[it.y]) {
       const unsigned long long at = a.seg_start[it.y] + p;
       const uint32_t hub = hub_of(a, it.b);
@@ -1918,7 +1928,7 @@ enum ScratchId : int {
   kScSegStart, kScSegLen, kScSegCap, kScAddCnt, kScAddCur, kScTouched, kScFgrp, kScHubSlot,
   kScSortedLen, kScHcnt, kScSortTmp, kScSortTmp32, kScCflat, kScCgroup, kScHubCcnt, kScHubOff,
   kScHubCur, kScChunkOff, kScF0, kScF1, kScGtile, kScCtrl, kScHeavy, kScExpKeys, kScExpSorted,
-  kScExpTmp, kScCount
+  kScExpTmp, kScFrank, kScCount
 };
 static_assert(kScCount <= hsdl_graph::kScratch, "scratch slots");
 
@@ -1931,6 +1941,7 @@ struct Gen {
   int sort_grid = 0;
   DevBuf cflat, cgroup, hub_ccnt, hub_off, hub_cur, chunk_off, F0, F1, gtile, ctrl, heavy;
   DevBuf exp_keys, exp_sorted, exp_tmp;  // export: 32-bit keys, sorted keys, sort space
+  DevBuf frank;                          // ranks of the next frontier's items
   uint64_t budget = 0;  // device bytes the build may hold
   uint64_t fixed = 0;   // per-vertex state and the dense slots
   int grid = 0;
@@ -1954,13 +1965,14 @@ struct Gen {
   double h_grow = 0, h_enqueue = 0, h_wait = 0;  // host-side phase timers (verbose)
   uint64_t held() const {
     return fixed + pool.bytes + pool32.bytes + cflat.bytes + cgroup.bytes + F0.bytes + F1.bytes +
+           frank.bytes +
            gtile.bytes + sort_tmp.bytes + sort_tmp32.bytes;
   }
   DevBuf *by_id(int id) {
     DevBuf *t[kScCount] = {&seg_start, &seg_len, &seg_cap, &add_cnt, &add_cur, &touched, &fgrp,
                            &hub_slot, &sorted_len, &hcnt, &sort_tmp, &sort_tmp32, &cflat, &cgroup,
                            &hub_ccnt, &hub_off, &hub_cur, &chunk_off, &F0, &F1, &gtile, &ctrl,
-                           &heavy, &exp_keys, &exp_sorted, &exp_tmp};
+                           &heavy, &exp_keys, &exp_sorted, &exp_tmp, &frank};
     return t[id];
   }
   // Buffer `id` of at least n bytes: the graph's kept one when it is large
@@ -2069,13 +2081,16 @@ static int alloc_lists(Gen &g, uint32_t ccap, uint32_t fcap) {
   if (fcap != g.a.fcap) {
     g.F0.release();
     g.F1.release();
-    if (g.held() + (uint64_t)fcap * 2 * sizeof(Item) > g.budget)
+    g.frank.release();
+    if (g.held() + (uint64_t)fcap * (2 * sizeof(Item) + 4) > g.budget)
       return fail(HSDL_ENOMEM, "frontier lists exceed the HBM budget");
     if ((rc = g.take(kScF0, (uint64_t)fcap * sizeof(Item))) ||
-        (rc = g.take(kScF1, (uint64_t)fcap * sizeof(Item))))
+        (rc = g.take(kScF1, (uint64_t)fcap * sizeof(Item))) ||
+        (rc = g.take(kScFrank, (uint64_t)fcap * 4)))
       return rc;
     g.a.F0 = g.F0.as<Item>();
     g.a.F1 = g.F1.as<Item>();
+    g.a.frank = g.frank.as<uint32_t>();
     g.a.fcap = fcap;
   }
   return HSDL_OK;
@@ -2132,7 +2147,7 @@ static bool same_buffers(const GenArgs &x, const GenArgs &y) {
   return x.pool == y.pool && x.pool_cap == y.pool_cap && x.pool32 == y.pool32 &&
          x.gtile == y.gtile && x.gtile_cap == y.gtile_cap && x.cflat == y.cflat &&
          x.cgroup == y.cgroup && x.ccap == y.ccap && x.F0 == y.F0 && x.F1 == y.F1 &&
-         x.fcap == y.fcap;
+         x.frank == y.frank && x.fcap == y.fcap;
 }
 
 // Captures the batch sequence (with or without the row-item prune) into a
