@@ -165,7 +165,6 @@ struct GenArgs {
   uint32_t *hcnt;
   uint32_t hcnt_stride;  // K - 1
   uint32_t *add_cnt;
-  uint32_t *add_cur;
   uint32_t *touched;
   FGroup *fgrp;               // frontier groups (n entries)
   uint32_t *hub_slot;         // group builds: hub slot of each hub of the batch
@@ -330,7 +329,6 @@ __global__ void k_init_labels(GenArgs a, uint32_t c0) {
     a.seg_len[v] = 1;
     a.seg_cap[v] = c0;
     a.add_cnt[v] = 0;
-    a.add_cur[v] = 0;
     if (a.pool) a.pool[v * c0] = pack_label((uint32_t)v, 0, 0);
     if (a.pool32) a.pool32[v * c0] = pack32(a, (uint32_t)v, 0, 0);
     if (a.hcnt) {
@@ -1653,7 +1651,6 @@ __global__ void __launch_bounds__(256) k_finalize(GenArgs a) {
       a.fgrp[t] = FGroup{y, nl - len, a.seg_start[y] + len};
     }
     a.add_cnt[y] = 0;
-    a.add_cur[y] = 0;
   }
   if (mx) atomicMax(&a.ctrl->maxlen, mx);
   if (blockIdx.x == 0 && threadIdx.x == 0) a.ctrl->gcnt = keep ? min(cnt, a.n) : 0u;
@@ -1678,7 +1675,6 @@ __global__ void __launch_bounds__(256) k_rollback(GenArgs a) {
     }
     a.seg_len[v] = len;
     a.add_cnt[v] = 0;
-    a.add_cur[v] = 0;
   }
 }
 
@@ -1925,7 +1921,7 @@ static int upload_graph(hsdl_graph *g, int dev) {
 
 // Slots of hsdl_graph::scratch for Gen's buffers.
 enum ScratchId : int {
-  kScSegStart, kScSegLen, kScSegCap, kScAddCnt, kScAddCur, kScTouched, kScFgrp, kScHubSlot,
+  kScSegStart, kScSegLen, kScSegCap, kScAddCnt, kScTouched, kScFgrp, kScHubSlot,
   kScSortedLen, kScHcnt, kScSortTmp, kScSortTmp32, kScCflat, kScCgroup, kScHubCcnt, kScHubOff,
   kScHubCur, kScChunkOff, kScF0, kScF1, kScGtile, kScCtrl, kScHeavy, kScExpKeys, kScExpSorted,
   kScExpTmp, kScFrank, kScCount
@@ -1935,7 +1931,7 @@ static_assert(kScCount <= hsdl_graph::kScratch, "scratch slots");
 struct Gen {
   GenArgs a{};
   hsdl_graph *mg = nullptr;  // the graph whose scratch the buffers come from and go back to
-  DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, add_cur, touched, fgrp, hub_slot;
+  DevBuf pool, pool32, seg_start, seg_len, seg_cap, add_cnt, touched, fgrp, hub_slot;
   DevBuf sorted_len, hcnt, sort_tmp, sort_tmp32;  // hop zones
   uint32_t sort_per_wave = 0;                     // labels per wave slice of sort_tmp
   int sort_grid = 0;
@@ -1969,7 +1965,7 @@ struct Gen {
            gtile.bytes + sort_tmp.bytes + sort_tmp32.bytes;
   }
   DevBuf *by_id(int id) {
-    DevBuf *t[kScCount] = {&seg_start, &seg_len, &seg_cap, &add_cnt, &add_cur, &touched, &fgrp,
+    DevBuf *t[kScCount] = {&seg_start, &seg_len, &seg_cap, &add_cnt, &touched, &fgrp,
                            &hub_slot, &sorted_len, &hcnt, &sort_tmp, &sort_tmp32, &cflat, &cgroup,
                            &hub_ccnt, &hub_off, &hub_cur, &chunk_off, &F0, &F1, &gtile, &ctrl,
                            &heavy, &exp_keys, &exp_sorted, &exp_tmp, &frank};
@@ -2325,9 +2321,9 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
       "HSDL_LIST_INIT",
       (uint32_t)std::min<uint64_t>(std::max<uint64_t>((uint64_t)n * 8, 8u << 20), 1u << 30), 64,
       1u << 30);
-  // per vertex: segment start, length, capacity, add counters, touched
+  // per vertex: segment start, length, capacity, add counter, touched
   // entry, frontier group (16 B), hub slot, hop zone counts
-  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 5 + 16 + 4 + 4 * (uint64_t)K);
+  const uint64_t vert_bytes = (uint64_t)n * (8 + 4 * 4 + 16 + 4 + 4 * (uint64_t)K);
   // dense dedupe slots: at most 1/slot_div of the budget (tuning knob
   // HSDL_SLOT_DIV), <= 65536 hubs
   const bool plain = opts->mode == HSDL_MODE_PLAIN;
@@ -2410,7 +2406,7 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   if ((!use32 && !g.pool.p && (rc = g.pool.alloc(pool0 * 8))) ||
       (rc = g.take(kScSegStart, n * 8ull)) ||
       (rc = g.take(kScSegLen, n * 4ull)) || (rc = g.take(kScSegCap, n * 4ull)) ||
-      (rc = g.take(kScAddCnt, n * 4ull)) || (rc = g.take(kScAddCur, n * 4ull)) ||
+      (rc = g.take(kScAddCnt, n * 4ull)) ||
       (rc = g.take(kScTouched, n * 4ull)) || (rc = g.take(kScFgrp, n * 16ull)) ||
       (group && (rc = g.take(kScHubSlot, n * 4ull))) ||
       (rc = g.take(kScHubCcnt, (Bmax + 1ull) * 4)) || (rc = g.take(kScHubOff, (Bmax + 1ull) * 4)) ||
@@ -2516,7 +2512,6 @@ int generate(const hsdl_graph *hg, uint8_t K, const hsdl_opts *opts, const uint3
   a.hcnt = zones ? g.hcnt.as<uint32_t>() : nullptr;
   a.hcnt_stride = K - 1;
   a.add_cnt = g.add_cnt.as<uint32_t>();
-  a.add_cur = g.add_cur.as<uint32_t>();
   a.touched = g.touched.as<uint32_t>();
   a.fgrp = g.fgrp.as<FGroup>();
   a.hub_slot = group ? g.hub_slot.as<uint32_t>() : nullptr;
