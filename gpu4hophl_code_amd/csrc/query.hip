@@ -367,8 +367,12 @@ static int ensure_query32(const hsdl_index *idx, hipStream_t st) {
     idx->q32.store(2, std::memory_order_release);
     return HSDL_OK;
   }
-  int rc = idx->lab32.alloc(std::max<uint64_t>(idx->total, 1) * 4);
-  if (rc) return rc;
+  // the copy is an optimisation: without room for it, queries read the
+  // 8-byte labels
+  if (idx->lab32.alloc(std::max<uint64_t>(idx->total, 1) * 4) != HSDL_OK) {
+    idx->q32.store(2, std::memory_order_release);
+    return HSDL_OK;
+  }
   const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (idx->total + 255) / 256 + 1);
   hipLaunchKernelGGL(k_pack_query32, dim3(grid), dim3(256), 0, st,
                      idx->lab.as<unsigned long long>(), idx->total, idx->lab32.as<uint32_t>(),
